@@ -1,0 +1,404 @@
+"""GPU HMC sampler with the reference's ``HMC(seed).sample(...)`` interface.
+
+Mirrors ``hmclab.Samplers.HMC`` (hmclab/Samplers.py:1105-1755) and the
+proposal loop of ``_AbstractSampler`` (Samplers.py:321-704): argument checks
+and messages, the samples-file layout and attributes, thinning, autotuning,
+``max_time`` and ``KeyboardInterrupt`` handling.  The trajectory, the
+Hamiltonians and the Metropolis test run on the GPU for a whole batch of
+``chains`` at once (the reference runs one chain per process,
+Samplers.py:1816-1977).
+
+Random numbers:
+* ``rng="host"`` (default for one chain) draws, per chain and per proposal,
+  ``normal(size=(d,1))`` (scaled by sqrt(M) for a diagonal mass), then
+  ``uniform(0.5,1.5)`` when ``randomize_stepsize``, then ``uniform(0,1)`` from
+  ``numpy.random.default_rng(seed)`` — the reference's draw order
+  (MassMatrices.py:142,227; Samplers.py:1533-1534,1486) — so chain c
+  reproduces a reference ``HMC(seed=seeds[c])`` chain up to floating-point
+  rounding.  Seeds default to ``seed + c``.
+* ``rng="device"`` (default for several chains) draws the same quantities on
+  the GPU with a Philox4x32-10 counter RNG keyed by ``seed``; whole chunks of
+  proposals then run without host round trips.
+"""
+from __future__ import annotations
+
+import math
+import time
+import warnings
+from datetime import datetime
+
+import numpy as np
+
+from . import _lib
+from .Distributions import _AbstractDistribution, build_problem
+from .engine import Chains
+from .MassMatrices import Unit, _AbstractMassMatrix
+from .Samples import Samples, SamplesWriter
+
+
+class HMC:
+    """Hamiltonian Monte Carlo on the GPU (Samplers.py:1105-1178)."""
+
+    name = "Hamiltonian Monte Carlo"
+    stepsize: float = 0.1
+    amount_of_steps: int = 10
+    minimal_stepsize: float = 1e-18
+    integrators_full_names = {
+        "lf": "leapfrog integrator",
+        "3s": "three stage integrator",
+        "4s": "four stage integrator",
+    }
+    available_integrators = integrators_full_names.keys()
+
+    def __init__(self, seed: int = None):
+        self.seed = seed
+        self.rng = np.random.default_rng() if seed is None else np.random.default_rng(seed)
+        self.times_started = 0
+        self.diagnostics = None
+
+    # ------------------------------------------------------------------ #
+    def sample(self, samples_filename, distribution, stepsize: float = 0.1,
+               randomize_stepsize: bool = True, amount_of_steps: int = 10, mass_matrix=None,
+               integrator: str = "lf", initial_model=None, proposals: int = 100,
+               online_thinning: int = 1, diagnostic_mode: bool = False,
+               overwrite_existing_file: bool = False, max_time: float = None,
+               autotuning: bool = False, target_acceptance_rate: float = 0.65,
+               learning_rate: float = 0.75, queue=None, disable_progressbar=False, *,
+               chains: int = 1, precision: str = "f64", engine: str = "auto", rng: str = None,
+               seeds=None, chunk: int = None):
+        """Sample ``distribution``; see the module docstring for the extras.
+
+        ``samples_filename``: a ``.npy`` path (one chain), a list of paths (one
+        per chain), a path used as a pattern ``<stem>_<c>.npy`` for several
+        chains, or None to keep the samples in memory (``self.results``,
+        shape (chains, d+1, N)).
+        """
+        self._init(samples_filename, distribution, stepsize, randomize_stepsize, amount_of_steps,
+                   mass_matrix, integrator, initial_model, proposals, online_thinning,
+                   diagnostic_mode, overwrite_existing_file, max_time, autotuning,
+                   target_acceptance_rate, learning_rate, disable_progressbar, chains, precision,
+                   engine, rng, seeds, chunk)
+        try:
+            self._loop()
+        finally:
+            self._close()
+        return self
+
+    # ------------------------------------------------------------------ #
+    def _init(self, samples_filename, distribution, stepsize, randomize_stepsize, amount_of_steps,
+              mass_matrix, integrator, initial_model, proposals, online_thinning, diagnostic_mode,
+              overwrite_existing_file, max_time, autotuning, target_acceptance_rate, learning_rate,
+              disable_progressbar, chains, precision, engine, rng, seeds, chunk):
+        # Samplers.py:347-416
+        assert isinstance(distribution, _AbstractDistribution), (
+            "The passed target distribution should be a derived class of _AbstractDistribution."
+        )
+        assert type(distribution.dimensions) == int and distribution.dimensions > 0, (
+            "The passed target distribution should have an integer dimension larger than zero."
+        )
+        self.distribution = distribution
+        self.dimensions = d = distribution.dimensions
+        assert type(proposals) == int and proposals > 0, (
+            "The amount of proposal requested (`proposals`) should be an integer number larger than zero."
+        )
+        assert type(online_thinning) == int and online_thinning > 0, (
+            "The amount of online thinning (`online_thinning`) should be an integer number larger than zero."
+        )
+        assert proposals % online_thinning == 0, (
+            "The amount of proposals (`proposals`) needs to be a multiple of the online thinning "
+            "(`online_thinning`) number, to prevent sample wastage."
+        )
+        assert type(chains) == int and chains > 0, "`chains` should be a positive integer."
+        self.proposals = proposals
+        self.online_thinning = online_thinning
+        self.proposals_after_thinning = proposals // online_thinning
+        self.chains = B = chains
+        # Samplers.py:1334-1384
+        if autotuning:
+            assert 0.5 < learning_rate <= 1.0, (
+                "The learning rate should be larger than 0.5 and smaller than or equal to 1.0, "
+                f"otherwise the Markov chain does not converge. Chosen: {learning_rate}"
+            )
+        self.stepsize = float(stepsize)
+        assert self.stepsize > 0.0, "Stepsize should be a float larger than zero."
+        assert type(amount_of_steps) == int, (
+            "The amount of steps (amount_of_steps) the HMC integrator should make should be an integer."
+        )
+        assert amount_of_steps > 0, (
+            "The amount of steps (amount_of_steps) the HMC integrator should make should be larger than zero."
+        )
+        self.amount_of_steps = amount_of_steps
+        self.randomize_stepsize = bool(randomize_stepsize)
+        self.autotuning = bool(autotuning)
+        self.target_acceptance_rate = float(target_acceptance_rate)
+        self.learning_rate = float(learning_rate)
+        if mass_matrix is None:
+            mass_matrix = Unit(d)
+        assert isinstance(mass_matrix, _AbstractMassMatrix), (
+            "The passed mass matrix (mass_matrix) should be a class derived from _AbstractMassMatrix."
+        )
+        assert mass_matrix.dimensions == d, (
+            "The passed mass matrix (mass_matrix) should have dimensions equal to the target "
+            f"distribution. Passed: {mass_matrix.dimensions}, required: {d}."
+        )
+        mass_matrix.rng = self.rng
+        self.mass_matrix = mass_matrix
+        self.integrator = str(integrator)
+        if self.integrator not in self.available_integrators:
+            raise ValueError(f"Unknown integrator used. Choices are: {self.available_integrators}")
+        if max_time is not None:
+            max_time = float(max_time)
+            assert max_time > 0.0, "The maximal runtime (`max_time`) should be a float larger than zero."
+        self.max_time = max_time
+        self.diagnostic_mode = bool(diagnostic_mode)
+        self.disable_progressbar = disable_progressbar
+        self.rng_mode = rng or ("host" if B == 1 else "device")
+        assert self.rng_mode in ("host", "device"), "rng must be 'host' or 'device'"
+
+        # initial models (Samplers.py:406-426)
+        if initial_model is None:
+            X0 = np.zeros((B, d))
+        else:
+            im = np.array(initial_model, dtype=np.float64)
+            if im.size == d:
+                X0 = np.broadcast_to(im.reshape(1, d), (B, d)).copy()
+            else:
+                assert im.shape == (d, B), (
+                    "The initial model (`initial_model`) dimension is incompatible with the target "
+                    f"distribution. Supplied model shape: {im.shape}.Required shape: {(d, 1)}"
+                )
+                X0 = np.ascontiguousarray(im.T)
+
+        # files
+        self.samples_filename = samples_filename
+        self._writers = []
+        if samples_filename is not None:
+            if isinstance(samples_filename, (list, tuple)):
+                names = list(samples_filename)
+                assert len(names) == B, "one samples filename per chain is required"
+            else:
+                assert type(samples_filename) == str, (
+                    "First argument should be a string containing the path of the file to which to "
+                    f"write samples. It was an object of type {type(samples_filename)}."
+                )
+                if B == 1:
+                    names = [samples_filename]
+                else:
+                    stem, ext = samples_filename.rsplit(".", 1) if "." in samples_filename else (samples_filename, "npy")
+                    names = [f"{stem}_{c}.{ext}" for c in range(B)]
+            self._filenames = names
+            for nme in names:
+                self._writers.append(SamplesWriter(nme, d, overwrite=bool(overwrite_existing_file)))
+        self._memory = [] if samples_filename is None else None
+
+        # device objects
+        self.problem = build_problem(distribution, precision=precision, engine=engine,
+                                     mass_diagonal=mass_matrix._device_diagonal())
+        self._chains = Chains(self.problem, B)
+        try:
+            self._chains.set_state(X0)
+        except ValueError as e:
+            self._close_files()
+            raise AssertionError(str(e)) from None
+        X, chi = self._chains.get_state()
+        self.current_model = X[0].reshape(d, 1) if B == 1 else X.T.copy()
+        self.current_x = float(chi[0]) if B == 1 else chi
+        self.accepted_proposals = 0 if B == 1 else np.zeros(B, dtype=np.int64)
+        self.amount_of_writes = 0
+        self.current_proposal = 0
+
+        # per-chain randoms for host mode
+        if self.rng_mode == "host":
+            if B == 1 and seeds is None:
+                self._rngs = [self.rng]
+            else:
+                if seeds is None:
+                    base = 0 if self.seed is None else int(self.seed)
+                    seeds = [base + c for c in range(B)]
+                assert len(seeds) == B, "one seed per chain is required"
+                self._rngs = [np.random.default_rng(s) for s in seeds]
+        self._device_seed = int(self.seed) if self.seed is not None else int(self.rng.integers(0, 2**63))
+        self._step = np.full(B, self.stepsize)
+        if self.autotuning:
+            self.acceptance_rates = np.empty((proposals, B))
+            self.stepsizes = np.empty((proposals, B))
+        self._chunk = chunk
+        if self.diagnostic_mode:
+            self._chains.enable_timing(True)
+
+        self.start_time = datetime.now()
+        self.times_started += 1
+        attrs = dict(write_index=-1, last_written_sample=-1, proposals=proposals, acceptance_rate=0,
+                     online_thinning=online_thinning,
+                     start_time=self.start_time.strftime("%d-%b-%Y (%H:%M:%S.%f)"), sampler=self.name,
+                     stepsize=self.stepsize, amount_of_steps=amount_of_steps,
+                     mass_matrix=mass_matrix.name, integrator=self.integrators_full_names[self.integrator])
+        for w in self._writers:
+            for k, v in attrs.items():
+                if k not in ("write_index", "last_written_sample"):
+                    w.write_attribute(k, v)
+
+    # ------------------------------------------------------------------ #
+    def _write(self, X, chi):
+        """Record one thinned sample per chain: X (B, d), chi (B,)."""
+        cols = np.concatenate([X, chi[:, None]], axis=1)  # (B, d+1)
+        if self._writers:
+            for c, w in enumerate(self._writers):
+                w.append(cols[c][:, None])
+        else:
+            self._memory.append(cols.copy())
+        self.amount_of_writes += 1
+
+    def _progress(self):
+        try:
+            import tqdm.auto as tq
+            return tq.tqdm(total=self.proposals, desc="Tot. acc rate: 0.00. Progress", leave=True,
+                           disable=self.disable_progressbar)
+        except Exception:   # pragma: no cover
+            return None
+
+    def _loop(self):
+        B = self.chains
+        deadline = None if self.max_time is None else time.time() + self.max_time
+        bar = self._progress()
+        done = 0
+        try:
+            if self.rng_mode == "host":
+                for i in range(self.proposals):
+                    self.current_proposal = i
+                    self._host_proposal(i)
+                    if i % self.online_thinning == 0:
+                        X, chi = self._chains.get_state()
+                        self._write(X, chi)
+                    done = i + 1
+                    if bar is not None:
+                        bar.update(1)
+                        if i == 0 or i == self.proposals - 1 or i % 50 == 0:
+                            bar.set_description(f"Tot. acc rate: {self._rate():.2f}. Progress", refresh=False)
+                    if deadline is not None and time.time() > deadline:
+                        raise TimeoutError
+            else:
+                chunk = self._chunk or max(self.online_thinning, min(self.proposals, 64))
+                chunk = max(self.online_thinning, (chunk // self.online_thinning) * self.online_thinning)
+                self._chains.set_trace(chunk // self.online_thinning + 1)
+                while done < self.proposals:
+                    n = min(chunk, self.proposals - done)
+                    self._chains.reset_trace()
+                    self._step, acc = self._chains.run(
+                        n, self.amount_of_steps, self._step, self.integrator, self.randomize_stepsize,
+                        self.autotuning, self.target_acceptance_rate, self.learning_rate,
+                        self._device_seed, done, self.online_thinning)
+                    self.accepted_proposals = self.accepted_proposals + (acc if B > 1 else int(acc[0]))
+                    cnt = self._chains.trace_count()
+                    if cnt:
+                        X, chi = self._chains.read_trace(0, cnt)
+                        for k in range(cnt):
+                            self._write(X[k], chi[k])
+                    done += n
+                    self.current_proposal = done - 1
+                    if bar is not None:
+                        bar.update(n)
+                        bar.set_description(f"Tot. acc rate: {self._rate():.2f}. Progress", refresh=False)
+                    if deadline is not None and time.time() > deadline:
+                        raise TimeoutError
+        except KeyboardInterrupt:
+            # Samplers.py:688-690: the interrupted proposal is dropped
+            self.current_proposal = max(done - 1, 0)
+        except TimeoutError:
+            pass
+        finally:
+            if bar is not None:
+                bar.close()
+
+    def _rate(self):
+        acc = np.sum(self.accepted_proposals) / self.chains
+        return acc / (self.current_proposal + 1)
+
+    def _host_proposal(self, i):
+        B, d = self.chains, self.dimensions
+        P0 = np.empty((B, d))
+        eps = np.empty(B)
+        u = np.empty(B)
+        mm = self.mass_matrix
+        for c, r in enumerate(self._rngs):
+            P0[c] = mm.generate_momentum(rng=r).reshape(-1)
+            eps[c] = r.uniform(0.5, 1.5) * self._step[c] if self.randomize_stepsize else self._step[c]
+            # the accept uniform is drawn after the trajectory in the reference; no
+            # other draw happens in between, so drawing it here consumes the same value
+            u[c] = r.uniform(0, 1)
+        res = self._chains.propose(P0, eps, u, self.amount_of_steps, self.integrator)
+        with np.errstate(over="ignore", invalid="ignore"):
+            a = np.exp(res["H0"] - res["H1"])
+        if self.autotuning:
+            for c in range(B):
+                self.acceptance_rates[i, c] = a[c]
+                self.stepsizes[i, c] = self._step[c]
+                w = (i + 1) ** (-self.learning_rate)
+                ac = 0 if np.isnan(a[c]) else a[c]
+                s = self._step[c] - w * (self.target_acceptance_rate - min(ac, 1))
+                if s <= 0:
+                    s = max(s, self.minimal_stepsize)
+                self._step[c] = s
+        acc = res["accepted"]
+        if B == 1:
+            self.accepted_proposals += int(acc[0])
+        else:
+            self.accepted_proposals += acc.astype(np.int64)
+
+    # ------------------------------------------------------------------ #
+    def _close_files(self):
+        for w in self._writers:
+            w.close()
+
+    def _close(self):
+        self.end_time = datetime.now()
+        B, d = self.chains, self.dimensions
+        X, chi = self._chains.get_state()
+        self.current_model = X[0].reshape(d, 1) if B == 1 else X.T.copy()
+        self.current_x = float(chi[0]) if B == 1 else chi
+        self.stepsize = float(self._step[0]) if B == 1 else self._step.copy()
+        if self.autotuning and self.rng_mode == "host":
+            self.acceptance_rates = self.acceptance_rates[: self.current_proposal]
+            self.stepsizes = self.stepsizes[: self.current_proposal]
+        rate = np.sum(self.accepted_proposals) / B / (self.current_proposal + 1)
+        for c, w in enumerate(self._writers):
+            acc_c = self.accepted_proposals if B == 1 else self.accepted_proposals[c]
+            w.write_attribute("acceptance_rate", acc_c / (self.current_proposal + 1))
+            w.write_attribute("end_time", self.end_time.strftime("%d-%b-%Y (%H:%M:%S.%f)"))
+            w.write_attribute("runtime", str(self.end_time - self.start_time))
+            w.write_attribute("runtime_seconds", (self.end_time - self.start_time).total_seconds())
+            if self.autotuning:
+                if self.rng_mode == "host":
+                    w.write_attribute("acceptance_rates", self.acceptance_rates[:, c:c + 1])
+                    w.write_attribute("stepsizes", self.stepsizes[:, c:c + 1])
+                else:
+                    w.write_attribute("stepsizes", np.array([[self._step[c]]]))
+        self._close_files()
+        if self._memory is not None:
+            self.results = (np.stack(self._memory, axis=2) if self._memory
+                            else np.empty((B, d + 1, 0)))
+        if self.diagnostic_mode:
+            total = (self.end_time - self.start_time).total_seconds()
+            names = ["trajectory gradient kernels", "endpoint misfit", "element-wise / Metropolis"]
+            self.diagnostics = {}
+            print("Detailed statistics:")
+            print(f"Total runtime: {total:.2f} seconds")
+            print("{:<30} {:<12} {:<10}".format("Kernel group", "GPU ms", "launches"))
+            for k, nm in enumerate(names):
+                ms, n = self._chains.timing(k)
+                self.diagnostics[nm] = (ms, n)
+                print("{:<30} {:<12.3f} {:<10d}".format(nm, ms, n))
+        self.acceptance_rate = rate
+
+    def get_diagnostics(self):
+        if not self.diagnostic_mode:
+            raise AssertionError("Can't return diagnostics if sampler is not in diagnostic mode")
+        return self.diagnostics
+
+    def load_results(self, burn_in: int = 0) -> np.ndarray:
+        """(d+1, N) array of chain 0 (or of the only chain)."""
+        assert burn_in >= 0
+        if self.samples_filename is None:
+            return self.results[0][:, burn_in:]
+        with Samples(self._filenames[0], burn_in=burn_in) as s:
+            return np.array(s.numpy)

@@ -1,0 +1,704 @@
+// Fused trajectory engine: one workgroup owns a tile of CT chains for a whole
+// proposal.  Positions live in LDS as Xs[d][CT] (parameter-major inside the
+// tile), residuals as Rs[n][CT], momenta in registers.  Each leapfrog step is
+//   forward  : Rs[i][c] = (sum_k G[i][k] Xs[k][c] - d_i) / var_i
+//   backward : g[j][c] = sum_i G[i][j] Rs[i][c] + prior'(x); kick; drift;
+//              corrector (Samplers.py:1560-1581, LinearMatrix.py:208,426)
+// with one barrier after each pass; the FP64 endpoint misfit
+// (LinearMatrix.py:193-202,405-415), kinetic energy and the Metropolis test
+// (Samplers.py:1471-1492) finish the proposal inside the same launch.
+//
+// Operators: SELL-64 (rows sorted by length, 64-row slices stored lane-
+// contiguous, one wavefront per slice) for sparse ray matrices, and a
+// k-major dense layout (element (row, k) at base[k*ld + row]) for small dense
+// G.  The forward pass uses the operator of G, the backward pass the
+// operator of G^T; thread (wave w, lane r) owns the parameters of the G^T
+// slices w, w+NW, ... for the whole proposal.
+#include "hmc_internal.hpp"
+
+namespace hmc {
+
+constexpr int FU_NT = 512;
+constexpr int FU_NW = FU_NT / 64;
+
+struct DevSell {
+  const int32_t* base;
+  const int32_t* width;
+  const int32_t* perm;
+  const int32_t* col;
+  int64_t nslices;
+};
+
+struct DevDense {
+  int64_t ld;
+  int64_t nrows, ncols;
+};
+
+template <int TP>
+struct TypeOf;
+template <>
+struct TypeOf<HMC_F64> { using T = double; };
+template <>
+struct TypeOf<HMC_F32> { using T = float; };
+
+// out-row callback receives (slice, row (-1 padding), acc[CT])
+template <typename Acc, typename V, typename Tin, int CT, class F>
+__device__ inline void sell_pass(const DevSell& S, const V* __restrict__ val,
+                                 const Tin* __restrict__ in, int w, int lane, F&& f) {
+  for (int64_t s = w; s < S.nslices; s += FU_NW) {
+    const int32_t base = S.base[s], width = S.width[s];
+    const int32_t row = S.perm[s * 64 + lane];
+    Acc acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = Acc(0);
+    for (int32_t k = 0; k < width; ++k) {
+      const int64_t idx = (int64_t)base + (int64_t)k * 64 + lane;
+      const int32_t col = S.col[idx];
+      const Acc v = (Acc)val[idx];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[c] += v * (Acc)in[(int64_t)col * CT + c];
+    }
+    f(s, (int64_t)row, acc);
+  }
+}
+
+template <typename Acc, typename V, typename Tin, int CT, class F>
+__device__ inline void dense_pass(const DevDense& D, const V* __restrict__ M,
+                                  const Tin* __restrict__ in, int w, int lane, F&& f) {
+  const int64_t nsl = (D.nrows + 63) / 64;
+  for (int64_t s = w; s < nsl; s += FU_NW) {
+    const int64_t row = s * 64 + lane;
+    Acc acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = Acc(0);
+    if (row < D.nrows) {
+      const V* mp = M + row;
+      for (int64_t k = 0; k < D.ncols; ++k) {
+        const Acc v = (Acc)mp[k * D.ld];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[c] += v * (Acc)in[k * CT + c];
+      }
+    }
+    f(s, row < D.nrows ? row : (int64_t)-1, acc);
+  }
+}
+
+// Operator bundle: forward = G (rows = data), backward = G^T (rows = params).
+template <typename T, typename VT, bool SPARSE>
+struct Ops {
+  // sparse
+  DevSell fwd_s, bwd_s;
+  // dense (k-major views): forward base = Gt (ld = ldgt), backward base = G (ld = ldg)
+  DevDense fwd_d, bwd_d;
+  const T* fwd_val;     // trajectory precision values
+  const T* bwd_val;
+  const VT* fwd_tgt;    // target precision values of the forward operator
+  int64_t n, d;
+
+  template <typename Acc, typename V, typename Tin, int CT, class F>
+  __device__ void forward(const V* val, const Tin* in, int w, int lane, F&& f) const {
+    if constexpr (SPARSE) sell_pass<Acc, V, Tin, CT>(fwd_s, val, in, w, lane, f);
+    else dense_pass<Acc, V, Tin, CT>(fwd_d, val, in, w, lane, f);
+  }
+  template <typename Acc, typename V, typename Tin, int CT, class F>
+  __device__ void backward(const V* val, const Tin* in, int w, int lane, F&& f) const {
+    if constexpr (SPARSE) sell_pass<Acc, V, Tin, CT>(bwd_s, val, in, w, lane, f);
+    else dense_pass<Acc, V, Tin, CT>(bwd_d, val, in, w, lane, f);
+  }
+  // owned parameter of (wave slice s, lane)
+  __device__ int64_t param_of(int64_t s, int lane) const {
+    if constexpr (SPARSE) return bwd_s.perm[s * 64 + lane];
+    else {
+      const int64_t j = s * 64 + lane;
+      return j < d ? j : -1;
+    }
+  }
+  __device__ int64_t param_slices() const {
+    if constexpr (SPARSE) return bwd_s.nslices;
+    else return (d + 63) / 64;
+  }
+};
+
+struct FusedArgs {
+  void* x0;            // [B][ldx] state dtype
+  double* chi0;
+  void* x_out;         // trajectory end positions or nullptr
+  void* p_out;         // trajectory end momenta or nullptr
+  const double* p_host;// host-supplied momenta [B][ldx] or nullptr
+  double* eps;
+  double* u;
+  double* step;
+  double* h0;
+  double* h1;
+  int32_t* acc;
+  int64_t* acc_count;
+  void* tr_x;
+  double* tr_chi;
+  int64_t slot;
+  int64_t ldx, B;
+  const double* dobs;
+  const double* var;
+  double mconst;
+  uint64_t seed;
+  int64_t prop;
+  int device_scalars, randomize, autotune;
+  double target, lr;
+  // schedule (units of the local step): L steps of P stages; every step but
+  // the last uses pat, the last uses pat_last; each stage = (kick, drift, drift2)
+  double first_drift;
+  int L, P;
+  double pat[4][3];
+  double pat_last[4][3];
+};
+
+template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
+__global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, FusedArgs a,
+                                                         DevModel md) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* Xs = reinterpret_cast<T*>(smem_raw);                         // [d][CT]
+  T* Rs = Xs + round_up(op.d, 2) * CT;                            // [n][CT]
+  __shared__ double scratch[FU_NW + 1];
+  __shared__ double s_k0[CT], s_eps[CT], s_chi1[CT], s_k1[CT];
+  __shared__ int s_ok[CT];
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * CT;
+  const int64_t nps = op.param_slices();
+  T* x0 = reinterpret_cast<T*>(a.x0);
+
+  // ---- per-chain scalars
+  if (tid < CT) {
+    const int64_t c = c0 + tid;
+    double e = 0.0;
+    if (c < a.B) {
+      if (a.device_scalars) {
+        double ue, ua;
+        philox_scalars(a.seed, (uint32_t)c, (uint64_t)a.prop, ue, ua);
+        e = a.randomize ? ue * a.step[c] : a.step[c];
+        a.eps[c] = e;
+        a.u[c] = ua;
+      } else {
+        e = a.eps[c];
+      }
+    }
+    s_eps[tid] = e;
+  }
+  // ---- load positions of the tile
+  for (int64_t j = tid; j < op.d; j += FU_NT)
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+      Xs[j * CT + c] = (c0 + c < a.B) ? x0[(c0 + c) * a.ldx + j] : T(0);
+  __syncthreads();
+
+  // ---- momentum draw (owned parameters), K0, first drift
+  T preg[KMAX][CT];
+  double kloc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) kloc[c] = 0.0;
+#pragma unroll
+  for (int ks = 0; ks < KMAX; ++ks) {
+    const int64_t s = w + (int64_t)ks * FU_NW;
+    const int64_t j = (s < nps) ? op.param_of(s, lane) : -1;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      T pv = T(0);
+      if (j >= 0 && c0 + c < a.B) {
+        if (a.p_host) {
+          pv = (T)a.p_host[(c0 + c) * a.ldx + j];
+        } else {
+          double z0, z1;
+          philox_normal2(a.seed, (uint32_t)(c0 + c), (uint64_t)a.prop, (uint32_t)(j >> 1), z0, z1);
+          const double z = (j & 1) ? z1 : z0;
+          pv = (T)(md.msqrt ? md.msqrt[j] * z : z);
+        }
+        kloc[c] += kterm(md, j, (double)pv);
+        T xv = Xs[j * CT + c];
+        xv += (T)(a.first_drift * s_eps[c]) * kgrad<T>(md, j, pv);
+        corrector<T>(md, j, xv, pv);
+        Xs[j * CT + c] = xv;
+      }
+      preg[ks][c] = pv;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const double s = block_sum<FU_NT>(kloc[c], scratch);
+    if (tid == 0) s_k0[c] = 0.5 * s;
+  }
+  __syncthreads();
+
+  // ---- stages: gradient, kick, drift (+ drift2)
+  for (int st = 0; st < a.L * a.P; ++st) {
+    const int step = st / a.P, k = st - step * a.P;
+    const double* sc = (step + 1 < a.L) ? a.pat[k] : a.pat_last[k];
+    const double kick = sc[0], drift = sc[1], drift2 = sc[2];
+    op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, lane, [&](int64_t, int64_t i, T* acc) {
+      if (i >= 0) {
+        const T di = (T)a.dobs[i], vi = (T)a.var[i];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - di) / vi;
+      }
+    });
+    __syncthreads();
+    // backward over owned slices (unrolled so momenta stay in registers)
+#pragma unroll
+    for (int ks = 0; ks < KMAX; ++ks) {
+      const int64_t s = w + (int64_t)ks * FU_NW;
+      if (s < nps) {
+        T g[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) g[c] = T(0);
+        if constexpr (SPARSE) {
+          const int32_t base = op.bwd_s.base[s], width = op.bwd_s.width[s];
+          for (int32_t k = 0; k < width; ++k) {
+            const int64_t idx = (int64_t)base + (int64_t)k * 64 + lane;
+            const int32_t col = op.bwd_s.col[idx];
+            const T v = op.bwd_val[idx];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) g[c] += v * Rs[(int64_t)col * CT + c];
+          }
+        } else {
+          const int64_t row = s * 64 + lane;
+          if (row < op.d) {
+            const T* mp = op.bwd_val + row;
+            for (int64_t k = 0; k < op.n; ++k) {
+              const T v = mp[k * op.bwd_d.ld];
+#pragma unroll
+              for (int c = 0; c < CT; ++c) g[c] += v * Rs[k * CT + c];
+            }
+          }
+        }
+        const int64_t j = op.param_of(s, lane);
+        if (j >= 0) {
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const double e = s_eps[c];
+            T xv = Xs[j * CT + c], pv = preg[ks][c];
+            T gg = g[c] + prior_grad<T>(md, j, xv);
+            if (out_of_bounds(md, j, (double)xv)) gg = (T)INFINITY;
+            pv -= (T)(kick * e) * gg;
+            xv += (T)(drift * e) * kgrad<T>(md, j, pv);
+            corrector<T>(md, j, xv, pv);
+            if (drift2 != 0.0) {
+              xv += (T)(drift2 * e) * kgrad<T>(md, j, pv);
+              corrector<T>(md, j, xv, pv);
+            }
+            Xs[j * CT + c] = xv;
+            preg[ks][c] = pv;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- endpoint: data misfit in FP64
+  double dm[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) dm[c] = 0.0;
+  if (op.n > 0) {
+    op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, [&](int64_t, int64_t i, double* acc) {
+      if (i >= 0) {
+        const double di = a.dobs[i], vi = a.var[i];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          const double rr = acc[c] - di;
+          dm[c] += rr * rr / vi;
+        }
+      }
+    });
+  }
+  // prior misfit, bounds, K1 over owned parameters
+  double pm[CT], kk[CT], ob[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) pm[c] = kk[c] = ob[c] = 0.0;
+#pragma unroll
+  for (int ks = 0; ks < KMAX; ++ks) {
+    const int64_t s = w + (int64_t)ks * FU_NW;
+    const int64_t j = (s < nps) ? op.param_of(s, lane) : -1;
+    if (j >= 0) {
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        const double xv = (double)Xs[j * CT + c];
+        pm[c] += prior_misfit(md, j, xv);
+        if (out_of_bounds(md, j, xv)) ob[c] = 1.0;
+        kk[c] += kterm(md, j, (double)preg[ks][c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const double sd = block_sum<FU_NT>(dm[c], scratch);
+    const double sp = block_sum<FU_NT>(pm[c], scratch);
+    const double sb = block_sum<FU_NT>(ob[c], scratch);
+    const double sk = block_sum<FU_NT>(kk[c], scratch);
+    if (tid == 0) {
+      double chi = 0.5 * sd + (sp + a.mconst);
+      if (sb > 0.0) chi += INFINITY;
+      s_chi1[c] = chi;
+      s_k1[c] = 0.5 * sk;
+    }
+  }
+  __syncthreads();
+  if (tid < CT) {
+    const int c = tid;
+    const int64_t ch = c0 + c;
+    int ok = 0;
+    if (ch < a.B) {
+      const double H0 = a.chi0[ch] + s_k0[c];
+      const double H1 = s_chi1[c] + s_k1[c];
+      const double av = exp(H0 - H1);
+      if (a.autotune) {
+        const double wgt = pow((double)(a.prop + 1), -a.lr);
+        const double ar = isnan(av) ? 0.0 : av;
+        double sv = a.step[ch] - wgt * (a.target - fmin(ar, 1.0));
+        if (sv <= 0.0) sv = fmax(sv, 1e-18);
+        a.step[ch] = sv;
+      }
+      ok = (av > a.u[ch]) ? 1 : 0;
+      if (a.h0) a.h0[ch] = H0;
+      if (a.h1) a.h1[ch] = H1;
+      if (a.acc) a.acc[ch] = ok;
+      if (a.acc_count) a.acc_count[ch] += ok;
+      if (ok) a.chi0[ch] = s_chi1[c];
+      if (a.slot >= 0) a.tr_chi[a.slot * a.B + ch] = a.chi0[ch];
+    }
+    s_ok[c] = ok;
+  }
+  __syncthreads();
+  // ---- write back accepted positions, trace, optional trajectory outputs
+  T* xo = reinterpret_cast<T*>(a.x_out);
+  T* po = reinterpret_cast<T*>(a.p_out);
+  T* tr = reinterpret_cast<T*>(a.tr_x);
+#pragma unroll
+  for (int ks = 0; ks < KMAX; ++ks) {
+    const int64_t s = w + (int64_t)ks * FU_NW;
+    const int64_t j = (s < nps) ? op.param_of(s, lane) : -1;
+    if (j >= 0) {
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        const int64_t ch = c0 + c;
+        if (ch < a.B) {
+          const int64_t o = ch * a.ldx + j;
+          const T xv = Xs[j * CT + c];
+          if (xo) xo[o] = xv;
+          if (po) po[o] = preg[ks][c];
+          if (s_ok[c]) x0[o] = xv;
+          if (a.slot >= 0) tr[(a.slot * a.B + ch) * op.d + j] = s_ok[c] ? xv : x0[o];
+        }
+      }
+    }
+  }
+}
+
+// misfit (FP64) and gradient (trajectory precision) of model rows
+template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
+__global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, const T* X, int64_t ldx,
+                                                      int64_t B, const double* dobs, const double* var,
+                                                      double mconst, double* chi_out, double* g_out,
+                                                      DevModel md) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* Xs = reinterpret_cast<T*>(smem_raw);
+  T* Rs = Xs + round_up(op.d, 2) * CT;
+  __shared__ double scratch[FU_NW + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * CT;
+  const int64_t nps = op.param_slices();
+  for (int64_t j = tid; j < op.d; j += FU_NT)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) Xs[j * CT + c] = (c0 + c < B) ? X[(c0 + c) * ldx + j] : T(0);
+  __syncthreads();
+  // chain out-of-bounds flags (whole-vector semantics of misfit_bounds)
+  double obl[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) obl[c] = 0.0;
+  for (int64_t j = tid; j < op.d; j += FU_NT)
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+      if (out_of_bounds(md, j, (double)Xs[j * CT + c])) obl[c] = 1.0;
+  double obc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) obc[c] = block_sum<FU_NT>(obl[c], scratch);
+
+  if (chi_out) {
+    double dm[CT], pm[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) dm[c] = pm[c] = 0.0;
+    if (op.n > 0) {
+      op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, [&](int64_t, int64_t i, double* acc) {
+        if (i >= 0) {
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const double rr = acc[c] - dobs[i];
+            dm[c] += rr * rr / var[i];
+          }
+        }
+      });
+    }
+    for (int64_t j = tid; j < op.d; j += FU_NT)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) pm[c] += prior_misfit(md, j, (double)Xs[j * CT + c]);
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const double sd = block_sum<FU_NT>(dm[c], scratch);
+      const double sp = block_sum<FU_NT>(pm[c], scratch);
+      if (tid == 0 && c0 + c < B) {
+        double chi = 0.5 * sd + (sp + mconst);
+        if (obc[c] > 0.0) chi += INFINITY;
+        chi_out[c0 + c] = chi;
+      }
+    }
+  }
+  if (g_out) {
+    if (op.n > 0) {
+      op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, lane, [&](int64_t, int64_t i, T* acc) {
+        if (i >= 0) {
+          const T di = (T)dobs[i], vi = (T)var[i];
+#pragma unroll
+          for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - di) / vi;
+        }
+      });
+      __syncthreads();
+    }
+    for (int64_t s = w; s < nps; s += FU_NW) {
+      T g[CT];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) g[c] = T(0);
+      if (op.n > 0) {
+        if constexpr (SPARSE) {
+          const int32_t base = op.bwd_s.base[s], width = op.bwd_s.width[s];
+          for (int32_t k = 0; k < width; ++k) {
+            const int64_t idx = (int64_t)base + (int64_t)k * 64 + lane;
+            const int32_t col = op.bwd_s.col[idx];
+            const T v = op.bwd_val[idx];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) g[c] += v * Rs[(int64_t)col * CT + c];
+          }
+        } else {
+          const int64_t row = s * 64 + lane;
+          if (row < op.d) {
+            const T* mp = op.bwd_val + row;
+            for (int64_t k = 0; k < op.n; ++k) {
+              const T v = mp[k * op.bwd_d.ld];
+#pragma unroll
+              for (int c = 0; c < CT; ++c) g[c] += v * Rs[k * CT + c];
+            }
+          }
+        }
+      }
+      const int64_t j = op.param_of(s, lane);
+      if (j >= 0) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          if (c0 + c < B) {
+            double gg = (double)(g[c] + prior_grad<T>(md, j, Xs[j * CT + c]));
+            if (obc[c] > 0.0) gg += INFINITY;
+            g_out[(c0 + c) * ldx + j] = gg;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int fused_ct(const Problem& p, int64_t B, size_t& lds_bytes) {
+  const size_t es = dtype_size(p.state_dtype);
+  int dev = 0;
+  hipGetDevice(&dev);
+  int maxlds = 65536;
+  hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  const size_t budget = (size_t)maxlds - 2048;
+  const int cands_f32[] = {4, 2, 1};
+  const int cands_f64[] = {2, 1};
+  const int* cands = p.state_dtype == HMC_F64 ? cands_f64 : cands_f32;
+  const int ncand = p.state_dtype == HMC_F64 ? 2 : 3;
+  for (int i = 0; i < ncand; ++i) {
+    const int ct = cands[i];
+    const size_t bytes = (size_t)(round_up(p.d, 2) + round_up(p.n, 2)) * ct * es;
+    if (bytes > budget) continue;
+    if (ct > 1 && B < ct) continue;
+    lds_bytes = bytes;
+    return ct;
+  }
+  throw Error{HMC_ERR_UNSUPPORTED, "problem too large for the fused engine's LDS tile"};
+}
+
+static int fused_kmax(const Problem& p) {
+  const int64_t nslices = (p.d + 63) / 64;
+  const int64_t per_wave = (nslices + FU_NW - 1) / FU_NW;
+  if (per_wave <= 2) return 2;
+  if (per_wave <= 8) return 8;
+  if (per_wave <= 16) return 16;
+  throw Error{HMC_ERR_UNSUPPORTED, "fused engine supports up to 8192 parameters"};
+}
+
+bool fused_supported(const Problem& p) {
+  if (p.precision == HMC_BF16) return false;
+  if ((p.d + 63) / 64 > (int64_t)16 * FU_NW) return false;
+  return true;
+}
+
+template <typename T, typename VT, bool SPARSE>
+static Ops<T, VT, SPARSE> make_ops(const Problem& p) {
+  Ops<T, VT, SPARSE> op{};
+  op.n = p.n;
+  op.d = p.d;
+  if constexpr (SPARSE) {
+    op.fwd_s = DevSell{p.sg.base, p.sg.width, p.sg.perm, p.sg.col, p.sg.nslices};
+    op.bwd_s = DevSell{p.sgt.base, p.sgt.width, p.sgt.perm, p.sgt.col, p.sgt.nslices};
+    op.fwd_val = (const T*)p.sg.val;
+    op.bwd_val = (const T*)p.sgt.val;
+    op.fwd_tgt = (const VT*)p.sg.val_tgt;
+  } else {
+    // forward over rows i: element (i, k=j) = G[i][j] = Gt[j][i] -> base Gt, ld = ldgt
+    op.fwd_d = DevDense{p.ldgt, p.n, p.d};
+    op.bwd_d = DevDense{p.ldg, p.d, p.n};
+    op.fwd_val = (const T*)p.Gt;
+    op.bwd_val = (const T*)p.G;
+    op.fwd_tgt = (const VT*)p.Gt_tgt;
+  }
+  return op;
+}
+
+template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
+static void launch_propose(Chains& c, const FusedArgs& a, size_t lds) {
+  Problem& p = *c.p;
+  auto kern = k_fused_propose<T, VT, SPARSE, CT, KMAX>;
+  if (lds > 65536) HMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int grid = (int)((c.B + CT - 1) / CT);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(FU_NT), lds, p.stream, make_ops<T, VT, SPARSE>(p), a, p.model());
+  HMC_HIP(hipGetLastError());
+}
+
+template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
+static void launch_eval(Chains& c, const void* X, double* chi, double* g, size_t lds) {
+  Problem& p = *c.p;
+  auto kern = k_fused_eval<T, VT, SPARSE, CT, KMAX>;
+  if (lds > 65536) HMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int grid = (int)((c.B + CT - 1) / CT);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(FU_NT), lds, p.stream, make_ops<T, VT, SPARSE>(p),
+                     (const T*)X, c.ldx, c.B, p.dobs, p.var, p.misfit_const, chi, g, p.model());
+  HMC_HIP(hipGetLastError());
+}
+
+// dispatch over (state dtype, target dtype, sparse, CT, KMAX)
+template <class F>
+static void dispatch(const Problem& p, int ct, int kmax, F&& f) {
+  auto by_k = [&](auto t_tag, auto v_tag, auto s_tag, auto ct_tag) {
+    if (kmax == 2) f(t_tag, v_tag, s_tag, ct_tag, std::integral_constant<int, 2>{});
+    else if (kmax == 8) f(t_tag, v_tag, s_tag, ct_tag, std::integral_constant<int, 8>{});
+    else f(t_tag, v_tag, s_tag, ct_tag, std::integral_constant<int, 16>{});
+  };
+  auto by_ct = [&](auto t_tag, auto v_tag, auto s_tag) {
+    using T = typename decltype(t_tag)::type;
+    if constexpr (std::is_same<T, double>::value) {
+      if (ct == 2) by_k(t_tag, v_tag, s_tag, std::integral_constant<int, 2>{});
+      else by_k(t_tag, v_tag, s_tag, std::integral_constant<int, 1>{});
+    } else {
+      if (ct == 4) by_k(t_tag, v_tag, s_tag, std::integral_constant<int, 4>{});
+      else if (ct == 2) by_k(t_tag, v_tag, s_tag, std::integral_constant<int, 2>{});
+      else by_k(t_tag, v_tag, s_tag, std::integral_constant<int, 1>{});
+    }
+  };
+  auto by_s = [&](auto t_tag, auto v_tag) {
+    if (p.op == 2) by_ct(t_tag, v_tag, std::true_type{});
+    else by_ct(t_tag, v_tag, std::false_type{});
+  };
+  using DT = std::common_type<double>;
+  using FT = std::common_type<float>;
+  if (p.state_dtype == HMC_F64) {
+    by_s(DT{}, DT{});
+  } else {
+    if (p.tgt_dtype == HMC_F64) by_s(FT{}, DT{});
+    else by_s(FT{}, FT{});
+  }
+}
+
+void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed, int64_t prop,
+                   int randomize, int autotune, double target, double learning_rate,
+                   int64_t trace_slot) {
+  Problem& p = *c.p;
+  size_t lds = 0;
+  const int ct = fused_ct(p, c.B, lds);
+  const int kmax = fused_kmax(p);
+  double first = 0.0;
+  std::vector<Stage> st;
+  integrator_schedule(integrator, L, first, st);
+  const int P = (int)st.size() / L;
+  if (trace_slot >= 0) HMC_REQUIRE(trace_slot < c.tr_cap, HMC_ERR_STATE, "trace capacity exceeded");
+
+  FusedArgs a{};
+  a.x0 = c.x0;
+  a.chi0 = c.chi0;
+  a.x_out = c.xt;
+  a.p_out = c.pt;
+  a.p_host = host_p ? c.p_host : nullptr;
+  a.eps = c.eps;
+  a.u = c.u;
+  a.step = c.step;
+  a.h0 = c.h0;
+  a.h1 = c.h1;
+  a.acc = c.acc;
+  a.acc_count = c.acc_count;
+  a.tr_x = c.tr_x;
+  a.tr_chi = c.tr_chi;
+  a.slot = trace_slot;
+  a.ldx = c.ldx;
+  a.B = c.B;
+  a.dobs = p.dobs;
+  a.var = p.var;
+  a.mconst = p.misfit_const;
+  a.seed = seed;
+  a.prop = prop;
+  a.device_scalars = host_p ? 0 : 1;
+  a.randomize = randomize;
+  a.autotune = autotune;
+  a.target = target;
+  a.lr = learning_rate;
+  a.first_drift = first;
+  a.L = L;
+  a.P = P;
+  for (int k = 0; k < P; ++k) {
+    const Stage& s0 = st[k];
+    const Stage& s1 = st[(size_t)(L - 1) * P + k];
+    a.pat[k][0] = s0.kick; a.pat[k][1] = s0.drift; a.pat[k][2] = s0.drift2;
+    a.pat_last[k][0] = s1.kick; a.pat_last[k][1] = s1.drift; a.pat_last[k][2] = s1.drift2;
+  }
+  hipEvent_t ev;
+  time_begin(c, 0, ev);
+  dispatch(p, ct, kmax, [&](auto t_tag, auto v_tag, auto s_tag, auto ct_tag, auto k_tag) {
+    using T = typename decltype(t_tag)::type;
+    using VT = typename decltype(v_tag)::type;
+    launch_propose<T, VT, decltype(s_tag)::value, decltype(ct_tag)::value, decltype(k_tag)::value>(c, a, lds);
+  });
+  time_end(c, 0, ev);
+}
+
+void fused_misfit(Chains& c, const void* X, double* out) {
+  Problem& p = *c.p;
+  size_t lds = 0;
+  const int ct = fused_ct(p, c.B, lds);
+  const int kmax = fused_kmax(p);
+  dispatch(p, ct, kmax, [&](auto t_tag, auto v_tag, auto s_tag, auto ct_tag, auto k_tag) {
+    using T = typename decltype(t_tag)::type;
+    using VT = typename decltype(v_tag)::type;
+    launch_eval<T, VT, decltype(s_tag)::value, decltype(ct_tag)::value, decltype(k_tag)::value>(c, X, out, nullptr, lds);
+  });
+}
+
+void fused_gradient(Chains& c, const void* X, double* out) {
+  Problem& p = *c.p;
+  size_t lds = 0;
+  const int ct = fused_ct(p, c.B, lds);
+  const int kmax = fused_kmax(p);
+  dispatch(p, ct, kmax, [&](auto t_tag, auto v_tag, auto s_tag, auto ct_tag, auto k_tag) {
+    using T = typename decltype(t_tag)::type;
+    using VT = typename decltype(v_tag)::type;
+    launch_eval<T, VT, decltype(s_tag)::value, decltype(ct_tag)::value, decltype(k_tag)::value>(c, X, nullptr, out, lds);
+  });
+}
+
+}  // namespace hmc

@@ -1,0 +1,531 @@
+// MFMA GEMM engine: the trajectory gradient of a dense linear-Gaussian
+// likelihood for a whole batch of chains as two NT GEMMs with fused
+// epilogues, plus the FP64 endpoint misfit.
+//
+//   GEMM1  R[c][i] = ((G X^T)[i][c] - d_i) / var_i        (G: n x d, X: B x d)
+//   GEMM2  g[c][j] = (Gt R^T)[j][c] + prior'(x) -> kick + drift + corrector
+//
+// This is the reference's residual form Gt @ ((G @ m - d) / data_variance)
+// (hmclab/Distributions/LinearMatrix.py:208) with the leapfrog update of
+// Samplers.py:1560-1581 applied in the GEMM2 epilogue, and the misfit
+// 0.5*||(G m - d)/sigma||^2 (LinearMatrix.py:193-202) in FP64.
+//
+// Both operands are K-contiguous ("NT"): A rows are G (or Gt) rows, B rows
+// are chain rows of X (or R).  Output element (m, n) is parameter/data index
+// m of chain n; MFMA accumulators hold four consecutive m of one chain, so
+// the epilogue reads/writes 4-element runs of chain-major rows.
+#include "hmc_internal.hpp"
+
+namespace hmc {
+
+// ---------------------------------------------------------------------------
+// global loads: 8 contiguous elements
+// ---------------------------------------------------------------------------
+__device__ inline void ld8(const double* p, double o[8]) {
+  const f64x2* q = reinterpret_cast<const f64x2*>(p);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const f64x2 v = q[i];
+    o[2 * i] = v[0];
+    o[2 * i + 1] = v[1];
+  }
+}
+__device__ inline void ld8(const float* p, double o[8]) {
+  const floatx4* q = reinterpret_cast<const floatx4*>(p);
+  const floatx4 a = q[0], b = q[1];
+  o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; o[3] = a[3];
+  o[4] = b[0]; o[5] = b[1]; o[6] = b[2]; o[7] = b[3];
+}
+__device__ inline void ld8(const uint16_t* p, double o[8]) {
+  const u32x4 v = *reinterpret_cast<const u32x4*>(p);
+  uint32_t w[4] = {v[0], v[1], v[2], v[3]};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    o[2 * i] = bf16_bits_to_f32((uint16_t)(w[i] & 0xffffu));
+    o[2 * i + 1] = bf16_bits_to_f32((uint16_t)(w[i] >> 16));
+  }
+}
+
+// XCD-aware tile order: consecutive workgroup ids are dealt round-robin to
+// the 8 XCDs; give each XCD a contiguous range of tiles (m-major) so tiles
+// sharing rows of A run on the same L2.
+__device__ inline bool map_tile(int gm, int gn, int& tm, int& tn) {
+  const int T = gm * gn;
+  const int chunk = (T + 7) / 8;
+  const int L = blockIdx.x;
+  const int t = (L % 8) * chunk + (L / 8);
+  if (t >= T) return false;
+  tm = t / gn;
+  tn = t % gn;
+  return true;
+}
+static inline int grid_for(int gm, int gn) { return ((gm * gn + 7) / 8) * 8; }
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA GEMM (v_mfma_f64_16x16x4f64): C = A * Bt^T, block tile BM x BN,
+// k-tile 16, wave tile WM x WN.  K is permuted inside a k-tile: lane group
+// q at MFMA step s uses k = 4q + s, so each lane reads 4 contiguous doubles.
+// Requires A rows up to the tile edge and K padded to 16 (zeros).
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WM, int WN, typename TA, typename TB, class Epi>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
+    const TA* __restrict__ A, int64_t lda, const TB* __restrict__ Bt, int64_t ldb, int M, int N,
+    int K, Epi epi) {
+  constexpr int BK = 16, LDS = BK + 2;
+  constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int WPR = BN / WN;  // waves per row of waves
+  __shared__ double As[BM * LDS];
+  __shared__ double Bs[BN * LDS];
+
+  int tm, tn;
+  if (!map_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn)) return;
+  const int64_t m_blk = (int64_t)tm * BM, n_blk = (int64_t)tn * BN;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w / WPR, wn = w % WPR;
+  const int q = lane >> 4, r = lane & 15;
+
+  // global -> LDS mapping: rows x 16 cols in chunks of 8 elements
+  constexpr int A_CH = BM * 2 / NT;  // chunks per thread (A)
+  constexpr int B_CH = BN * 2 / NT;
+  static_assert(A_CH >= 1 && B_CH >= 1, "tile too small for thread count");
+
+  double ra[A_CH][8], rb[B_CH][8];
+  auto gload = [&](int kt) {
+#pragma unroll
+    for (int i = 0; i < A_CH; ++i) {
+      const int c = tid + i * NT, row = c >> 1, col = (c & 1) * 8;
+      ld8(A + (m_blk + row) * lda + (int64_t)kt * BK + col, ra[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < B_CH; ++i) {
+      const int c = tid + i * NT, row = c >> 1, col = (c & 1) * 8;
+      ld8(Bt + (n_blk + row) * ldb + (int64_t)kt * BK + col, rb[i]);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < A_CH; ++i) {
+      const int c = tid + i * NT, row = c >> 1, col = (c & 1) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) As[row * LDS + col + e] = ra[i][e];
+    }
+#pragma unroll
+    for (int i = 0; i < B_CH; ++i) {
+      const int c = tid + i * NT, row = c >> 1, col = (c & 1) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) Bs[row * LDS + col + e] = rb[i][e];
+    }
+  };
+
+  doublex4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+
+  const int nk = (K + BK - 1) / BK;
+  gload(0);
+  for (int kt = 0; kt < nk; ++kt) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (kt + 1 < nk) gload(kt + 1);
+    double af[TM][4], bfr[TN][4];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const double* src = &As[(wm * WM + i * 16 + r) * LDS + 4 * q];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) af[i][s] = src[s];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const double* src = &Bs[(wn * WN + j * 16 + r) * LDS + 4 * q];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) bfr[j][s] = src[s];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i][s], bfr[j][s], acc[i][j], 0, 0, 0);
+  }
+
+  // D layout: lane (q, r) holds rows 4q..4q+3 of column r of each 16x16 tile
+  if constexpr (Epi::kReduce) {
+    // per-column sums over the block's rows, deterministic order
+    __shared__ double red[(BM / WM)][4][BN];
+    double colsum[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      colsum[j] = 0.0;
+      const int64_t n = n_blk + wn * WN + j * 16 + r;
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int64_t m0 = m_blk + wm * WM + i * 16 + 4 * q;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) colsum[j] += epi.term(m0 + v, n, acc[i][j][v]);
+      }
+      red[wm][q][wn * WN + j * 16 + r] = colsum[j];
+    }
+    __syncthreads();
+    for (int col = tid; col < BN; col += NT) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < BM / WM; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) s += red[a][b][col];
+      epi.store_partial(tm, n_blk + col, s);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int64_t m0 = m_blk + wm * WM + i * 16 + 4 * q;
+        const int64_t n = n_blk + wn * WN + j * 16 + r;
+        double v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+        epi.apply(m0, n, v);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA GEMM (v_mfma_f32_32x32x16_bf16, gfx950): C = A * (B0 + B1)^T
+// with NSPLIT = 1 or 2 bf16 planes of B (hi/lo split of an FP32 operand),
+// FP32 accumulation.  Block tile BM x BN, k-tile 64, wave tile WM x WN.
+// K permutation: lane group q (0/1) at MFMA step s uses k = 32q + 8s + e.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WM, int WN, int NSPLIT, class Epi>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
+    const uint16_t* __restrict__ A, int64_t lda, const uint16_t* __restrict__ B0,
+    const uint16_t* __restrict__ B1, int64_t ldb, int M, int N, int K, Epi epi) {
+  constexpr int BK = 64, LDS = BK + 8;  // elements; row stride 144 B
+  constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  constexpr int TM = WM / 32, TN = WN / 32;
+  constexpr int WPR = BN / WN;
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[NSPLIT][BN * LDS];
+
+  int tm, tn;
+  if (!map_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn)) return;
+  const int64_t m_blk = (int64_t)tm * BM, n_blk = (int64_t)tn * BN;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w / WPR, wn = w % WPR;
+  const int q = lane >> 5, r = lane & 31;
+
+  constexpr int A_CH = BM * 8 / NT;  // 16-byte chunks per thread: rows*8 chunks per row
+  constexpr int B_CH = BN * 8 / NT;
+  static_assert(A_CH >= 1 && B_CH >= 1, "tile too small for thread count");
+
+  u32x4 ra[A_CH], rb0[B_CH], rb1[B_CH];
+#define HMC_BF_GLOAD(kt)                                                                   \
+  do {                                                                                     \
+    _Pragma("unroll") for (int i = 0; i < A_CH; ++i) {                                     \
+      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
+      ra[i] = *reinterpret_cast<const u32x4*>(A + (m_blk + row_) * lda + (int64_t)(kt)*BK + col_); \
+    }                                                                                      \
+    _Pragma("unroll") for (int i = 0; i < B_CH; ++i) {                                     \
+      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
+      const int64_t off_ = (n_blk + row_) * ldb + (int64_t)(kt)*BK + col_;                 \
+      rb0[i] = *reinterpret_cast<const u32x4*>(B0 + off_);                                 \
+      if constexpr (NSPLIT == 2) rb1[i] = *reinterpret_cast<const u32x4*>(B1 + off_);      \
+    }                                                                                      \
+  } while (0)
+#define HMC_BF_SSTORE()                                                                    \
+  do {                                                                                     \
+    _Pragma("unroll") for (int i = 0; i < A_CH; ++i) {                                     \
+      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
+      *reinterpret_cast<u32x4*>(&As[row_ * LDS + col_]) = ra[i];                           \
+    }                                                                                      \
+    _Pragma("unroll") for (int i = 0; i < B_CH; ++i) {                                     \
+      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
+      *reinterpret_cast<u32x4*>(&Bs[0][row_ * LDS + col_]) = rb0[i];                       \
+      if constexpr (NSPLIT == 2) *reinterpret_cast<u32x4*>(&Bs[NSPLIT - 1][row_ * LDS + col_]) = rb1[i]; \
+    }                                                                                      \
+  } while (0)
+
+  floatx16 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+  const int nk = (K + BK - 1) / BK;
+  HMC_BF_GLOAD(0);
+  for (int kt = 0; kt < nk; ++kt) {
+    __syncthreads();
+    HMC_BF_SSTORE();
+    __syncthreads();
+    HMC_BF_GLOAD(kt + 1 < nk ? kt + 1 : kt);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      bf16x8 af[TM], bfr[NSPLIT][TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        af[i] = *reinterpret_cast<const bf16x8*>(&As[(wm * WM + i * 32 + r) * LDS + 32 * q + 8 * s]);
+#pragma unroll
+      for (int sp = 0; sp < NSPLIT; ++sp)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          bfr[sp][j] =
+              *reinterpret_cast<const bf16x8*>(&Bs[sp][(wn * WN + j * 32 + r) * LDS + 32 * q + 8 * s]);
+#pragma unroll
+      for (int sp = 0; sp < NSPLIT; ++sp)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[sp][j], acc[i][j], 0, 0, 0);
+    }
+  }
+
+  // D layout (32x32): lane (q, r) holds rows 8g + 4q + v (g, v = 0..3) of column r
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int64_t m0 = m_blk + wm * WM + i * 32 + 8 * g + 4 * q;
+        const int64_t n = n_blk + wn * WN + j * 32 + r;
+        float v[4] = {acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]};
+        epi.apply(m0, n, v);
+      }
+#undef HMC_BF_GLOAD
+#undef HMC_BF_SSTORE
+}
+
+// ---------------------------------------------------------------------------
+// Epilogues
+// ---------------------------------------------------------------------------
+// residual: out[n][m] = (acc - d_m) / var_m   (m < M, n < N)
+template <typename T>
+struct EpiResid {
+  static constexpr bool kReduce = false;
+  T* out;           // state dtype [N_pad][ld]
+  uint16_t* oh;     // bf16 planes (or nullptr)
+  uint16_t* ol;
+  int64_t ld;
+  const double* dobs;
+  const double* var;
+  int M, N;
+  template <typename V>
+  __device__ void apply(int64_t m0, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t m = m0 + e;
+      if (m >= M) return;
+      const T rr = ((T)v[e] - (T)dobs[m]) / (T)var[m];
+      const int64_t o = n * ld + m;
+      if (out) out[o] = rr;
+      if (oh) {
+        const float f = (float)rr;
+        const uint16_t h = f32_to_bf16_bits(f);
+        oh[o] = h;
+        ol[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
+      }
+    }
+  }
+};
+
+// trajectory step: g = acc + prior'(x); p -= kick*eps*g; x += drift*eps*kgrad(p);
+// corrector; optional second drift + corrector.
+template <typename T>
+struct EpiLeap {
+  static constexpr bool kReduce = false;
+  T* x;
+  T* p;
+  uint16_t* xh;
+  uint16_t* xl;
+  int64_t ld;
+  const double* eps;
+  double kick, drift, drift2;
+  DevModel md;
+  int M, N;
+  template <typename V>
+  __device__ void apply(int64_t m0, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+    const double e = eps[n];
+    const T kk = (T)(kick * e), dd = (T)(drift * e), dd2 = (T)(drift2 * e);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t j = m0 + k;
+      if (j >= M) return;
+      const int64_t o = n * ld + j;
+      T xv = x[o], pv = p[o];
+      T g = (T)v[k] + prior_grad<T>(md, j, xv);
+      if (out_of_bounds(md, j, (double)xv)) g = (T)INFINITY;
+      pv -= kk * g;
+      xv += dd * kgrad<T>(md, j, pv);
+      corrector<T>(md, j, xv, pv);
+      if (drift2 != 0.0) {
+        xv += dd2 * kgrad<T>(md, j, pv);
+        corrector<T>(md, j, xv, pv);
+      }
+      x[o] = xv;
+      p[o] = pv;
+      if (xh) {
+        const float f = (float)xv;
+        const uint16_t h = f32_to_bf16_bits(f);
+        xh[o] = h;
+        xl[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
+      }
+    }
+  }
+};
+
+// gradient output: out[n][m] = acc + prior'(x) (+inf for a chain out of bounds)
+template <typename T>
+struct EpiGrad {
+  static constexpr bool kReduce = false;
+  const T* x;
+  double* out;
+  int64_t ld;
+  const int32_t* oob;
+  DevModel md;
+  int M, N;
+  template <typename V>
+  __device__ void apply(int64_t m0, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t j = m0 + k;
+      if (j >= M) return;
+      const int64_t o = n * ld + j;
+      double g = (double)((T)v[k] + prior_grad<T>(md, j, x[o]));
+      if (oob[n]) g += INFINITY;
+      out[o] = g;
+    }
+  }
+};
+
+// endpoint misfit partial sums: sum_m (acc - d_m)^2 / var_m over the block rows
+struct EpiMisfit {
+  static constexpr bool kReduce = true;
+  const double* dobs;
+  const double* var;
+  double* partial;   // [gm][ldp]
+  int64_t ldp;
+  int M, N;
+  __device__ double term(int64_t m, int64_t n, double v) const {
+    if (m >= M || n >= N) return 0.0;
+    const double rr = v - dobs[m];
+    return rr * rr / var[m];
+  }
+  __device__ void store_partial(int tm, int64_t n, double s) const {
+    if (n < N) partial[(int64_t)tm * ldp + n] = s;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+// FP64 tiles: 128 x 128, 4 waves of 64 x 64.
+constexpr int F64_BM = 128, F64_BN = 128, F64_WM = 64, F64_WN = 64;
+// BF16 tiles: 128 x 128, 4 waves of 64 x 64.
+constexpr int BF_BM = 128, BF_BN = 128, BF_WM = 64, BF_WN = 64;
+
+template <typename TA, typename TB, class Epi>
+static void launch_f64(hipStream_t s, const TA* A, int64_t lda, const TB* Bt, int64_t ldb, int M,
+                       int N, int K, const Epi& epi) {
+  const int gm = (M + F64_BM - 1) / F64_BM, gn = (N + F64_BN - 1) / F64_BN;
+  constexpr int NT = (F64_BM / F64_WM) * (F64_BN / F64_WN) * 64;
+  hipLaunchKernelGGL((k_gemm_f64<F64_BM, F64_BN, F64_WM, F64_WN, TA, TB, Epi>), dim3(grid_for(gm, gn)),
+                     dim3(NT), 0, s, A, lda, Bt, ldb, M, N, K, epi);
+  HMC_HIP(hipGetLastError());
+}
+
+template <int NSPLIT, class Epi>
+static void launch_bf16(hipStream_t s, const uint16_t* A, int64_t lda, const uint16_t* B0,
+                        const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+  const int gm = (M + BF_BM - 1) / BF_BM, gn = (N + BF_BN - 1) / BF_BN;
+  constexpr int NT = (BF_BM / BF_WM) * (BF_BN / BF_WN) * 64;
+  hipLaunchKernelGGL((k_gemm_bf16<BF_BM, BF_BN, BF_WM, BF_WN, NSPLIT, Epi>), dim3(grid_for(gm, gn)),
+                     dim3(NT), 0, s, A, lda, B0, B1, ldb, M, N, K, epi);
+  HMC_HIP(hipGetLastError());
+}
+
+int64_t gemm_row_pad() { return 128; }   // M-tile of every GEMM config
+int64_t gemm_k_pad() { return 64; }      // k-tile multiple of every GEMM config
+int64_t gemm_chain_pad() { return 128; } // N-tile of every GEMM config
+
+// Trajectory: residual GEMM then gradient GEMM with kick/drift epilogue.
+void gemm_kick_drift(Chains& c, double kick, double drift, double drift2) {
+  Problem& p = *c.p;
+  const int n = (int)p.n, d = (int)p.d, B = (int)c.B;
+  const DevModel md = p.model();
+  hipEvent_t ev;
+  time_begin(c, 0, ev);
+  if (p.precision == HMC_BF16) {
+    EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
+    launch_bf16<2>(p.stream, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, c.xl, c.ldx, c.eps, kick, drift, drift2, md, d, B};
+    launch_bf16<2>(p.stream, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+  } else if (p.precision == HMC_F64) {
+    EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
+    launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)c.xt, c.ldx, n, B, (int)p.ldg, e1);
+    EpiLeap<double> e2{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, kick, drift, drift2, md, d, B};
+    launch_f64(p.stream, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
+  } else {
+    throw Error{HMC_ERR_UNSUPPORTED, "GEMM engine supports float64 and bfloat16 trajectories"};
+  }
+  time_end(c, 0, ev);
+}
+
+// Endpoint data misfit in FP64: partial sums of (G x - d)^2 / var per row block.
+void gemm_endpoint(Chains& c, const void* X, int x_dtype) {
+  Problem& p = *c.p;
+  const int n = (int)p.n, B = (int)c.B;
+  EpiMisfit e{p.dobs, p.var, c.partial, c.B_pad, n, B};
+  hipEvent_t ev;
+  time_begin(c, 1, ev);
+  if (p.tgt_dtype == HMC_BF16) {
+    if (x_dtype == HMC_F32)
+      launch_f64(p.stream, (const uint16_t*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
+    else
+      launch_f64(p.stream, (const uint16_t*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
+  } else if (p.tgt_dtype == HMC_F32) {
+    if (x_dtype == HMC_F32)
+      launch_f64(p.stream, (const float*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
+    else
+      launch_f64(p.stream, (const float*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
+  } else {
+    if (x_dtype == HMC_F32)
+      launch_f64(p.stream, (const double*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
+    else
+      launch_f64(p.stream, (const double*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
+  }
+  time_end(c, 1, ev);
+}
+
+// Gradient of the target at model rows X (state dtype) into out (f64, stride ldx).
+void gemm_gradient(Chains& c, const void* X, double* out) {
+  Problem& p = *c.p;
+  const int n = (int)p.n, d = (int)p.d, B = (int)c.B;
+  const DevModel md = p.model();
+  elem_oob_flags(c, X);
+  if (p.precision == HMC_BF16) {
+    // hi/lo planes of X were prepared by the caller into c.xh / c.xl
+    EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
+    launch_bf16<2>(p.stream, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    EpiGrad<float> e2{(const float*)X, out, c.ldx, c.oob, md, d, B};
+    launch_bf16<2>(p.stream, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+  } else if (p.precision == HMC_F64) {
+    EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
+    launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e1);
+    EpiGrad<double> e2{(const double*)X, out, c.ldx, c.oob, md, d, B};
+    launch_f64(p.stream, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
+  } else {
+    throw Error{HMC_ERR_UNSUPPORTED, "GEMM engine supports float64 and bfloat16 trajectories"};
+  }
+}
+
+}  // namespace hmc

@@ -1,0 +1,167 @@
+// Host-side objects behind the C-ABI handles and the engine entry points.
+#pragma once
+
+#include <vector>
+
+#include "hmc_common.hpp"
+
+namespace hmc {
+
+// Sliced ELL with 64-row slices (one wavefront per slice), rows sorted by
+// descending length inside the whole matrix so each slice is nearly uniform.
+// Element k of slice s, lane r lives at base[s] + k*64 + r (coalesced).
+struct Sell {
+  int64_t rows = 0, cols = 0, nslices = 0, total = 0;
+  int32_t* base = nullptr;   // [nslices] element offset of the slice
+  int32_t* width = nullptr;  // [nslices] entries per lane in the slice
+  int32_t* perm = nullptr;   // [nslices*64] original row of (slice, lane), -1 = padding
+  int32_t* col = nullptr;    // [total]
+  void* val = nullptr;       // [total] trajectory precision
+  void* val_tgt = nullptr;   // [total] target precision (aliases val when equal)
+  int max_width = 0;
+};
+
+struct Problem {
+  int64_t d = 0;            // parameters
+  int64_t n = 0;            // data (0: no likelihood)
+  int precision = HMC_F64;  // trajectory arithmetic
+  int state_dtype = HMC_F64;// storage of positions/momenta (F64 or F32)
+  int op = 0;               // 0 none, 1 dense, 2 csr
+  int engine = HMC_ENGINE_AUTO;
+  int tgt_dtype = HMC_F64;  // element type defining the target's G
+
+  // dense (row-major, zero padded): G [n_pad][ldg], Gt [d_pad][ldgt]
+  int64_t n_pad = 0, d_pad = 0, ldg = 0, ldgt = 0;
+  void* G = nullptr;
+  void* Gt = nullptr;
+  void* G_tgt = nullptr;    // target-precision copy of G (aliases G when equal)
+  void* Gt_tgt = nullptr;   // target-precision copy of Gt (aliases Gt when equal)
+
+  // sparse
+  Sell sg, sgt;
+
+  double* dobs = nullptr;   // [n]
+  double* var = nullptr;    // [n] data variance
+
+  // element-wise part
+  int n_priors = 0;
+  int prior_kind[kMaxPriors] = {0, 0, 0, 0};
+  double* prior_a[kMaxPriors] = {nullptr, nullptr, nullptr, nullptr};
+  double* prior_b[kMaxPriors] = {nullptr, nullptr, nullptr, nullptr};
+  double misfit_const = 0.0;
+  double* lo = nullptr;
+  double* hi = nullptr;
+  double* minv = nullptr;
+  double* msqrt = nullptr;
+  int mass_kind = HMC_MASS_UNIT;
+
+  hipStream_t stream = nullptr;
+  std::vector<void*> owned;   // device allocations freed at destroy
+
+  DevModel model() const {
+    DevModel m{};
+    m.n_priors = n_priors;
+    for (int k = 0; k < n_priors; ++k) m.pr[k] = DevPrior{prior_kind[k], prior_a[k], prior_b[k]};
+    m.lo = lo;
+    m.hi = hi;
+    m.minv = minv;
+    m.msqrt = msqrt;
+    return m;
+  }
+  int resolved_engine() const;
+};
+
+struct Chains {
+  Problem* p = nullptr;
+  int64_t B = 0, B_pad = 0;
+  int64_t ldx = 0;          // row stride of position/momentum arrays
+  int64_t ldr = 0;          // row stride of residual arrays
+  void* x0 = nullptr;       // current positions [B_pad][ldx] (state dtype)
+  double* chi0 = nullptr;   // [B_pad]
+  void* xt = nullptr;       // trajectory positions
+  void* pt = nullptr;       // trajectory momenta
+  double* p_host = nullptr; // host-supplied momenta staging [B_pad][ldx] f64
+  uint16_t* xh = nullptr;   // bf16 hi/lo planes of xt (GEMM engine, BF16)
+  uint16_t* xl = nullptr;
+  uint16_t* rh = nullptr;   // bf16 hi/lo planes of the residual
+  uint16_t* rl = nullptr;
+  void* r = nullptr;        // residual [B_pad][ldr] (state dtype, F64/F32 GEMM)
+  double* partial = nullptr;// endpoint partial sums [mblocks][B_pad]
+  int64_t n_mblocks = 0;
+  double* k0 = nullptr;     // kinetic energy at the start of the trajectory
+  double* eps = nullptr;    // local step per chain
+  double* u = nullptr;      // accept uniform per chain
+  double* h0 = nullptr;
+  double* h1 = nullptr;
+  int32_t* acc = nullptr;   // accept flag per chain
+  double* step = nullptr;   // device per-chain step size (run mode)
+  int64_t* acc_count = nullptr;
+  int32_t* oob = nullptr;   // per-chain out-of-bounds flag (gradient_batch)
+
+  // trace
+  void* tr_x = nullptr;
+  double* tr_chi = nullptr;
+  int64_t tr_cap = 0, tr_n = 0;
+
+  // timing
+  bool timing = false;
+  double t_ms[3] = {0, 0, 0};
+  int64_t t_n[3] = {0, 0, 0};
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  size_t ev_next = 0;
+
+  std::vector<void*> owned;
+};
+
+// allocation helpers
+void* dev_alloc(std::vector<void*>& owned, size_t bytes);
+size_t dtype_size(int dt);
+
+// kernel timing brackets (no-ops unless c.timing)
+void time_begin(Chains& c, int kind, hipEvent_t& a);
+void time_end(Chains& c, int kind, hipEvent_t a);
+void time_collect(Chains& c);
+
+// ---- GEMM engine (hmc_gemm.hip) -------------------------------------------
+// Trajectory gradient + fused kick/drift for every chain:
+//   p -= kick*eps*g(xt); xt += drift*eps*Minv p; corrector; [xt += drift2*eps*Minv p; corrector]
+void gemm_kick_drift(Chains& c, double kick, double drift, double drift2);
+// Endpoint partial sums of the data misfit in FP64 into c.partial.
+void gemm_endpoint(Chains& c, const void* X, int x_dtype);
+// gradient (B x d) of model rows X into out (device, f64, row stride ldx)
+void gemm_gradient(Chains& c, const void* X, double* out);
+
+// ---- element-wise kernels (hmc_elem.hip) -----------------------------------
+// momentum refresh + first drift (drift0*eps), K0.  host_p: use c.p_host.
+void elem_refresh(Chains& c, bool host_p, double drift0, uint64_t seed, int64_t prop,
+                  bool device_scalars, int randomize);
+// standalone drift + corrector: xt += drift*eps*Minv pt
+void elem_drift(Chains& c, double drift);
+// endpoint Hamiltonian + Metropolis + state update (+ autotune, trace record)
+void elem_metropolis(Chains& c, bool have_partials, int autotune, double target,
+                     double learning_rate, int64_t prop_index, int64_t trace_slot);
+// misfit of X (device, state dtype) -> out[B] using c.partial (or none)
+void elem_finish_misfit(Chains& c, const void* X, bool have_partials, double* out);
+// convert f64 host-layout rows into state dtype rows (+bf16 planes when given)
+void elem_load_rows(Chains& c, const double* src, void* dst, uint16_t* hi, uint16_t* lo);
+void elem_store_rows(Chains& c, const void* src, int src_dtype, int64_t ld, double* dst);
+void elem_oob_flags(Chains& c, const void* X);
+
+// ---- fused engine (hmc_fused.hip) ------------------------------------------
+bool fused_supported(const Problem& p);
+// Whole proposal (refresh, L steps, endpoint, Metropolis) in one launch.
+void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed, int64_t prop,
+                   int randomize, int autotune, double target, double learning_rate,
+                   int64_t trace_slot);
+void fused_misfit(Chains& c, const void* X, double* out);
+void fused_gradient(Chains& c, const void* X, double* out);
+
+// integrator schedules: coefficients of the first drift and of each
+// (kick, drift, drift2) stage; returned in units of the local step.
+struct Stage {
+  double kick, drift, drift2;
+};
+void integrator_schedule(int integrator, int L, double& first_drift, std::vector<Stage>& stages);
+
+}  // namespace hmc

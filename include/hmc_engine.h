@@ -1,0 +1,195 @@
+/*
+ * hmc_engine.h — C-ABI of the hmclab_amd HIP engine (libhmc.so, gfx950).
+ *
+ * The reference (larsgeb/hmclab) is pure Python; its only native call site
+ * is a ctypes binding of a CSR SpMV (hmclab/Helpers/InterfaceMKL.py:32-35,
+ * 119-121, 187-189) that takes raw pointers, 0-based int32 CSR arrays and
+ * caller-allocated outputs.  This header follows that precedent: plain
+ * pointers and sizes, host arrays in float64 unless stated, int status codes,
+ * no exceptions and no aborts across the boundary.
+ *
+ * Each entry point names the reference interface it replaces.  Arrays of
+ * chains are row-major (B, d): chain c's coordinates are contiguous, i.e.
+ * the reference's (d, 1) column vectors laid side by side as rows.
+ */
+#ifndef HMC_ENGINE_H
+#define HMC_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HMC_OK 0
+#define HMC_ERR_ARG 1       /* invalid argument (Python: ValueError)          */
+#define HMC_ERR_STATE 2     /* call out of order (Python: AssertionError)     */
+#define HMC_ERR_HIP 3       /* HIP runtime failure (Python: RuntimeError)     */
+#define HMC_ERR_NOMEM 4     /* allocation failure (Python: MemoryError)       */
+#define HMC_ERR_UNSUPPORTED 5 /* combination not built (NotImplementedError)  */
+#define HMC_ERR_INJECTED 6  /* fault injected by HMC_FAULT_INJECT (tests)     */
+
+/* element types */
+#define HMC_F64 0
+#define HMC_F32 1
+#define HMC_BF16 2
+
+/* prior term kinds */
+#define HMC_PRIOR_NORMAL 1  /* a = means, b = inverse variances    */
+#define HMC_PRIOR_LAPLACE 2 /* a = means, b = inverse dispersions  */
+
+/* mass matrix kinds */
+#define HMC_MASS_UNIT 0
+#define HMC_MASS_DIAGONAL 1
+
+/* integrators (Samplers.py:1728-1737) */
+#define HMC_INT_LEAPFROG 0
+#define HMC_INT_3STAGE 1
+#define HMC_INT_4STAGE 2
+
+/* engines */
+#define HMC_ENGINE_AUTO 0
+#define HMC_ENGINE_GEMM 1   /* MFMA GEMM per gradient over the whole batch     */
+#define HMC_ENGINE_FUSED 2  /* one kernel per proposal per chain tile          */
+
+typedef struct hmc_problem hmc_problem;
+typedef struct hmc_chains hmc_chains;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int hmc_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* hmc_last_error(void);
+
+/* Number of visible HIP devices (0 when no GPU is present). */
+int hmc_device_count(int* count);
+
+/* ---------------------------------------------------------------------------
+ * Problem = one BayesRule target (hmclab/Distributions/base.py:1005-1205):
+ * an optional linear-Gaussian likelihood plus element-wise priors, collapsed
+ * bounds and the mass matrix.  Host inputs are copied to the device at set
+ * time, as the reference copies/casts at construction (LinearMatrix.py:152-159).
+ * ------------------------------------------------------------------------- */
+
+/* Create an empty target of `dimensions` parameters.  `precision` is the
+ * arithmetic type of trajectory gradients (HMC_F64, HMC_F32 or HMC_BF16);
+ * the Metropolis Hamiltonian is always evaluated in float64.
+ * Replaces: BayesRule.__init__ (base.py:1012-1034). */
+int hmc_problem_create(int64_t dimensions, int precision, hmc_problem** out);
+
+/* Dense forward model G (n_data x dimensions, row-major, element type
+ * g_dtype), observed data d (n_data) and data variance (n_data values, or one
+ * value when variance_is_scalar).  The target is defined by G as given in
+ * g_dtype; trajectory arithmetic rounds it to `precision`.
+ * Replaces: _LinearMatrix_dense_forward_simple_covariance.__init__
+ * (LinearMatrix.py:139-182), residual form. */
+int hmc_problem_set_dense(hmc_problem* p, const void* G, int g_dtype, int64_t n_data,
+                          const double* d_obs, const double* data_variance,
+                          int variance_is_scalar);
+
+/* Sparse CSR forward model (0-based int32 indptr[n_data+1], indices[nnz],
+ * values[nnz] of type dtype).  Replaces:
+ * _LinearMatrix_sparse_forward_simple_covariance.__init__ (LinearMatrix.py:309-359)
+ * and the CSR SpMV binding sparse_gemv (InterfaceMKL.py:38-64). */
+int hmc_problem_set_csr(hmc_problem* p, const int32_t* indptr, const int32_t* indices,
+                        const void* values, int dtype, int64_t n_data, int64_t nnz,
+                        const double* d_obs, const double* data_variance,
+                        int variance_is_scalar);
+
+/* Add one element-wise prior term (kind HMC_PRIOR_*), per-parameter arrays
+ * a, b of length `dimensions`; `constant` is its normalization constant.
+ * Replaces: Normal.misfit/gradient (base.py:539-574, diagonal case) and
+ * Laplace.misfit/gradient (base.py:689-710). */
+int hmc_problem_add_prior(hmc_problem* p, int kind, const double* a, const double* b,
+                          double constant);
+
+/* Collapsed bounds (either may be NULL).  Replaces: update_bounds /
+ * collapse_bounds / corrector / misfit_bounds (base.py:239-374, 1061-1142). */
+int hmc_problem_set_bounds(hmc_problem* p, const double* lower, const double* upper);
+
+/* Mass matrix (HMC_MASS_UNIT with NULL, or HMC_MASS_DIAGONAL with the
+ * diagonal).  Replaces: Unit / Diagonal (MassMatrices.py:82-238). */
+int hmc_problem_set_mass(hmc_problem* p, int kind, const double* diagonal);
+
+/* Choose the trajectory engine (HMC_ENGINE_*). */
+int hmc_problem_set_engine(hmc_problem* p, int engine);
+
+int hmc_problem_destroy(hmc_problem* p);
+
+/* Batched misfit chi(m) for B models M (B x dimensions) -> out[B].
+ * Replaces: BayesRule.misfit (base.py:1036-1043). */
+int hmc_misfit_batch(hmc_problem* p, const double* M, int64_t B, double* out);
+
+/* Batched gradient for B models -> G_out (B x dimensions).
+ * Replaces: BayesRule.gradient (base.py:1045-1054). */
+int hmc_gradient_batch(hmc_problem* p, const double* M, int64_t B, double* out);
+
+/* ---------------------------------------------------------------------------
+ * Chains = B independent Markov chains on one problem, state resident on the
+ * device.  Replaces the per-chain HMC sampler state of Samplers.HMC
+ * (Samplers.py:1105-1178) and ParallelSampleSMP's one-process-per-chain
+ * layout (Samplers.py:1816-1977) by columns of one batch.
+ * ------------------------------------------------------------------------- */
+int hmc_chains_create(hmc_problem* p, int64_t B, hmc_chains** out);
+int hmc_chains_destroy(hmc_chains* c);
+
+/* Set positions (B x dimensions) and evaluate their misfits.  Fails with
+ * HMC_ERR_ARG when a misfit is NaN or inf (Samplers.py:419-426). */
+int hmc_chains_set_state(hmc_chains* c, const double* X);
+
+/* Copy positions (B x dimensions, may be NULL) and misfits (B, may be NULL). */
+int hmc_chains_get_state(hmc_chains* c, double* X, double* chi);
+
+/* One proposal for every chain with host-supplied randoms (parity mode):
+ * momenta P0 (B x dimensions, already scaled by sqrt(M)), local step sizes
+ * eps[B] (stepsize * uniform(0.5,1.5) as the reference draws them), accept
+ * uniforms u[B].  Outputs (any may be NULL): accepted[B], H0[B], H1[B] and
+ * the trajectory end point X1/P1 (B x dimensions).
+ * Replaces: HMC._propose + _evaluate_acceptance (Samplers.py:1463-1492) with
+ * the integrator of Samplers.py:1524-1726. */
+int hmc_chains_propose(hmc_chains* c, const double* P0, const double* eps, const double* u,
+                       int amount_of_steps, int integrator, int32_t* accepted, double* H0,
+                       double* H1, double* X1, double* P1);
+
+/* Run `proposals` proposals entirely on the device with the Philox RNG
+ * (seed, counter offset `first_proposal`).  Per-chain step sizes stepsize[B]
+ * (host, in/out when autotuning).  randomize: multiply by uniform(0.5,1.5).
+ * autotune != 0 applies the Robbins-Monro update of Samplers.py:1494-1522
+ * with (target, learning_rate).  Every `thinning`-th proposal (index i with
+ * i % thinning == 0) is recorded into the device trace (see below).
+ * accepted_total[B] (may be NULL) receives accept counts of this call.
+ * Replaces: _AbstractSampler._sample_loop (Samplers.py:529-704). */
+int hmc_chains_run(hmc_chains* c, int64_t proposals, int amount_of_steps, int integrator,
+                   double* stepsize, int randomize, int autotune, double target,
+                   double learning_rate, uint64_t seed, int64_t first_proposal,
+                   int64_t thinning, int64_t* accepted_total);
+
+/* Device trace of recorded samples: capacity in slots; each slot holds
+ * B x dimensions coordinates (stored in the trajectory precision) and B
+ * misfits.  Setting a capacity discards recorded samples. */
+int hmc_chains_set_trace(hmc_chains* c, int64_t capacity_slots, int64_t param_stride);
+/* Number of recorded slots. */
+int hmc_chains_trace_count(hmc_chains* c, int64_t* slots);
+/* Copy slots [first, first+count) to host as float64: X (count x B x dims)
+ * and chi (count x B); either may be NULL.  */
+int hmc_chains_read_trace(hmc_chains* c, int64_t first, int64_t count, double* X, double* chi);
+/* Device pointer and element type of the trace (for device-side analysis). */
+int hmc_chains_trace_device(hmc_chains* c, void** x_ptr, double** chi_ptr, int* dtype);
+int hmc_chains_reset_trace(hmc_chains* c);
+
+/* Accumulated kernel times since the last reset, from HIP events on the
+ * engine's stream.  kind: 0 = trajectory gradient kernels (the dominant
+ * kernel), 1 = endpoint misfit, 2 = element-wise/metropolis.  total_ms and
+ * launches may be NULL.  Timing is off until hmc_chains_enable_timing(c,1). */
+int hmc_chains_enable_timing(hmc_chains* c, int on);
+int hmc_chains_timing(hmc_chains* c, int kind, double* total_ms, int64_t* launches);
+
+/* Synchronise the engine's stream. */
+int hmc_chains_sync(hmc_chains* c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMC_ENGINE_H */

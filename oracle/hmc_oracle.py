@@ -1,0 +1,509 @@
+"""NumPy restatement of the reference HMC hot path (test infrastructure only).
+
+Every class/function cites the reference file:line it restates.  The
+arithmetic keeps the reference's NumPy expressions (operand order, scalar
+association such as ``(0.5 * eps) * v``) so that a chain run here follows the
+reference's op order; shapes are the reference's column vectors ``(d, 1)``.
+
+Nothing in ``hmclab_amd`` imports this module.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+
+# --------------------------------------------------------------------------- #
+# Distributions (hmclab/Distributions/base.py)
+# --------------------------------------------------------------------------- #
+class _Bounded:
+    """Bounds, misfit_bounds and reflection corrector.
+
+    Restates ``_AbstractDistribution.corrector`` (base.py:239-270),
+    ``update_bounds`` (base.py:272-359) and ``misfit_bounds`` (base.py:361-374).
+    """
+
+    lower_bounds = None
+    upper_bounds = None
+    dimensions = None
+    normalization_constant = 0.0
+
+    def update_bounds(self, lower=None, upper=None):
+        old = (self.lower_bounds, self.upper_bounds)
+        if type(upper) == list:
+            upper = np.array(upper)[:, None]
+        if type(lower) == list:
+            lower = np.array(lower)[:, None]
+        self.upper_bounds, self.lower_bounds = upper, lower
+        if lower is not None and type(lower) is not np.ndarray:
+            self.lower_bounds, self.upper_bounds = old
+            raise ValueError("Lower bounds object not understood.")
+        if upper is not None and type(upper) is not np.ndarray:
+            self.lower_bounds, self.upper_bounds = old
+            raise ValueError("Upper bounds object not understood.")
+        if (self.lower_bounds is not None and self.lower_bounds.shape != (self.dimensions, 1)) or (
+            self.upper_bounds is not None and self.upper_bounds.shape != (self.dimensions, 1)
+        ):
+            self.lower_bounds, self.upper_bounds = old
+            raise ValueError("Bounds vectors are of incorrect size.")
+        if (
+            self.lower_bounds is not None
+            and self.upper_bounds is not None
+            and np.any(self.upper_bounds <= self.lower_bounds)
+        ):
+            self.lower_bounds, self.upper_bounds = old
+            raise ValueError("Bounds vectors are incompatible.")
+
+    def misfit_bounds(self, x):
+        if (self.lower_bounds is not None and np.any(x < self.lower_bounds)) or (
+            self.upper_bounds is not None and np.any(x > self.upper_bounds)
+        ):
+            return np.inf
+        return 0.0
+
+    def corrector(self, x, p):
+        if self.lower_bounds is not None:
+            low = x < self.lower_bounds
+            x[low] += 2 * (self.lower_bounds[low] - x[low])
+            p[low] *= -1.0
+        if self.upper_bounds is not None:
+            high = x > self.upper_bounds
+            x[high] += 2 * (self.upper_bounds[high] - x[high])
+            p[high] *= -1.0
+
+
+class Normal(_Bounded):
+    """Diagonal / scalar / full-covariance Gaussian.  base.py:427-574."""
+
+    def __init__(self, means, covariance, lower_bounds=None, upper_bounds=None):
+        means = np.array(means, dtype=np.float64).reshape(-1, 1)
+        self.dimensions = means.size
+        self.means = means
+        if np.isscalar(covariance) or np.ndim(covariance) == 0:
+            covariance = np.float64(covariance)
+            self.diagonal = True
+        else:
+            covariance = np.array(covariance, dtype=np.float64)
+            if covariance.shape == (means.size, means.size) and means.size > 1:
+                self.diagonal = False
+            else:
+                self.diagonal = True
+                covariance = covariance.reshape(self.dimensions, 1)
+        self.covariance = covariance
+        if self.diagonal:
+            self.inverse_covariance = 1.0 / self.covariance          # base.py:528
+        else:
+            self.inverse_covariance = np.linalg.inv(self.covariance)  # base.py:531
+        self.update_bounds(lower_bounds, upper_bounds)
+
+    def normalize(self):
+        """base.py:576-594."""
+        if np.ndim(self.covariance) == 0:
+            det = self.covariance ** self.dimensions
+        elif self.covariance.shape == (self.dimensions, self.dimensions) and not self.diagonal:
+            det = np.linalg.det(self.covariance)
+        else:
+            det = np.prod(self.covariance)
+        self.normalization_constant = 0.5 * (
+            np.log(np.abs(det)) + self.dimensions * np.log(2 * np.pi)
+        )
+
+    def misfit(self, x):
+        """base.py:539-562."""
+        if self.diagonal:
+            return (
+                self.misfit_bounds(x)
+                + 0.5 * ((self.means - x).T @ (self.inverse_covariance * (self.means - x))).flatten()[0]
+                + self.normalization_constant
+            )
+        return (
+            self.misfit_bounds(x)
+            + 0.5 * ((self.means - x).T @ self.inverse_covariance @ (self.means - x)).flatten()[0]
+            + self.normalization_constant
+        )
+
+    def gradient(self, x):
+        """base.py:564-574."""
+        if self.diagonal:
+            return -self.inverse_covariance * (self.means - x) + self.misfit_bounds(x)
+        return -self.inverse_covariance @ (self.means - x) + self.misfit_bounds(x)
+
+
+class Laplace(_Bounded):
+    """Uncorrelated Laplace.  base.py:646-710."""
+
+    def __init__(self, means, dispersions, lower_bounds=None, upper_bounds=None):
+        means = np.array(means, dtype=np.float64).reshape(-1, 1)
+        self.dimensions = means.size
+        self.means = means
+        disp = np.array(dispersions, dtype=np.float64)
+        disp = disp.reshape(self.dimensions, 1) if disp.size == self.dimensions else disp
+        self.dispersions = disp
+        self.inverse_dispersions = 1.0 / disp
+        self.update_bounds(lower_bounds, upper_bounds)
+
+    def normalize(self):
+        """base.py:712-725."""
+        self.normalization_constant = np.log(
+            1.0 / (2.0 * (np.prod(self.dispersions) ** (1.0 / self.dimensions)))
+        )
+
+    def misfit(self, x):
+        """base.py:689-700."""
+        return (
+            self.normalization_constant
+            + self.misfit_bounds(x)
+            + (np.sum(np.abs(x - self.means) * self.inverse_dispersions)).flatten()[0]
+        )
+
+    def gradient(self, x):
+        """base.py:702-710 (sign(0) = 0)."""
+        return self.misfit_bounds(x) + np.sign(x - self.means) * self.inverse_dispersions
+
+
+class Uniform(_Bounded):
+    """Bounded flat density.  base.py:747-786."""
+
+    def __init__(self, lower_bounds, upper_bounds):
+        lo = np.resize(np.asarray(lower_bounds, dtype=np.float64), (np.size(lower_bounds), 1))
+        hi = np.resize(np.asarray(upper_bounds, dtype=np.float64), (np.size(upper_bounds), 1))
+        self.dimensions = lo.size
+        self.update_bounds(lo, hi)
+
+    def misfit(self, x):
+        return self.misfit_bounds(x)
+
+    def gradient(self, x):
+        return np.zeros((self.dimensions, 1)) + self.misfit_bounds(x)
+
+
+class BayesRule(_Bounded):
+    """Sum of component misfits with collapsed bounds.  base.py:1005-1205."""
+
+    def __init__(self, components, lower_bounds=None, upper_bounds=None):
+        self.dimensions = components[0].dimensions
+        self.separate_distributions = list(components)
+        for c in components:
+            assert c.dimensions == self.dimensions
+        self.lower_bounds = lower_bounds
+        self.upper_bounds = upper_bounds
+        self.collapse_bounds()
+
+    def collapse_bounds(self):
+        """base.py:1061-1101: max of lowers, min of uppers."""
+        for c in self.separate_distributions:
+            if c.lower_bounds is not None:
+                self.lower_bounds = (
+                    c.lower_bounds if self.lower_bounds is None
+                    else np.maximum(self.lower_bounds, c.lower_bounds)
+                )
+            if c.upper_bounds is not None:
+                self.upper_bounds = (
+                    c.upper_bounds if self.upper_bounds is None
+                    else np.minimum(self.upper_bounds, c.upper_bounds)
+                )
+
+    def misfit(self, x):
+        """base.py:1036-1043."""
+        misfit = 0.0
+        for c in self.separate_distributions:
+            misfit += c.misfit(x)
+        return misfit + self.misfit_bounds(x)
+
+    def gradient(self, x):
+        """base.py:1045-1054."""
+        g = np.zeros((self.dimensions, 1))
+        for c in self.separate_distributions:
+            g += c.gradient(x)
+        return g + self.misfit_bounds(x)
+
+
+# --------------------------------------------------------------------------- #
+# LinearMatrix backends (hmclab/Distributions/LinearMatrix.py)
+# --------------------------------------------------------------------------- #
+class LinearMatrixSimple(_Bounded):
+    """Dense or CSR forward model with scalar / vector data variance.
+
+    Restates ``_LinearMatrix_dense_forward_simple_covariance``
+    (LinearMatrix.py:139-208) and ``_LinearMatrix_sparse_forward_simple_covariance``
+    (LinearMatrix.py:309-426): the residual form
+    ``chi = 0.5*||(G m - d)/sigma||^2``, ``grad = G^T ((G m - d)/var)`` and the
+    premultiplied form ``chi = 0.5*(m^T(GtG m - 2 Gtd0) + dtd)``,
+    ``grad = GtG m - Gtd0``.  ``dtype`` is the storage type of G and d
+    (the backends default to float32, LinearMatrix.py:148,318).
+    """
+
+    def __init__(self, G, d, data_variance, dtype=np.float64, premultiplication=None):
+        sparse = sp.issparse(G)
+        self.dimensions = G.shape[1]
+        self.sparse = sparse
+        G_in = G
+        self.G = sp.csr_matrix(G, dtype=dtype) if sparse else np.asarray(G).astype(dtype)
+        self.d = np.asarray(d).astype(dtype)
+        if type(data_variance) == np.ndarray:
+            self.data_variance = data_variance.astype(dtype)
+        else:
+            self.data_variance = data_variance
+        self.data_sigma = self.data_variance ** 0.5
+        self.premultiplication = (
+            premultiplication if premultiplication is not None else self.G.shape[0] > self.G.shape[1]
+        )
+        if self.premultiplication:
+            if sparse:
+                if type(self.data_variance) == float:
+                    invcov = sp.eye(self.d.size).tocsr() / self.data_variance
+                else:
+                    invcov = sp.diags(1.0 / self.data_variance[:, 0], offsets=0).tocsr()
+            else:
+                if type(self.data_variance) == float:
+                    invcov = np.eye(self.d.size) / self.data_variance
+                else:
+                    invcov = np.diag(1.0 / self.data_variance[:, 0])
+            self.GtG = self.G.T @ invcov @ self.G
+            self.Gtd0 = G_in.T @ invcov @ self.d      # un-cast G, LinearMatrix.py:176,353
+            self.dtd = (self.d.T @ invcov @ self.d).item()
+        else:
+            self.Gt = self.G.T
+
+    def misfit(self, x):
+        if self.premultiplication:
+            return self.misfit_bounds(x) + (0.5 * (x.T @ (self.GtG @ x - 2 * self.Gtd0) + self.dtd)).item()
+        return self.misfit_bounds(x) + (
+            0.5 * np.linalg.norm((self.G @ x - self.d) / self.data_sigma) ** 2
+        ).item()
+
+    def gradient(self, x):
+        if self.premultiplication:
+            return self.GtG @ x - self.Gtd0
+        return self.Gt @ ((self.G @ x - self.d) / self.data_variance)
+
+
+# --------------------------------------------------------------------------- #
+# Mass matrices (hmclab/MassMatrices.py)
+# --------------------------------------------------------------------------- #
+class Unit:
+    """MassMatrices.py:82-156."""
+
+    name = "unit mass matrix"
+
+    def __init__(self, dimensions, rng=None):
+        self.dimensions = dimensions
+        self.rng = rng
+
+    def kinetic_energy(self, p):
+        if p.shape != (self.dimensions, 1):
+            raise ValueError()
+        return 0.5 * (p.T @ p).item(0)
+
+    def kinetic_energy_gradient(self, p, position=None, g=None):
+        if p.shape != (self.dimensions, 1):
+            raise ValueError()
+        return p
+
+    def generate_momentum(self):
+        return self.rng.normal(size=(self.dimensions, 1))
+
+
+class Diagonal:
+    """MassMatrices.py:159-238."""
+
+    name = "diagonal mass matrix"
+
+    def __init__(self, diagonal, rng=None):
+        diagonal = np.asarray(diagonal, dtype=np.float64).reshape(-1, 1)
+        self.dimensions = diagonal.size
+        self.diagonal = diagonal
+        self.inverse_diagonal = 1.0 / diagonal
+        self.rng = rng
+
+    def kinetic_energy(self, p):
+        if p.shape != (self.dimensions, 1):
+            raise ValueError()
+        return 0.5 * np.vdot(p, self.inverse_diagonal * p)
+
+    def kinetic_energy_gradient(self, p, position=None, g=None):
+        if p.shape != (self.dimensions, 1):
+            raise ValueError()
+        return self.inverse_diagonal * p
+
+    def generate_momentum(self):
+        return np.sqrt(self.diagonal) * self.rng.normal(size=(self.dimensions, 1))
+
+
+# --------------------------------------------------------------------------- #
+# Integrators (hmclab/Samplers.py:1524-1738)
+# --------------------------------------------------------------------------- #
+def leapfrog(dist, mass, x0, p0, local_eps, L):
+    """Position-first leapfrog, Samplers.py:1524-1584 (L gradient evaluations)."""
+    x = x0.copy()
+    p = p0.copy()
+    x += 0.5 * local_eps * mass.kinetic_energy_gradient(p, x, None)
+    dist.corrector(x, p)
+    for _ in range(L - 1):
+        g = dist.gradient(x)
+        p -= local_eps * g
+        x += local_eps * mass.kinetic_energy_gradient(p, x, g)
+        dist.corrector(x, p)
+    g = dist.gradient(x)
+    p -= local_eps * g
+    x += 0.5 * local_eps * mass.kinetic_energy_gradient(p, x, g)
+    dist.corrector(x, p)
+    return x, p
+
+
+THREE_STAGE = (0.11888010966548, 0.29619504261126)          # Samplers.py:1666-1669
+FOUR_STAGE = (0.071353913450279725904, 0.268548791161230105820, 0.191667800000000000000)  # :1588-1592
+
+
+def _stages(kind):
+    """Drift/kick coefficient schedule (a1,b1,a2,b2,...) of the multi-stage integrators."""
+    if kind == "3s":
+        a1, b1 = THREE_STAGE
+        a2 = 1.0 / 2.0 - a1
+        b2 = 1.0 - 2.0 * b1
+        return [("a", a1), ("b", b1), ("a", a2), ("b", b2), ("a", a2), ("b", b1), ("a", a1)]
+    a1, a2, b1 = FOUR_STAGE
+    a3 = 1.0 - 2.0 * a1 - 2.0 * a2
+    b2 = 1.0 / 2.0 - b1
+    return [("a", a1), ("b", b1), ("a", a2), ("b", b2), ("a", a3), ("b", b2), ("a", a2), ("b", b1), ("a", a1)]
+
+
+def multistage(dist, mass, x0, p0, local_eps, L, kind):
+    """3-stage (Samplers.py:1663-1726) / 4-stage (:1586-1661) integrators.
+
+    Coefficients are pre-multiplied by the local step (``a1 *= local_stepsize``,
+    :1599-1603, :1676-1679) before use, as in the reference.
+    """
+    sched = [(t, c * local_eps) for t, c in _stages(kind)]
+    x = x0.copy()
+    p = p0.copy()
+    g = None
+    for _ in range(L):
+        for t, c in sched:
+            if t == "a":
+                x += c * mass.kinetic_energy_gradient(p, x, g)
+                dist.corrector(x, p)
+            else:
+                g = dist.gradient(x)
+                p -= c * g
+    return x, p
+
+
+def integrate(dist, mass, x0, p0, local_eps, L, integrator="lf"):
+    if integrator == "lf":
+        return leapfrog(dist, mass, x0, p0, local_eps, L)
+    return multistage(dist, mass, x0, p0, local_eps, L, integrator)
+
+
+def metropolis(current_h, proposed_h, u):
+    """Samplers.py:1481-1486: accept iff exp(H0 - H1) > u (strict)."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        a = np.exp(current_h - proposed_h)
+    return a, bool(a > u)
+
+
+def autotune_step(stepsize, proposal_index, acceptance_rate, target, learning_rate, minimal=1e-18):
+    """Robbins-Monro step-size update, Samplers.py:1494-1522."""
+    w = (proposal_index + 1) ** (-learning_rate)
+    if np.isnan(acceptance_rate):
+        acceptance_rate = 0
+    stepsize -= w * (target - min(acceptance_rate, 1))
+    if stepsize <= 0:
+        stepsize = max(stepsize, minimal)
+    return stepsize
+
+
+# --------------------------------------------------------------------------- #
+# One proposal with caller-supplied randoms (teacher forcing)
+# --------------------------------------------------------------------------- #
+def propose(dist, mass, x0, chi0, p0, local_eps, L, u, integrator="lf"):
+    """Integrate + Metropolis for given momentum, local step and uniform.
+
+    Restates ``HMC._propose`` + ``_evaluate_acceptance`` (Samplers.py:1463-1492)
+    with the randoms passed in.  ``chi0`` is recomputed exactly as the
+    reference does (``misfit(current_model)``, :1473) when None.
+    Returns a dict with the end state, energies and the decision.
+    """
+    x1, p1 = integrate(dist, mass, x0, p0, local_eps, L, integrator)
+    if chi0 is None:
+        chi0 = dist.misfit(x0)
+    k0 = mass.kinetic_energy(p0)
+    chi1 = dist.misfit(x1)
+    k1 = mass.kinetic_energy(p1)
+    h0 = chi0 + k0
+    h1 = chi1 + k1
+    a, acc = metropolis(h0, h1, u)
+    return dict(x1=x1, p1=p1, chi0=chi0, chi1=chi1, k0=k0, k1=k1, h0=h0, h1=h1, a=a, accepted=acc)
+
+
+# --------------------------------------------------------------------------- #
+# Full seeded chain (the reference's _sample_loop with one Generator)
+# --------------------------------------------------------------------------- #
+def sample_chain(dist, mass, seed, stepsize=0.1, randomize_stepsize=True, amount_of_steps=10,
+                 integrator="lf", initial_model=None, proposals=100, online_thinning=1,
+                 autotuning=False, target_acceptance_rate=0.65, learning_rate=0.75,
+                 record=False):
+    """Seeded chain with the reference draw order.
+
+    Draw order per proposal, from one ``numpy.random.default_rng(seed)``
+    shared with the mass matrix (Samplers.py:316-319,1365):
+    momentum ``normal(size=(d,1))`` (MassMatrices.py:142/227), then
+    ``uniform(0.5,1.5)`` if ``randomize_stepsize`` (Samplers.py:1533-1534),
+    then the accept ``uniform(0,1)`` (Samplers.py:1486).
+    Column ``i`` of the returned (d+1, N) array is written when
+    ``i % online_thinning == 0`` (Samplers.py:675-678); the initial model is
+    not written.
+    """
+    rng = np.random.default_rng(seed)
+    mass.rng = rng
+    d = dist.dimensions
+    x = np.zeros((d, 1)) if initial_model is None else np.array(initial_model, dtype=np.float64).reshape(d, 1)
+    chi = dist.misfit(x)
+    cols = []
+    rec = []
+    accepted = 0
+    for i in range(proposals):
+        p0 = mass.generate_momentum()
+        local_eps = rng.uniform(0.5, 1.5) * stepsize if randomize_stepsize else stepsize
+        x1, p1 = integrate(dist, mass, x, p0, local_eps, amount_of_steps, integrator)
+        chi0 = dist.misfit(x)
+        h0 = chi0 + mass.kinetic_energy(p0)
+        chi1 = dist.misfit(x1)
+        h1 = chi1 + mass.kinetic_energy(p1)
+        with np.errstate(over="ignore", invalid="ignore"):
+            a = np.exp(h0 - h1)
+        step_before = stepsize
+        if autotuning:
+            stepsize = autotune_step(stepsize, i, a, target_acceptance_rate, learning_rate)
+        u = rng.uniform(0, 1)
+        acc = bool(a > u)
+        if acc:
+            x = np.copy(x1)
+            chi = chi1
+            accepted += 1
+        else:
+            chi = chi0
+        if record:
+            rec.append(dict(p0=p0, local_eps=local_eps, u=u, accepted=acc, h0=h0, h1=h1,
+                            stepsize=step_before))
+        if i % online_thinning == 0:
+            cols.append(np.vstack([x, chi]))
+    out = np.hstack(cols)
+    return (out, rec, accepted) if record else out
+
+
+def analytic_linear_gaussian(G, d, data_variance, prior_means, prior_variance):
+    """Posterior of a linear-Gaussian problem with diagonal prior/noise.
+
+    ``Sigma = (G^T C_d^-1 G + C_m^-1)^-1``, ``mu = Sigma (G^T C_d^-1 d + C_m^-1 mu_m)``
+    (notebooks/examples/Sampling sparse linear equations.ipynb, cell 52).
+    """
+    G = G.toarray() if sp.issparse(G) else np.asarray(G, dtype=np.float64)
+    n, dd = G.shape
+    w = np.broadcast_to(1.0 / np.asarray(data_variance, dtype=np.float64).reshape(-1), (n,))
+    cm_inv = np.broadcast_to(1.0 / np.asarray(prior_variance, dtype=np.float64).reshape(-1), (dd,))
+    H = G.T @ (w[:, None] * G) + np.diag(cm_inv)
+    rhs = G.T @ (w * np.asarray(d, dtype=np.float64).reshape(-1)) + cm_inv * np.asarray(prior_means, dtype=np.float64).reshape(-1)
+    Sigma = np.linalg.inv(H)
+    mu = Sigma @ rhs
+    return mu, Sigma

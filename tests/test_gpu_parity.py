@@ -1,0 +1,254 @@
+"""GPU parity against the oracle, through the C-ABI.
+
+Tolerances (relative to the oracle's float64 values, per element, scaled by
+the vector's largest magnitude): float64 1e-10, float32 1e-4; bfloat16
+trajectories (G given in bf16, hi/lo split operands, FP32 accumulation):
+gradients 2e-4, misfits (always FP64) 1e-10.  Accept/reject decisions must be
+identical except for near-ties |log u - (H0 - H1)| < 1e-6 (reported, not
+failed).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from hmclab_amd import Distributions as D
+from hmclab_amd import MassMatrices as MM
+from hmclab_amd.Distributions import build_problem
+from hmclab_amd.engine import Chains, bf16_round, bf16_to_f32
+from oracle import hmc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-10, "f32": 1e-4}
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def _dense_case(seed, n, d, prior="normal", bounds=False, sparse=False, density=0.1, g_dtype=np.float64):
+    rng = np.random.default_rng(seed)
+    G = rng.normal(size=(n, d)) / np.sqrt(d)
+    if sparse:
+        G[rng.uniform(size=G.shape) > density] = 0.0
+    G = G.astype(g_dtype)
+    m_true = rng.normal(size=(d, 1))
+    dd = G.astype(np.float64) @ m_true + rng.normal(0, 0.1, size=(n, 1))
+    var = rng.uniform(0.005, 0.02, size=(n, 1))
+    Gp = sp.csr_matrix(G) if sparse else G
+    comps_gpu, comps_or = [], []
+    if prior in ("normal", "both"):
+        mu, pv = rng.normal(size=(d, 1)) * 0.1, rng.uniform(0.5, 2.0, size=(d, 1))
+        comps_gpu.append(D.Normal(mu.copy(), pv.copy()))
+        comps_or.append(O.Normal(mu, pv))
+    if prior in ("laplace", "both"):
+        mu, b = rng.normal(size=(d, 1)) * 0.1, rng.uniform(0.3, 1.0, size=(d, 1))
+        comps_gpu.append(D.Laplace(mu.copy(), b.copy()))
+        comps_or.append(O.Laplace(mu, b))
+    if bounds:
+        lo, hi = -3.0 * np.ones(d), 3.0 * np.ones(d)
+        comps_gpu.append(D.Uniform(lo, hi))
+        comps_or.append(O.Uniform(lo, hi))
+    comps_gpu.append(D.LinearMatrix(Gp, dd, var))
+    comps_or.append(O.LinearMatrixSimple(Gp, dd, var, dtype=g_dtype, premultiplication=False))
+    return D.BayesRule(comps_gpu), O.BayesRule(comps_or), rng
+
+
+# --------------------------------------------------------------------------- #
+@pytest.mark.parametrize("engine", ["gemm", "fused"])
+@pytest.mark.parametrize("n,d", [(100, 50), (300, 200), (130, 257)])
+def test_dense_misfit_gradient_f64(engine, n, d):
+    post, orc, rng = _dense_case(1, n, d, prior="both", bounds=True)
+    prob = build_problem(post, precision="f64", engine=engine)
+    X = rng.normal(size=(5, d))
+    X[4, 0] = 3.5                      # out of bounds -> inf misfit, inf gradient
+    chi = prob.misfit_batch(X)
+    g = prob.gradient_batch(X)
+    for b in range(4):
+        x = X[b].reshape(d, 1)
+        assert _rel(chi[b], orc.misfit(x)) < TOL["f64"]
+        assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL["f64"]
+    assert np.isinf(chi[4]) and np.all(np.isinf(g[4]))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_sparse_misfit_gradient(precision):
+    dtype = np.float64 if precision == "f64" else np.float32
+    post, orc, rng = _dense_case(2, 400, 300, prior="normal", sparse=True, g_dtype=dtype)
+    prob = build_problem(post, precision=precision)
+    X = rng.normal(size=(7, 300))
+    chi = prob.misfit_batch(X)
+    g = prob.gradient_batch(X)
+    for b in range(7):
+        x = X[b].reshape(-1, 1)
+        assert _rel(chi[b], orc.misfit(x)) < 1e-10         # endpoint misfit is FP64
+        assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
+
+
+def test_bf16_gemm_gradient_and_fp64_misfit():
+    rng = np.random.default_rng(3)
+    n, d = 520, 384
+    G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
+    dd = rng.normal(size=(n, 1))
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, dtype="bfloat16")])
+    orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
+                       O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
+    prob = build_problem(post, precision="bf16")
+    X = rng.normal(size=(3, d))
+    chi = prob.misfit_batch(X)
+    g = prob.gradient_batch(X)
+    for b in range(3):
+        x = X[b].reshape(d, 1)
+        assert _rel(chi[b], orc.misfit(x)) < 1e-10
+        assert _rel(g[b], orc.gradient(x)[:, 0]) < 2e-4
+
+
+# --------------------------------------------------------------------------- #
+def _teacher(post, orc, mass_g, mass_o, rng, B, L, integ, precision, engine, eps=0.01):
+    d = post.dimensions
+    prob = build_problem(post, precision=precision, engine=engine,
+                         mass_diagonal=mass_g._device_diagonal())
+    ch = Chains(prob, B)
+    X0 = rng.normal(size=(B, d)) * 0.3
+    ch.set_state(X0)
+    X0s, chi0 = ch.get_state()
+    P0 = np.stack([mass_o.generate_momentum()[:, 0] for _ in range(B)])
+    e = eps * rng.uniform(0.5, 1.5, size=B)
+    u = rng.uniform(size=B)
+    res = ch.propose(P0, e, u, L, integ, trajectory=True)
+    tol = TOL[precision]
+    near = 0
+    for b in range(B):
+        x0 = X0s[b].reshape(d, 1)
+        ref = O.propose(orc, mass_o, x0, None, P0[b].reshape(d, 1), e[b], L, u[b], integ)
+        assert _rel(chi0[b], ref["chi0"]) < max(tol, 1e-10)
+        assert _rel(res["X1"][b], ref["x1"][:, 0]) < tol
+        assert _rel(res["P1"][b], ref["p1"][:, 0]) < tol
+        assert _rel(res["H1"][b], ref["h1"]) < tol
+        assert _rel(res["H0"][b], ref["h0"]) < tol
+        margin = abs(np.log(u[b]) - (ref["h0"] - ref["h1"]))
+        if margin < 1e-6:
+            near += 1
+        else:
+            assert bool(res["accepted"][b]) == ref["accepted"]
+    return near
+
+
+@pytest.mark.parametrize("engine", ["gemm", "fused"])
+@pytest.mark.parametrize("integ", ["lf", "3s", "4s"])
+@pytest.mark.parametrize("L", [1, 10])
+def test_teacher_forced_proposal_f64(engine, integ, L):
+    post, orc, rng = _dense_case(4, 150, 70, prior="both", bounds=True)
+    d = post.dimensions
+    _teacher(post, orc, MM.Unit(d), O.Unit(d, np.random.default_rng(5)), rng, 6, L, integ, "f64", engine)
+
+
+@pytest.mark.parametrize("engine", ["gemm", "fused"])
+def test_teacher_forced_diagonal_mass_f64(engine):
+    post, orc, rng = _dense_case(6, 120, 64, prior="normal")
+    d = post.dimensions
+    diag = rng.uniform(0.5, 3.0, size=(d, 1))
+    _teacher(post, orc, MM.Diagonal(diag), O.Diagonal(diag, np.random.default_rng(7)), rng, 4, 12,
+             "lf", "f64", engine)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_teacher_forced_sparse(precision):
+    dtype = np.float64 if precision == "f64" else np.float32
+    post, orc, rng = _dense_case(8, 300, 200, prior="normal", sparse=True, g_dtype=dtype)
+    d = post.dimensions
+    diag = rng.uniform(0.5, 2.0, size=(d, 1))
+    _teacher(post, orc, MM.Diagonal(diag), O.Diagonal(diag, np.random.default_rng(9)), rng, 9, 8,
+             "lf", precision, "auto", eps=0.02)
+
+
+def test_bounds_reflection_in_trajectory():
+    # tight bounds force reflections inside the fused kick/drift epilogue
+    rng = np.random.default_rng(10)
+    d = 20
+    lo, hi = -0.2 * np.ones(d), 0.25 * np.ones(d)
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.Uniform(lo, hi)])
+    orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0), O.Uniform(lo, hi)])
+    _teacher(post, orc, MM.Unit(d), O.Unit(d, np.random.default_rng(11)), rng, 5, 15, "lf", "f64",
+             "auto", eps=0.05)
+
+
+# --------------------------------------------------------------------------- #
+def _c1(seed=0):
+    """The dense 1-chain configuration: G 100x50 N(0,1), noise 0.1, prior N(0,1)."""
+    rng = np.random.default_rng(seed)
+    G = rng.normal(size=(100, 50))
+    m_true = rng.normal(size=(50, 1))
+    dd = G @ m_true + rng.normal(0, 0.1, size=(100, 1))
+    post = D.BayesRule([D.Normal(np.zeros((50, 1)), 1.0), D.LinearMatrix(G, dd, 0.01)])
+    orc = O.BayesRule([O.Normal(np.zeros((50, 1)), 1.0),
+                       O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
+    return G, dd, post, orc
+
+
+@pytest.mark.parametrize("engine", ["gemm", "fused"])
+@pytest.mark.parametrize("stepsize", [0.005, 0.05])
+def test_seeded_chain_matches_oracle(engine, stepsize):
+    from hmclab_amd import Samplers
+
+    G, dd, post, orc = _c1()
+    sam = Samplers.HMC(seed=7)
+    sam.sample(None, post, stepsize=stepsize, amount_of_steps=10, proposals=60, engine=engine,
+               disable_progressbar=True)
+    got = sam.results[0]
+    ref = O.sample_chain(orc, O.Unit(50), seed=7, stepsize=stepsize, amount_of_steps=10, proposals=60)
+    assert got.shape == ref.shape
+    assert _rel(got, ref) < 1e-9
+
+
+@pytest.mark.parametrize("engine", ["gemm", "fused"])
+def test_device_rng_posterior_matches_analytic(engine):
+    rng = np.random.default_rng(12)
+    n, d, B = 40, 6, 256
+    G = rng.normal(size=(n, d))
+    dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.3, size=(n, 1))
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.09)])
+    mu, Sigma = O.analytic_linear_gaussian(G, dd, 0.09, np.zeros(d), 1.0)
+    prob = build_problem(post, precision="f64", engine=engine)
+    ch = Chains(prob, B)
+    ch.set_state(np.broadcast_to(mu[:, 0], (B, d)))
+    lam = np.linalg.eigvalsh(np.linalg.inv(Sigma)).max()
+    eps = 1.0 / np.sqrt(lam)
+    ch.run(50, 8, eps, seed=3)                      # burn-in
+    ch.set_trace(200)
+    step, acc = ch.run(200, 8, eps, seed=3, first_proposal=50)
+    X, chi = ch.read_trace(0, 200)
+    S = X.reshape(-1, d)
+    assert acc.mean() / 200 > 0.5
+    sd = np.sqrt(np.diag(Sigma))
+    se = sd / np.sqrt(S.shape[0] / 10.0)
+    assert np.all(np.abs(S.mean(axis=0) - mu[:, 0]) < 5 * se)
+    C = np.cov(S.T)
+    assert np.allclose(np.sqrt(np.diag(C)), sd, rtol=0.05)
+    corr = C / np.outer(np.sqrt(np.diag(C)), np.sqrt(np.diag(C)))
+    ref_corr = Sigma / np.outer(sd, sd)
+    assert np.max(np.abs(corr - ref_corr)) < 0.05
+
+
+def test_device_rng_is_reproducible_and_engine_independent():
+    G, dd, post, orc = _c1(1)
+    outs = []
+    for engine in ("gemm", "fused", "fused"):
+        prob = build_problem(post, precision="f64", engine=engine)
+        ch = Chains(prob, 4)
+        ch.set_state(np.zeros((4, 50)))
+        ch.set_trace(30)
+        ch.run(30, 10, 0.004, seed=99)
+        outs.append(ch.read_trace(0, 30)[0])
+    assert np.array_equal(outs[1], outs[2])            # bitwise reproducible
+    assert _rel(outs[0], outs[1]) < 1e-9               # same draws in both engines
+
+
+def test_set_state_rejects_infinite_misfit():
+    d = 4
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.Uniform(-np.ones(d), np.ones(d))])
+    ch = Chains(build_problem(post), 2)
+    with pytest.raises(ValueError, match="inf"):
+        ch.set_state(np.full((2, d), 2.0))

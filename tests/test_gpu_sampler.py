@@ -1,0 +1,94 @@
+"""End-to-end ``HMC.sample`` on the GPU with the reference's file semantics:
+file shape (d+1, floor(i/thinning)+1) (tests/test_sampling.py:134-152),
+bit-identical files for equal seeds (tests/test_sampling_reproducibility.py:
+117-119), existing file -> FileExistsError, max_time, autotuning, several
+chains, diagnostic timing."""
+import os
+import pickle
+
+import numpy as np
+import pytest
+
+import hmclab_amd as hmc
+from hmclab_amd import Distributions as D
+from hmclab_amd import MassMatrices as MM
+from hmclab_amd.Samples import Samples
+
+pytestmark = pytest.mark.gpu
+
+
+def _post(d=5, bounded=False, seed=0):
+    rng = np.random.default_rng(seed)
+    G = rng.normal(size=(12, d))
+    dd = rng.normal(size=(12, 1))
+    comps = [D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.5)]
+    if bounded:
+        comps.append(D.Uniform(-2 * np.ones(d), 2 * np.ones(d)))
+    return D.BayesRule(comps)
+
+
+@pytest.mark.parametrize("thin,proposals", [(1, 20), (5, 50)])
+def test_file_shape_and_attributes(tmp_path, thin, proposals):
+    fn = str(tmp_path / "s.npy")
+    post = _post()
+    s = hmc.HMC(seed=1).sample(fn, post, proposals=proposals, online_thinning=thin, stepsize=0.05,
+                               disable_progressbar=True)
+    with Samples(fn) as smp:
+        assert smp.numpy.shape == (post.dimensions + 1, s.current_proposal // thin + 1)
+        assert smp.read_attribute("sampler") == "Hamiltonian Monte Carlo"
+        assert smp.read_attribute("integrator") == "leapfrog integrator"
+        assert 0.0 <= smp.read_attribute("acceptance_rate") <= 1.0
+    with pytest.raises(FileExistsError):
+        hmc.HMC(seed=1).sample(fn, post, proposals=5, disable_progressbar=True)
+
+
+def test_reproducible_files(tmp_path):
+    post = _post(bounded=True)
+    a, b = str(tmp_path / "a.npy"), str(tmp_path / "b.npy")
+    for fn in (a, b):
+        hmc.HMC(seed=3).sample(fn, post, proposals=30, stepsize=0.1, autotuning=True,
+                               mass_matrix=MM.Diagonal(np.full(5, 2.0)), integrator="3s",
+                               disable_progressbar=True)
+    assert np.array_equal(np.load(a), np.load(b))
+
+
+def test_max_time_stops_gracefully(tmp_path):
+    fn = str(tmp_path / "t.npy")
+    s = hmc.HMC(seed=2).sample(fn, _post(), proposals=1_000_000, max_time=0.5, disable_progressbar=True)
+    assert s.current_proposal < 1_000_000 - 1
+    with Samples(fn) as smp:
+        assert smp.numpy.shape[1] == s.current_proposal + 1
+
+
+def test_many_chains_device_rng_files(tmp_path):
+    fn = str(tmp_path / "m.npy")
+    post = _post(d=8)
+    s = hmc.HMC(seed=4).sample(fn, post, proposals=40, online_thinning=4, chains=16, stepsize=0.05,
+                               autotuning=True, disable_progressbar=True)
+    for c in range(16):
+        with Samples(str(tmp_path / f"m_{c}.npy")) as smp:
+            assert smp.numpy.shape == (9, 10)
+            assert np.all(np.isfinite(smp.numpy))
+    assert s.stepsize.shape == (16,)
+
+
+def test_in_memory_results_and_diagnostics(capsys):
+    s = hmc.HMC(seed=5).sample(None, _post(), proposals=10, chains=3, rng="host", diagnostic_mode=True,
+                               disable_progressbar=True)
+    assert s.results.shape == (3, 6, 10)
+    out = capsys.readouterr().out
+    assert "Detailed statistics" in out
+    d = s.get_diagnostics()
+    assert d["trajectory gradient kernels"][1] > 0
+
+
+def test_gpu_misfit_gradient_methods():
+    post = _post(d=4)
+    x = np.ones((4, 1)) * 0.3
+    g = post.gradient(x)
+    h = 1e-6
+    fd = np.array([(post.misfit(x + h * e[:, None]) - post.misfit(x - h * e[:, None])) / (2 * h)
+                   for e in np.eye(4)])
+    assert np.allclose(g[:, 0], fd, rtol=1e-6, atol=1e-8)
+    M = np.random.default_rng(0).normal(size=(4, 7))
+    assert np.allclose(post.misfit_batch(M), [post.misfit(M[:, k:k + 1]) for k in range(7)])
