@@ -347,11 +347,17 @@ class LinearMatrix(_AbstractDistribution):
         self.use_mkl = use_mkl
         self.g_dtype = None
         if dtype is not None and str(dtype) in ("bfloat16", "bf16"):
+            # the target is defined by bf16-rounded G; data and covariance stay float64
             self.g_dtype = "bf16"
-            dtype = np.float32
+            G = G.astype(np.float32)
+            dtype = None
         if dtype is not None:
+            # LinearMatrix.py:30-34,152-155: G, d and a vector/matrix covariance
+            # take the storage type
             G = G.astype(dtype)
             d = np.asarray(d).astype(dtype)
+            if type(data_covariance) not in (float, np.float32, np.float64):
+                data_covariance = data_covariance.astype(dtype)
         self.dimensions = G.shape[1]
         if not (type(d) is np.ndarray and d.shape == (d.size, 1)):
             raise ValueError(

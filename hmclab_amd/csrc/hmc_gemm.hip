@@ -12,8 +12,9 @@
 //
 // Both operands are K-contiguous ("NT"): A rows are G (or Gt) rows, B rows
 // are chain rows of X (or R).  Output element (m, n) is parameter/data index
-// m of chain n; MFMA accumulators hold four consecutive m of one chain, so
-// the epilogue reads/writes 4-element runs of chain-major rows.
+// m of chain n; MFMA accumulators hold four m of one chain, so
+// the epilogue reads/writes 4-element runs of chain-major rows (bf16: 4
+// consecutive rows; f64: rows q, q+4, q+8, q+12 of a 16-row tile).
 #include "hmc_internal.hpp"
 
 namespace hmc {
@@ -154,7 +155,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i][s], bfr[j][s], acc[i][j], 0, 0, 0);
   }
 
-  // D layout: lane (q, r) holds rows 4q..4q+3 of column r of each 16x16 tile
+  // D layout (measured on gfx950): lane (q, r) holds rows q, q+4, q+8, q+12 of
+  // column r of each 16x16 tile
   if constexpr (Epi::kReduce) {
     // per-column sums over the block's rows, deterministic order
     __shared__ double red[(BM / WM)][4][BN];
@@ -165,9 +167,9 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
       const int64_t n = n_blk + wn * WN + j * 16 + r;
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
-        const int64_t m0 = m_blk + wm * WM + i * 16 + 4 * q;
+        const int64_t m0 = m_blk + wm * WM + i * 16 + q;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) colsum[j] += epi.term(m0 + v, n, acc[i][j][v]);
+        for (int v = 0; v < 4; ++v) colsum[j] += epi.term(m0 + 4 * v, n, acc[i][j][v]);
       }
       red[wm][q][wn * WN + j * 16 + r] = colsum[j];
     }
@@ -185,10 +187,10 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int64_t m0 = m_blk + wm * WM + i * 16 + 4 * q;
+        const int64_t m0 = m_blk + wm * WM + i * 16 + q;
         const int64_t n = n_blk + wn * WN + j * 16 + r;
         double v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
-        epi.apply(m0, n, v);
+        epi.apply(m0, 4, n, v);
       }
   }
 }
@@ -296,7 +298,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
         const int64_t m0 = m_blk + wm * WM + i * 32 + 8 * g + 4 * q;
         const int64_t n = n_blk + wn * WN + j * 32 + r;
         float v[4] = {acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]};
-        epi.apply(m0, n, v);
+        epi.apply(m0, 1, n, v);
       }
 #undef HMC_BF_GLOAD
 #undef HMC_BF_SSTORE
@@ -317,11 +319,11 @@ struct EpiResid {
   const double* var;
   int M, N;
   template <typename V>
-  __device__ void apply(int64_t m0, int64_t n, const V v[4]) const {
+  __device__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int64_t m = m0 + e;
+      const int64_t m = m0 + (int64_t)e * ms;
       if (m >= M) return;
       const T rr = ((T)v[e] - (T)dobs[m]) / (T)var[m];
       const int64_t o = n * ld + m;
@@ -351,13 +353,13 @@ struct EpiLeap {
   DevModel md;
   int M, N;
   template <typename V>
-  __device__ void apply(int64_t m0, int64_t n, const V v[4]) const {
+  __device__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
     const double e = eps[n];
     const T kk = (T)(kick * e), dd = (T)(drift * e), dd2 = (T)(drift2 * e);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int64_t j = m0 + k;
+      const int64_t j = m0 + (int64_t)k * ms;
       if (j >= M) return;
       const int64_t o = n * ld + j;
       T xv = x[o], pv = p[o];
@@ -393,11 +395,11 @@ struct EpiGrad {
   DevModel md;
   int M, N;
   template <typename V>
-  __device__ void apply(int64_t m0, int64_t n, const V v[4]) const {
+  __device__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int64_t j = m0 + k;
+      const int64_t j = m0 + (int64_t)k * ms;
       if (j >= M) return;
       const int64_t o = n * ld + j;
       double g = (double)((T)v[k] + prior_grad<T>(md, j, x[o]));

@@ -51,7 +51,7 @@ def _dense_case(seed, n, d, prior="normal", bounds=False, sparse=False, density=
         lo, hi = -3.0 * np.ones(d), 3.0 * np.ones(d)
         comps_gpu.append(D.Uniform(lo, hi))
         comps_or.append(O.Uniform(lo, hi))
-    comps_gpu.append(D.LinearMatrix(Gp, dd, var))
+    comps_gpu.append(D.LinearMatrix(Gp, dd, var, dtype=None if g_dtype == np.float64 else g_dtype))
     comps_or.append(O.LinearMatrixSimple(Gp, dd, var, dtype=g_dtype, premultiplication=False))
     return D.BayesRule(comps_gpu), O.BayesRule(comps_or), rng
 
@@ -62,7 +62,7 @@ def _dense_case(seed, n, d, prior="normal", bounds=False, sparse=False, density=
 def test_dense_misfit_gradient_f64(engine, n, d):
     post, orc, rng = _dense_case(1, n, d, prior="both", bounds=True)
     prob = build_problem(post, precision="f64", engine=engine)
-    X = rng.normal(size=(5, d))
+    X = 0.5 * rng.normal(size=(5, d))  # inside the +-3 bounds
     X[4, 0] = 3.5                      # out of bounds -> inf misfit, inf gradient
     chi = prob.misfit_batch(X)
     g = prob.gradient_batch(X)
@@ -78,12 +78,15 @@ def test_sparse_misfit_gradient(precision):
     dtype = np.float64 if precision == "f64" else np.float32
     post, orc, rng = _dense_case(2, 400, 300, prior="normal", sparse=True, g_dtype=dtype)
     prob = build_problem(post, precision=precision)
-    X = rng.normal(size=(7, 300))
+    # positions are stored in the trajectory precision; give both sides the same x
+    X = rng.normal(size=(7, 300)).astype(dtype).astype(np.float64)
     chi = prob.misfit_batch(X)
     g = prob.gradient_batch(X)
     for b in range(7):
         x = X[b].reshape(-1, 1)
-        assert _rel(chi[b], orc.misfit(x)) < 1e-10         # endpoint misfit is FP64
+        # the endpoint misfit is FP64; in float32 storage the reference also rounds
+        # sigma = sqrt(var) to float32 (LinearMatrix.py:326-329), a 1e-7 effect
+        assert _rel(chi[b], orc.misfit(x)) < (1e-10 if precision == "f64" else 1e-6)
         assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
 
 
@@ -96,7 +99,7 @@ def test_bf16_gemm_gradient_and_fp64_misfit():
     orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
                        O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
     prob = build_problem(post, precision="bf16")
-    X = rng.normal(size=(3, d))
+    X = rng.normal(size=(3, d)).astype(np.float32).astype(np.float64)   # float32 state
     chi = prob.misfit_batch(X)
     g = prob.gradient_batch(X)
     for b in range(3):
@@ -106,12 +109,12 @@ def test_bf16_gemm_gradient_and_fp64_misfit():
 
 
 # --------------------------------------------------------------------------- #
-def _teacher(post, orc, mass_g, mass_o, rng, B, L, integ, precision, engine, eps=0.01):
+def _teacher(post, orc, mass_g, mass_o, rng, B, L, integ, precision, engine, eps=0.01, x_scale=0.3):
     d = post.dimensions
     prob = build_problem(post, precision=precision, engine=engine,
                          mass_diagonal=mass_g._device_diagonal())
     ch = Chains(prob, B)
-    X0 = rng.normal(size=(B, d)) * 0.3
+    X0 = rng.normal(size=(B, d)) * x_scale
     ch.set_state(X0)
     X0s, chi0 = ch.get_state()
     P0 = np.stack([mass_o.generate_momentum()[:, 0] for _ in range(B)])
@@ -172,7 +175,7 @@ def test_bounds_reflection_in_trajectory():
     post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.Uniform(lo, hi)])
     orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0), O.Uniform(lo, hi)])
     _teacher(post, orc, MM.Unit(d), O.Unit(d, np.random.default_rng(11)), rng, 5, 15, "lf", "f64",
-             "auto", eps=0.05)
+             "auto", eps=0.05, x_scale=0.05)
 
 
 # --------------------------------------------------------------------------- #
@@ -189,18 +192,26 @@ def _c1(seed=0):
 
 
 @pytest.mark.parametrize("engine", ["gemm", "fused"])
-@pytest.mark.parametrize("stepsize", [0.005, 0.05])
+@pytest.mark.parametrize("stepsize", [0.004, 0.05])
 def test_seeded_chain_matches_oracle(engine, stepsize):
     from hmclab_amd import Samplers
 
     G, dd, post, orc = _c1()
+    mu, _ = O.analytic_linear_gaussian(G, dd, 0.01, np.zeros(50), 1.0)
     sam = Samplers.HMC(seed=7)
     sam.sample(None, post, stepsize=stepsize, amount_of_steps=10, proposals=60, engine=engine,
-               disable_progressbar=True)
+               initial_model=mu, disable_progressbar=True)
     got = sam.results[0]
-    ref = O.sample_chain(orc, O.Unit(50), seed=7, stepsize=stepsize, amount_of_steps=10, proposals=60)
+    ref, rec, accepted = O.sample_chain(orc, O.Unit(50), seed=7, stepsize=stepsize, amount_of_steps=10,
+                                        proposals=60, initial_model=mu, record=True)
     assert got.shape == ref.shape
-    assert _rel(got, ref) < 1e-9
+    assert _rel(got[:-1], ref[:-1]) < 1e-9            # positions
+    assert _rel(got[-1], ref[-1]) < 1e-9              # misfits
+    assert sam.accepted_proposals == accepted
+    if stepsize < 0.01:
+        assert accepted > 20                          # the chain moves (stable step)
+    else:
+        assert accepted == 0                          # dt=0.05 exceeds the leapfrog stability limit
 
 
 @pytest.mark.parametrize("engine", ["gemm", "fused"])
@@ -213,7 +224,7 @@ def test_device_rng_posterior_matches_analytic(engine):
     mu, Sigma = O.analytic_linear_gaussian(G, dd, 0.09, np.zeros(d), 1.0)
     prob = build_problem(post, precision="f64", engine=engine)
     ch = Chains(prob, B)
-    ch.set_state(np.broadcast_to(mu[:, 0], (B, d)))
+    ch.set_state(np.broadcast_to(mu.reshape(1, d), (B, d)))
     lam = np.linalg.eigvalsh(np.linalg.inv(Sigma)).max()
     eps = 1.0 / np.sqrt(lam)
     ch.run(50, 8, eps, seed=3)                      # burn-in
@@ -224,7 +235,7 @@ def test_device_rng_posterior_matches_analytic(engine):
     assert acc.mean() / 200 > 0.5
     sd = np.sqrt(np.diag(Sigma))
     se = sd / np.sqrt(S.shape[0] / 10.0)
-    assert np.all(np.abs(S.mean(axis=0) - mu[:, 0]) < 5 * se)
+    assert np.all(np.abs(S.mean(axis=0) - mu) < 5 * se)
     C = np.cov(S.T)
     assert np.allclose(np.sqrt(np.diag(C)), sd, rtol=0.05)
     corr = C / np.outer(np.sqrt(np.diag(C)), np.sqrt(np.diag(C)))
@@ -235,14 +246,19 @@ def test_device_rng_posterior_matches_analytic(engine):
 def test_device_rng_is_reproducible_and_engine_independent():
     G, dd, post, orc = _c1(1)
     outs = []
+    G, dd, _, _ = _c1(1)
+    mu, _ = O.analytic_linear_gaussian(G, dd, 0.01, np.zeros(50), 1.0)
+    accs = []
     for engine in ("gemm", "fused", "fused"):
         prob = build_problem(post, precision="f64", engine=engine)
         ch = Chains(prob, 4)
-        ch.set_state(np.zeros((4, 50)))
+        ch.set_state(np.broadcast_to(mu.reshape(1, 50), (4, 50)))
         ch.set_trace(30)
-        ch.run(30, 10, 0.004, seed=99)
+        _, acc = ch.run(30, 10, 0.004, seed=99)
         outs.append(ch.read_trace(0, 30)[0])
+        accs.append(acc)
     assert np.array_equal(outs[1], outs[2])            # bitwise reproducible
+    assert np.array_equal(accs[0], accs[1]) and accs[0].sum() > 20
     assert _rel(outs[0], outs[1]) < 1e-9               # same draws in both engines
 
 
