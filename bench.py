@@ -72,29 +72,40 @@ def posterior_draws(torch, mu, L, B, seed):
     return (mu[:, None] + x).T.contiguous().cpu().numpy()
 
 
-def cpu_baseline(Gf, dd, sigma2, L, eps, proposals, seed):
-    """Reference-faithful single-chain proposals with the oracle on one core."""
-    try:
-        from threadpoolctl import threadpool_limits
-    except Exception:   # pragma: no cover
-        threadpool_limits = None
+def cpu_baseline_child(args):
+    """Runs in a fresh process with BLAS pinned to one thread: regenerate the
+    problem, then time single-chain oracle proposals (the reference's op
+    sequence: L gradients + 2 misfits, residual form, float64 G)."""
     from oracle import hmc_oracle as O
 
-    d = Gf.shape[1]
+    _, Gf, dd = make_dense_problem(args.seed, args.n, args.d, 0.01)
+    d = args.d
     G64 = Gf.astype(np.float64)
+    del Gf
     post = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
-                        O.LinearMatrixSimple(G64, dd, sigma2, dtype=np.float64, premultiplication=False)])
-    ctx = threadpool_limits(limits=1) if threadpool_limits else None
-    if ctx:
-        ctx.__enter__()
-    try:
-        t0 = time.perf_counter()
-        O.sample_chain(post, O.Unit(d), seed=seed, stepsize=eps, amount_of_steps=L, proposals=proposals)
-        dt = time.perf_counter() - t0
-    finally:
-        if ctx:
-            ctx.__exit__(None, None, None)
-    return proposals / dt, dt
+                        O.LinearMatrixSimple(G64, dd, 1e-4, dtype=np.float64, premultiplication=False)])
+    t0 = time.perf_counter()
+    O.sample_chain(post, O.Unit(d), seed=args.seed, stepsize=args.cpu_eps, amount_of_steps=args.steps_per_traj,
+                   proposals=args.cpu_proposals)
+    dt = time.perf_counter() - t0
+    print(json.dumps({"proposals_per_s": args.cpu_proposals / dt, "seconds": dt}), flush=True)
+
+
+def cpu_baseline(args, eps):
+    import subprocess
+
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "BLIS_NUM_THREADS"):
+        env[k] = "1"
+    env["HIP_VISIBLE_DEVICES"] = ""
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--seed", str(args.seed),
+           "--n", str(args.n), "--d", str(args.d), "--steps-per-traj", str(args.steps_per_traj),
+           "--cpu-proposals", str(args.cpu_proposals), "--cpu-eps", repr(float(eps))]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed: {out.stderr[-2000:]}")
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    return res["proposals_per_s"], res["seconds"]
 
 
 def trace_to_torch(torch, ch, K):
@@ -121,37 +132,57 @@ def run_dense(args):
     Gb, Gf, dd = make_dense_problem(args.seed, n, d, sigma)
     log(f"problem {n}x{d} generated in {time.perf_counter() - t_setup:.1f}s")
     mu, var, Lc, lam_max = analytic_posterior(torch, Gf, dd, sigma * sigma)
-    eps = args.eps_scale / np.sqrt(lam_max)
     X0 = posterior_draws(torch, mu, Lc, B, args.seed + 1)
     mu_np, sd_np = mu.cpu().numpy(), np.sqrt(var.cpu().numpy())
     del Lc
     torch.cuda.empty_cache()
-    log(f"analytic posterior: lam_max={lam_max:.4g} eps={eps:.4g} ({time.perf_counter() - t_setup:.1f}s)")
+    log(f"analytic posterior: lam_max={lam_max:.4g} ({time.perf_counter() - t_setup:.1f}s)")
 
     post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0),
                         D.LinearMatrix(Gf.astype(np.float64), dd, sigma * sigma, dtype="bfloat16")])
-    prob = build_problem(post, precision=args.precision, engine="gemm")
+    opts = {}
+    if args.precision == "bf16":
+        opts = {"bf16_split": args.bf16_split, "gemm_config": args.gemm_config}
+    prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)
     log(f"device problem ready ({time.perf_counter() - t_setup:.1f}s)")
 
+    # warm-up: tune one global step size towards acceptance 0.65 (multiplicative
+    # updates on the log scale), then run the remaining untimed proposals
+    eps = args.eps_scale / np.sqrt(lam_max)
+    done = 0
+    for _ in range(args.tune_rounds):
+        _, acc_t = ch.run(args.tune_props, L, np.full(B, eps), seed=args.seed, first_proposal=done)
+        done += args.tune_props
+        rate = acc_t.mean() / args.tune_props
+        eps *= float(np.exp(1.5 * (rate - 0.65)))
+        log(f"tune: acceptance {rate:.3f} -> eps {eps:.4g}")
     step = np.full(B, eps)
     if args.warmup:
-        ch.run(args.warmup, L, step, seed=args.seed, first_proposal=0)
+        ch.run(args.warmup, L, step, seed=args.seed, first_proposal=done)
+    done += args.warmup
     ch.set_trace(args.steps)
     ch.enable_timing(True)
     ch.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, acc = ch.run(args.steps, L, step, seed=args.seed, first_proposal=args.warmup)
+    _, acc = ch.run(args.steps, L, step, seed=args.seed, first_proposal=done)
     ch.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    gemm_ms, gemm_n = ch.timing(0)
+    res_ms, res_n = ch.timing(0)      # residual GEMM launches
+    leap_ms, leap_n = ch.timing(3)    # gradient GEMM launches with the kick/drift epilogue
     end_ms, end_n = ch.timing(1)
     ew_ms, ew_n = ch.timing(2)
-    log(f"timed {args.steps} proposals in {elapsed:.3f}s; gemm {gemm_ms:.1f} ms/{gemm_n}, "
-        f"endpoint {end_ms:.1f} ms/{end_n}, elementwise {ew_ms:.1f} ms/{ew_n}")
+    log(f"timed {args.steps} proposals in {elapsed:.3f}s; residual gemm {res_ms:.1f} ms/{res_n}, "
+        f"kick-drift gemm {leap_ms:.1f} ms/{leap_n}, endpoint {end_ms:.1f} ms/{end_n}, "
+        f"elementwise {ew_ms:.1f} ms/{ew_n}")
+    # the dominant kernel is the slower of the two trajectory GEMMs
+    if leap_ms >= res_ms:
+        gemm_ms, gemm_n, gemm_name = leap_ms, leap_n, "k_gemm_bf16<...,EpiLeap> (gradient GEMM + kick/drift)"
+    else:
+        gemm_ms, gemm_n, gemm_name = res_ms, res_n, "k_gemm_bf16<...,EpiResid> (residual GEMM)"
 
     draws = trace_to_torch(torch, ch, args.steps)
     ess = diagnostics.ess_bulk(draws)
@@ -201,43 +232,214 @@ def run_dense(args):
         },
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "k_gemm_bf16 (trajectory GEMMs)", "avg_launch_us": avg_s * 1e6},
+                     "kernel": gemm_name, "avg_launch_us": avg_s * 1e6,
+                     "other_gemm_avg_us": (min(res_ms, leap_ms) / 1e3 / max(gemm_n, 1)) * 1e6,
+                     "endpoint_fp64_avg_us": end_ms / max(end_n, 1) * 1e3},
     }
 
     if args.cpu_baseline:
         ess_per_prop_chain = min_ess / (args.steps * B)
-        props_s, dt = cpu_baseline(Gf, dd, sigma * sigma, L, eps, args.cpu_proposals, args.seed)
+        props_s, dt = cpu_baseline(args, eps)
         result["cpu_baseline"] = {
             "value": ess_per_prop_chain * props_s, "unit": "ESS/s", "cores": 1, "kind": "port",
-            "sample": f"{args.cpu_proposals} proposals of one chain (oracle, float64 G, residual form, "
-                      f"L={L}) in {dt:.1f}s = {props_s:.4g} proposals/s, times the GPU's measured "
-                      f"min-ESS per proposal per chain {ess_per_prop_chain:.4g}",
+            "sample": f"{args.cpu_proposals} proposals of one chain (oracle in a separate process with "
+                      f"BLAS pinned to 1 thread, float64 G, residual form, L={L}) in {dt:.1f}s = "
+                      f"{props_s:.4g} proposals/s, times the GPU's measured min-ESS per proposal per "
+                      f"chain {ess_per_prop_chain:.4g}",
         }
     ch.close()
     prob.close()
     return result
 
 
+def _tune(ch, B, L, eps, args, seed, log_prefix=""):
+    """Warm-up: multiplicative step tuning towards acceptance 0.65, then W
+    untimed proposals.  Returns (eps, proposals done)."""
+    done = 0
+    for _ in range(args.tune_rounds):
+        _, acc_t = ch.run(args.tune_props, L, np.full(B, eps), seed=seed, first_proposal=done)
+        done += args.tune_props
+        rate = acc_t.mean() / args.tune_props
+        eps *= float(np.exp(1.5 * (rate - 0.65)))
+        log(f"{log_prefix}tune: acceptance {rate:.3f} -> eps {eps:.4g}")
+    if args.warmup:
+        ch.run(args.warmup, L, np.full(B, eps), seed=seed, first_proposal=done)
+    return eps, done + args.warmup
+
+
+def _timed(ch, B, L, eps, steps, seed, first, torch=None):
+    ch.set_trace(steps)
+    ch.enable_timing(True)
+    ch.sync()
+    if torch is not None:
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, acc = ch.run(steps, L, np.full(B, eps), seed=seed, first_proposal=first)
+    ch.sync()
+    if torch is not None:
+        torch.cuda.synchronize()
+    return time.perf_counter() - t0, acc
+
+
+def run_tomography(args):
+    """Batched straight-ray tomography (64x64 cells, 2000 rays, CSR G) on the
+    fused engine: one kernel per proposal per tile of chains."""
+    import torch
+
+    from hmclab_amd import Distributions as D
+    from hmclab_amd import MassMatrices as MM
+    from hmclab_amd import diagnostics, tomography
+    from hmclab_amd.Distributions import build_problem
+    from hmclab_amd.engine import Chains
+
+    B, L = args.chains, args.steps_per_traj
+    prob_d = tomography.synthetic_problem(seed=args.seed)
+    G, dd = prob_d["G"], prob_d["d"]
+    n, d = G.shape
+    nnz = G.nnz
+    sd2, sm2 = prob_d["noise_std"] ** 2, prob_d["prior_std"] ** 2
+    Gt = torch.from_numpy(G.toarray()).to("cuda", torch.float64)
+    H = Gt.T @ Gt / sd2
+    H.diagonal().add_(1.0 / sm2)
+    rhs = Gt.T @ torch.from_numpy(dd[:, 0]).to("cuda", torch.float64) / sd2 + prob_d["prior_mean"] / sm2
+    Lc = torch.linalg.cholesky(H)
+    mu = torch.cholesky_solve(rhs[:, None], Lc)[:, 0]
+    Linv = torch.linalg.solve_triangular(Lc, torch.eye(d, device="cuda", dtype=torch.float64), upper=False)
+    sd = torch.sqrt((Linv * Linv).sum(dim=0)).cpu().numpy()
+    mass = H.diagonal().cpu().numpy().copy()          # Diagonal mass = diag of the posterior precision
+    Ms = torch.from_numpy(1.0 / np.sqrt(mass)).to("cuda")
+    lam_max = torch.linalg.eigvalsh(Ms[:, None] * H * Ms[None, :])[-1].item()
+    X0 = posterior_draws(torch, mu, Lc, B, args.seed + 1)
+    mu_np = mu.cpu().numpy()
+    del Gt, H, Linv, Lc
+    torch.cuda.empty_cache()
+    dtype = np.float32 if args.precision == "f32" else np.float64
+    post = D.BayesRule([D.Normal(np.full((d, 1), prob_d["prior_mean"]), sm2),
+                        D.LinearMatrix(G.astype(dtype), dd.astype(dtype) if dtype == np.float32 else dd,
+                                       sd2, dtype=dtype if dtype == np.float32 else None)])
+    prob = build_problem(post, precision=args.precision, mass_diagonal=mass)
+    ch = Chains(prob, B)
+    ch.set_state(X0)
+    eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), args, args.seed)
+    elapsed, acc = _timed(ch, B, L, eps, args.steps, args.seed, done, torch)
+    k_ms, k_n = ch.timing(0)
+    draws = trace_to_torch(torch, ch, args.steps)
+    ess = diagnostics.ess_bulk(draws)
+    pooled = draws.to(torch.float64).reshape(-1, d)
+    mean, std = pooled.mean(dim=0).cpu().numpy(), pooled.std(dim=0).cpu().numpy()
+    del draws, pooled
+    sx = 4 if args.precision == "f32" else 8
+    sv = sx
+    step_bytes = nnz * (sv + 4) + (n + 1) * 4 + 4 * d * B * sx
+    launch_bytes = L * step_bytes + (nnz * (sv + 4) + d * B * sx) + d * B * sx
+    avg_s = k_ms / 1e3 / max(k_n, 1)
+    achieved = launch_bytes / avg_s
+    result = {
+        "metric": METRIC, "value": float(np.min(ess)) / elapsed, "unit": "ESS/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": done, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"straight-ray tomography 64x64 cells, {n} rays (nnz {nnz}), {B} chains, "
+                               f"L={L}, Diagonal mass, fused sparse engine",
+                   "min_ess": float(np.min(ess)), "acceptance_rate": float(acc.mean() / args.steps),
+                   "stepsize": eps, "proposals_per_s": args.steps / elapsed,
+                   "posterior_rms_mean_err_over_sd": float(np.sqrt(np.mean(((mean - mu_np) / sd) ** 2))),
+                   "posterior_rms_std_rel_err": float(np.sqrt(np.mean((std / sd - 1.0) ** 2)))},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK["hbm"], "traffic": None,
+                     "kernel": "k_fused_propose<sparse> (whole proposal)", "avg_launch_us": avg_s * 1e6,
+                     "algorithmic_bytes_per_launch": launch_bytes},
+    }
+    ch.close()
+    prob.close()
+    return result
+
+
+def run_small(args):
+    """The 1-chain dense case (G 100x50): latency per proposal on the fused
+    engine at the stated step 0.05 and ESS/s at a tuned step."""
+    import torch
+
+    from hmclab_amd import Distributions as D
+    from hmclab_amd import diagnostics
+    from hmclab_amd.Distributions import build_problem
+    from hmclab_amd.engine import Chains
+    from oracle import hmc_oracle as O
+
+    rng = np.random.default_rng(args.seed)
+    G = rng.normal(size=(100, 50))
+    dd = G @ rng.normal(size=(50, 1)) + rng.normal(0, 0.1, size=(100, 1))
+    post = D.BayesRule([D.Normal(np.zeros((50, 1)), 1.0), D.LinearMatrix(G, dd, 0.01)])
+    mu, Sigma = O.analytic_linear_gaussian(G, dd, 0.01, np.zeros(50), 1.0)
+    prob = build_problem(post, precision="f64")
+    ch = Chains(prob, 1)
+    ch.set_state(mu.reshape(1, 50))
+    L = 10
+    # stated step 0.05: latency only (it exceeds the stability limit 2/sqrt(lam_max))
+    ch.enable_timing(True)
+    t0 = time.perf_counter()
+    ch.run(args.steps, L, np.full(1, 0.05), seed=args.seed)
+    ch.sync()
+    lat_stated = (time.perf_counter() - t0) / args.steps
+    eps0 = args.eps_scale / np.sqrt(np.linalg.eigvalsh(np.linalg.inv(Sigma)).max())
+    eps, done = _tune(ch, 1, L, eps0, args, args.seed)
+    elapsed, acc = _timed(ch, 1, L, eps, args.steps, args.seed, done + args.steps, torch)
+    k_ms, k_n = ch.timing(0)
+    X, _ = ch.read_trace(0, args.steps)
+    ess = diagnostics.ess_bulk(X.transpose(1, 0, 2))
+    result = {
+        "metric": METRIC, "value": float(np.min(ess)) / elapsed, "unit": "ESS/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": done, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "dense linear-Gaussian 100x50, 1 chain, L=10, fused engine",
+                   "us_per_proposal_tuned": elapsed / args.steps * 1e6,
+                   "us_per_proposal_dt0.05": lat_stated * 1e6, "stepsize": eps,
+                   "acceptance_rate": float(acc.mean() / args.steps), "min_ess": float(np.min(ess))},
+        "roofline": {"bound": "hbm", "achieved": None, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+                     "frac": None, "traffic": None, "kernel": "k_fused_propose<dense> (latency-bound)",
+                     "avg_launch_us": k_ms / max(k_n, 1) * 1e3},
+    }
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small"])
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=20000)
     ap.add_argument("--d", type=int, default=8192)
     ap.add_argument("--chains", type=int, default=512)
     ap.add_argument("--steps-per-traj", type=int, default=25)
-    ap.add_argument("--eps-scale", type=float, default=1.0)
-    ap.add_argument("--precision", default="bf16", choices=["bf16", "f64"])
+    ap.add_argument("--eps-scale", type=float, default=0.3)
+    ap.add_argument("--tune-rounds", type=int, default=6)
+    ap.add_argument("--tune-props", type=int, default=3)
+    ap.add_argument("--cpu-baseline-child", action="store_true")
+    ap.add_argument("--cpu-eps", type=float, default=1e-3)
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "f64", "f32"])
+    ap.add_argument("--bf16-split", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--gemm-config", type=int, default=1, choices=[0, 1])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-proposals", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); chains shard across GPUs "
               "without communication, so the one-GPU figure is the per-GPU rate.", file=sys.stderr)
         sys.exit(2)
-    res = run_dense(args)
+    if args.workload == "tomography":
+        if args.precision == "bf16":
+            args.precision = "f32"
+        res = run_tomography(args)
+    elif args.workload == "small":
+        res = run_small(args)
+    else:
+        res = run_dense(args)
     print(json.dumps(res), flush=True)
 
 

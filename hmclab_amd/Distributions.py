@@ -431,11 +431,13 @@ class _Spec:
 
 
 def build_problem(dist: _AbstractDistribution, precision: str = "f64", engine: str = "auto",
-                  mass_diagonal=None) -> Problem:
+                  mass_diagonal=None, options: dict | None = None) -> Problem:
     """Compile a target into a device problem."""
     spec = _Spec()
     dist._contribute(spec)
     prob = Problem(dist.dimensions, precision=precision, engine=engine)
+    for k, v in (options or {}).items():
+        prob.set_option(k, v)
     lik = spec.likelihood
     if lik is not None:
         G, d, var = lik._whitened()

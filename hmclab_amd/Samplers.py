@@ -379,7 +379,8 @@ class HMC:
                             else np.empty((B, d + 1, 0)))
         if self.diagnostic_mode:
             total = (self.end_time - self.start_time).total_seconds()
-            names = ["trajectory gradient kernels", "endpoint misfit", "element-wise / Metropolis"]
+            names = ["trajectory gradient kernels", "endpoint misfit", "element-wise / Metropolis",
+                     "trajectory kick/drift GEMM"]
             self.diagnostics = {}
             print("Detailed statistics:")
             print(f"Total runtime: {total:.2f} seconds")

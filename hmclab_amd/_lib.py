@@ -61,6 +61,7 @@ SIGNATURES = {
     "hmc_problem_set_bounds": [_p, _dp, _dp],
     "hmc_problem_set_mass": [_p, _i32, _dp],
     "hmc_problem_set_engine": [_p, _i32],
+    "hmc_problem_set_option": [_p, C.c_char_p, C.c_double],
     "hmc_problem_destroy": [_p],
     "hmc_misfit_batch": [_p, _dp, _i64, _dp],
     "hmc_gradient_batch": [_p, _dp, _i64, _dp],

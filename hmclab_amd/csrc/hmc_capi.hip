@@ -448,6 +448,23 @@ int hmc_problem_set_engine(hmc_problem* hp, int engine) {
   });
 }
 
+int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
+  return guarded("hmc_problem_set_option", [&] {
+    HMC_REQUIRE(hp && name, HMC_ERR_ARG, "NULL argument");
+    Problem& P = hp->p;
+    const std::string key(name);
+    if (key == "bf16_split") {
+      HMC_REQUIRE(value == 1.0 || value == 2.0, HMC_ERR_ARG, "bf16_split must be 1 or 2");
+      P.bf16_split = (int)value;
+    } else if (key == "gemm_config") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_config must be 0 or 1");
+      P.gemm_cfg = (int)value;
+    } else {
+      throw Error{HMC_ERR_ARG, "unknown option " + key};
+    }
+  });
+}
+
 int hmc_problem_destroy(hmc_problem* hp) {
   return guarded("hmc_problem_destroy", [&] {
     if (!hp) return;
@@ -738,7 +755,7 @@ int hmc_chains_enable_timing(hmc_chains* hc, int on) {
     Chains& c = hc->c;
     time_collect(c);
     c.timing = on != 0;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       c.t_ms[k] = 0.0;
       c.t_n[k] = 0;
     }
@@ -747,7 +764,7 @@ int hmc_chains_enable_timing(hmc_chains* hc, int on) {
 
 int hmc_chains_timing(hmc_chains* hc, int kind, double* total_ms, int64_t* launches) {
   return guarded("hmc_chains_timing", [&] {
-    HMC_REQUIRE(hc && kind >= 0 && kind < 3, HMC_ERR_ARG, "invalid argument");
+    HMC_REQUIRE(hc && kind >= 0 && kind < 4, HMC_ERR_ARG, "invalid argument");
     Chains& c = hc->c;
     time_collect(c);
     if (total_ms) *total_ms = c.t_ms[kind];

@@ -198,19 +198,51 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
 // ---------------------------------------------------------------------------
 // BF16 MFMA GEMM (v_mfma_f32_32x32x16_bf16, gfx950): C = A * (B0 + B1)^T
 // with NSPLIT = 1 or 2 bf16 planes of B (hi/lo split of an FP32 operand),
-// FP32 accumulation.  Block tile BM x BN, k-tile 64, wave tile WM x WN.
-// K permutation: lane group q (0/1) at MFMA step s uses k = 32q + 8s + e.
+// FP32 accumulation.  Block tile BM x BN, k-tile 64 (one 128-byte row per
+// operand row), wave tile WM x WN.
+//
+// Operand tiles stream global -> LDS with gfx950's direct-to-LDS loads
+// (global_load_lds_dwordx4: each wavefront instruction fills 1 KB = 8 rows)
+// through an S-stage ring, so S-1 k-tiles are in flight without register
+// staging; each wave waits only for its own loads of the tile it is about to
+// use (s_waitcnt vmcnt), then one workgroup barrier per k-tile.  Rows are
+// stored with their 16-byte chunks XOR-swizzled by (row & 7) so the
+// ds_read_b128 fragment loads of 32 consecutive rows are bank-conflict free.
+// K permutation: lane group q (0/1) at MFMA step s uses k = 32q + 8s + e, i.e.
+// logical chunk 4q + s of the row.
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int WM, int WN, int NSPLIT, class Epi>
+#define HMC_LDS __attribute__((address_space(3)))
+
+// s_waitcnt vmcnt(n) leaving expcnt / lgkmcnt unconstrained (gfx9 encoding)
+template <int N>
+__device__ inline void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt out of range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+// workgroup barrier without the release/acquire fences of __syncthreads():
+// LDS-DMA completion is ordered explicitly with wait_vmcnt.
+__device__ inline void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
     const uint16_t* __restrict__ A, int64_t lda, const uint16_t* __restrict__ B0,
     const uint16_t* __restrict__ B1, int64_t ldb, int M, int N, int K, Epi epi) {
-  constexpr int BK = 64, LDS = BK + 8;  // elements; row stride 144 B
-  constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  constexpr int BK = 64;                 // elements per row per k-tile (128 bytes)
+  constexpr int NW = (BM / WM) * (BN / WN);
   constexpr int TM = WM / 32, TN = WN / 32;
   constexpr int WPR = BN / WN;
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[NSPLIT][BN * LDS];
+  constexpr int A_GROUPS = BM / 8, B_GROUPS = NSPLIT * BN / 8;      // 1 KB groups per stage
+  constexpr int GROUPS = A_GROUPS + B_GROUPS;
+  static_assert(GROUPS % NW == 0, "load groups must divide over waves");
+  constexpr int GPW = GROUPS / NW;       // load instructions per wave per k-tile
+  constexpr int STAGE = (BM + NSPLIT * BN) * BK;   // elements per stage
+  static_assert(GPW * (S - 2) < 64, "vmcnt range");
+  __shared__ __attribute__((aligned(1024))) uint16_t smem[S * STAGE];
 
   int tm, tn;
   if (!map_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn)) return;
@@ -220,36 +252,32 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
   const int wm = w / WPR, wn = w % WPR;
   const int q = lane >> 5, r = lane & 31;
 
-  constexpr int A_CH = BM * 8 / NT;  // 16-byte chunks per thread: rows*8 chunks per row
-  constexpr int B_CH = BN * 8 / NT;
-  static_assert(A_CH >= 1 && B_CH >= 1, "tile too small for thread count");
-
-  u32x4 ra[A_CH], rb0[B_CH], rb1[B_CH];
-#define HMC_BF_GLOAD(kt)                                                                   \
-  do {                                                                                     \
-    _Pragma("unroll") for (int i = 0; i < A_CH; ++i) {                                     \
-      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
-      ra[i] = *reinterpret_cast<const u32x4*>(A + (m_blk + row_) * lda + (int64_t)(kt)*BK + col_); \
-    }                                                                                      \
-    _Pragma("unroll") for (int i = 0; i < B_CH; ++i) {                                     \
-      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
-      const int64_t off_ = (n_blk + row_) * ldb + (int64_t)(kt)*BK + col_;                 \
-      rb0[i] = *reinterpret_cast<const u32x4*>(B0 + off_);                                 \
-      if constexpr (NSPLIT == 2) rb1[i] = *reinterpret_cast<const u32x4*>(B1 + off_);      \
-    }                                                                                      \
-  } while (0)
-#define HMC_BF_SSTORE()                                                                    \
-  do {                                                                                     \
-    _Pragma("unroll") for (int i = 0; i < A_CH; ++i) {                                     \
-      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
-      *reinterpret_cast<u32x4*>(&As[row_ * LDS + col_]) = ra[i];                           \
-    }                                                                                      \
-    _Pragma("unroll") for (int i = 0; i < B_CH; ++i) {                                     \
-      const int c_ = tid + i * NT, row_ = c_ >> 3, col_ = (c_ & 7) * 8;                    \
-      *reinterpret_cast<u32x4*>(&Bs[0][row_ * LDS + col_]) = rb0[i];                       \
-      if constexpr (NSPLIT == 2) *reinterpret_cast<u32x4*>(&Bs[NSPLIT - 1][row_ * LDS + col_]) = rb1[i]; \
-    }                                                                                      \
-  } while (0)
+  // per-lane part of the load address: row within group, swizzled chunk
+  const int lrow = lane >> 3;
+  auto issue = [&](int kt, int st) {
+    uint16_t* stage = smem + st * STAGE;
+#pragma unroll
+    for (int i = 0; i < GPW; ++i) {
+      const int g = w * GPW + i;
+      const uint16_t* src;
+      uint16_t* dst;
+      if (g < A_GROUPS) {
+        const int row = g * 8 + lrow;
+        const int c = (lane & 7) ^ (row & 7);
+        src = A + (m_blk + row) * lda + (int64_t)kt * BK + c * 8;
+        dst = stage + g * 8 * BK;
+      } else {
+        const int gb = g - A_GROUPS;
+        const int sp = gb / (BN / 8), gr = gb % (BN / 8);
+        const int row = gr * 8 + lrow;
+        const int c = (lane & 7) ^ (row & 7);
+        const uint16_t* Bp = (NSPLIT == 2 && sp == 1) ? B1 : B0;
+        src = Bp + (n_blk + row) * ldb + (int64_t)kt * BK + c * 8;
+        dst = stage + BM * BK + sp * BN * BK + gr * 8 * BK;
+      }
+      __builtin_amdgcn_global_load_lds((const void*)src, (HMC_LDS void*)dst, 16, 0, 0);
+    }
+  };
 
   floatx16 acc[TM][TN];
 #pragma unroll
@@ -260,31 +288,47 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
   const int nk = (K + BK - 1) / BK;
-  HMC_BF_GLOAD(0);
+#pragma unroll
+  for (int s0 = 0; s0 < S - 1; ++s0)
+    if (s0 < nk) issue(s0, s0);
   for (int kt = 0; kt < nk; ++kt) {
-    __syncthreads();
-    HMC_BF_SSTORE();
-    __syncthreads();
-    HMC_BF_GLOAD(kt + 1 < nk ? kt + 1 : kt);
+    // this wave's loads of tile kt are done once at most (S-2) later tiles are pending
+    if (kt + S - 2 < nk) wait_vmcnt<GPW * (S - 2)>();
+    else wait_vmcnt<0>();
+    lds_barrier();
+    if (kt + S - 1 < nk) issue(kt + S - 1, (kt + S - 1) % S);
+    const uint16_t* As = smem + (kt % S) * STAGE;
+    const uint16_t* Bs = As + BM * BK;
+    // fragments of k-step s+1 are read from LDS while the MFMAs of step s run
+    bf16x8 af[2][TM], bfr[2][NSPLIT][TN];
+    auto load_frags = [&](int s, int buf) {
+      const int c = 4 * q + s;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      bf16x8 af[TM], bfr[NSPLIT][TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-        af[i] = *reinterpret_cast<const bf16x8*>(&As[(wm * WM + i * 32 + r) * LDS + 32 * q + 8 * s]);
+      for (int i = 0; i < TM; ++i) {
+        const int row = wm * WM + i * 32 + r;
+        af[buf][i] = *reinterpret_cast<const bf16x8*>(&As[row * BK + ((c ^ (row & 7)) << 3)]);
+      }
 #pragma unroll
       for (int sp = 0; sp < NSPLIT; ++sp)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-          bfr[sp][j] =
-              *reinterpret_cast<const bf16x8*>(&Bs[sp][(wn * WN + j * 32 + r) * LDS + 32 * q + 8 * s]);
+        for (int j = 0; j < TN; ++j) {
+          const int row = wn * WN + j * 32 + r;
+          bfr[buf][sp][j] =
+              *reinterpret_cast<const bf16x8*>(&Bs[sp * BN * BK + row * BK + ((c ^ (row & 7)) << 3)]);
+        }
+    };
+    load_frags(0, 0);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (s < 3) load_frags(s + 1, (s + 1) & 1);
 #pragma unroll
       for (int sp = 0; sp < NSPLIT; ++sp)
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[sp][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[s & 1][i], bfr[s & 1][sp][j],
+                                                                acc[i][j], 0, 0, 0);
     }
   }
 
@@ -300,8 +344,6 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
         float v[4] = {acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]};
         epi.apply(m0, 1, n, v);
       }
-#undef HMC_BF_GLOAD
-#undef HMC_BF_SSTORE
 }
 
 // ---------------------------------------------------------------------------
@@ -432,8 +474,9 @@ struct EpiMisfit {
 // ---------------------------------------------------------------------------
 // FP64 tiles: 128 x 128, 4 waves of 64 x 64.
 constexpr int F64_BM = 128, F64_BN = 128, F64_WM = 64, F64_WN = 64;
-// BF16 tiles: 128 x 128, 4 waves of 64 x 64.
-constexpr int BF_BM = 128, BF_BN = 128, BF_WM = 64, BF_WN = 64;
+// BF16 tiles: 128 x 128 with a 3-stage LDS ring (144 KB); config 0 = 4 waves
+// of 64 x 64, config 1 = 8 waves of 64 x 32 (two waves per SIMD).
+constexpr int BF_BM = 128, BF_BN = 128, BF_STAGES = 3;
 
 template <typename TA, typename TB, class Epi>
 static void launch_f64(hipStream_t s, const TA* A, int64_t lda, const TB* Bt, int64_t ldb, int M,
@@ -445,14 +488,26 @@ static void launch_f64(hipStream_t s, const TA* A, int64_t lda, const TB* Bt, in
   HMC_HIP(hipGetLastError());
 }
 
-template <int NSPLIT, class Epi>
-static void launch_bf16(hipStream_t s, const uint16_t* A, int64_t lda, const uint16_t* B0,
-                        const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+template <int WM, int WN, int NSPLIT, class Epi>
+static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const uint16_t* B0,
+                            const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
   const int gm = (M + BF_BM - 1) / BF_BM, gn = (N + BF_BN - 1) / BF_BN;
-  constexpr int NT = (BF_BM / BF_WM) * (BF_BN / BF_WN) * 64;
-  hipLaunchKernelGGL((k_gemm_bf16<BF_BM, BF_BN, BF_WM, BF_WN, NSPLIT, Epi>), dim3(grid_for(gm, gn)),
-                     dim3(NT), 0, s, A, lda, B0, B1, ldb, M, N, K, epi);
+  constexpr int NT = (BF_BM / WM) * (BF_BN / WN) * 64;
+  hipLaunchKernelGGL((k_gemm_bf16<BF_BM, BF_BN, WM, WN, NSPLIT, BF_STAGES, Epi>),
+                     dim3(grid_for(gm, gn)), dim3(NT), 0, s, A, lda, B0, B1, ldb, M, N, K, epi);
   HMC_HIP(hipGetLastError());
+}
+
+template <class Epi>
+static void launch_bf16(const Problem& p, const uint16_t* A, int64_t lda, const uint16_t* B0,
+                        const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+  if (p.gemm_cfg == 0) {
+    if (p.bf16_split == 2) launch_bf16_cfg<64, 64, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+    else launch_bf16_cfg<64, 64, 1>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+  } else {
+    if (p.bf16_split == 2) launch_bf16_cfg<64, 32, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+    else launch_bf16_cfg<64, 32, 1>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+  }
 }
 
 int64_t gemm_row_pad() { return 128; }   // M-tile of every GEMM config
@@ -464,22 +519,28 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2) {
   Problem& p = *c.p;
   const int n = (int)p.n, d = (int)p.d, B = (int)c.B;
   const DevModel md = p.model();
-  hipEvent_t ev;
-  time_begin(c, 0, ev);
+  hipEvent_t ev1, ev2;
   if (p.precision == HMC_BF16) {
     EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
-    launch_bf16<2>(p.stream, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    time_begin(c, 0, ev1);
+    launch_bf16(p, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    time_end(c, 0, ev1);
     EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, c.xl, c.ldx, c.eps, kick, drift, drift2, md, d, B};
-    launch_bf16<2>(p.stream, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    time_begin(c, 3, ev2);
+    launch_bf16(p, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    time_end(c, 3, ev2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
+    time_begin(c, 0, ev1);
     launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)c.xt, c.ldx, n, B, (int)p.ldg, e1);
+    time_end(c, 0, ev1);
     EpiLeap<double> e2{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, kick, drift, drift2, md, d, B};
+    time_begin(c, 3, ev2);
     launch_f64(p.stream, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
+    time_end(c, 3, ev2);
   } else {
     throw Error{HMC_ERR_UNSUPPORTED, "GEMM engine supports float64 and bfloat16 trajectories"};
   }
-  time_end(c, 0, ev);
 }
 
 // Endpoint data misfit in FP64: partial sums of (G x - d)^2 / var per row block.
@@ -517,9 +578,9 @@ void gemm_gradient(Chains& c, const void* X, double* out) {
   if (p.precision == HMC_BF16) {
     // hi/lo planes of X were prepared by the caller into c.xh / c.xl
     EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
-    launch_bf16<2>(p.stream, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    launch_bf16(p, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
     EpiGrad<float> e2{(const float*)X, out, c.ldx, c.oob, md, d, B};
-    launch_bf16<2>(p.stream, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_bf16(p, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
     launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e1);

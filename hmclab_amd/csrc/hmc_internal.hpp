@@ -28,6 +28,8 @@ struct Problem {
   int state_dtype = HMC_F64;// storage of positions/momenta (F64 or F32)
   int op = 0;               // 0 none, 1 dense, 2 csr
   int engine = HMC_ENGINE_AUTO;
+  int bf16_split = 2;       // bf16 planes of the FP32 GEMM operand (1 or 2)
+  int gemm_cfg = 1;         // bf16 GEMM tile configuration
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 
   // dense (row-major, zero padded): G [n_pad][ldg], Gt [d_pad][ldgt]
@@ -105,8 +107,8 @@ struct Chains {
 
   // timing
   bool timing = false;
-  double t_ms[3] = {0, 0, 0};
-  int64_t t_n[3] = {0, 0, 0};
+  double t_ms[4] = {0, 0, 0, 0};
+  int64_t t_n[4] = {0, 0, 0, 0};
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   size_t ev_next = 0;

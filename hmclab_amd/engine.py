@@ -40,6 +40,10 @@ class Problem:
         if engine != "auto":
             call("hmc_problem_set_engine", self._h, _lib.ENGINES[engine])
 
+    def set_option(self, name: str, value: float):
+        """Engine tuning knob (see hmc_problem_set_option)."""
+        call("hmc_problem_set_option", self._h, name.encode(), float(value))
+
     # -- likelihood ------------------------------------------------------------
     def set_dense(self, G, d_obs, variance, g_dtype: str | None = None):
         G = np.asarray(G)

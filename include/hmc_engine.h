@@ -115,6 +115,14 @@ int hmc_problem_set_mass(hmc_problem* p, int kind, const double* diagonal);
 /* Choose the trajectory engine (HMC_ENGINE_*). */
 int hmc_problem_set_engine(hmc_problem* p, int engine);
 
+/* Engine tuning knobs (no counterpart in the reference):
+ *   "bf16_split"  1 or 2: bf16 planes of the FP32 operand of the trajectory
+ *                 GEMMs (2 = hi/lo split, ~16-bit operand precision; default 2)
+ *   "gemm_config" 0 or 1: bf16 GEMM tile (0 = 128x128, 4 waves of 64x64;
+ *                 1 = 128x128, 8 waves of 64x32; default 1)
+ * Must be set before hmc_chains_create. */
+int hmc_problem_set_option(hmc_problem* p, const char* name, double value);
+
 int hmc_problem_destroy(hmc_problem* p);
 
 /* Batched misfit chi(m) for B models M (B x dimensions) -> out[B].
@@ -178,10 +186,12 @@ int hmc_chains_read_trace(hmc_chains* c, int64_t first, int64_t count, double* X
 int hmc_chains_trace_device(hmc_chains* c, void** x_ptr, double** chi_ptr, int* dtype);
 int hmc_chains_reset_trace(hmc_chains* c);
 
-/* Accumulated kernel times since the last reset, from HIP events on the
- * engine's stream.  kind: 0 = trajectory gradient kernels (the dominant
- * kernel), 1 = endpoint misfit, 2 = element-wise/metropolis.  total_ms and
- * launches may be NULL.  Timing is off until hmc_chains_enable_timing(c,1). */
+/* Accumulated kernel times since the last reset, from HIP events recorded on
+ * the engine's stream around each launch.  kind: 0 = first trajectory pass
+ * (GEMM engine: residual GEMM; fused engine: the whole-proposal kernel),
+ * 1 = endpoint misfit, 2 = element-wise / Metropolis, 3 = second trajectory
+ * pass (GEMM engine: gradient GEMM with the kick/drift epilogue).  total_ms
+ * and launches may be NULL.  Timing is off until hmc_chains_enable_timing(c,1). */
 int hmc_chains_enable_timing(hmc_chains* c, int on);
 int hmc_chains_timing(hmc_chains* c, int kind, double* total_ms, int64_t* launches);
 
