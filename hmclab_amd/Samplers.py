@@ -19,6 +19,14 @@ Random numbers:
 * ``rng="device"`` (default for several chains) draws the same quantities on
   the GPU with a Philox4x32-10 counter RNG keyed by ``seed``; whole chunks of
   proposals then run without host round trips.
+
+Learned surrogate (``surrogate=``, a trained :class:`hmclab_amd.surrogate.Surrogate`
+of the likelihood misfit): its gradient drives the trajectory while every
+Metropolis test uses the exact FP64 Hamiltonian.  When the acceptance rate of
+a chunk of proposals (device RNG) or of the last ``surrogate_window``
+proposals (host RNG) falls below ``surrogate_fallback``, the sampler detaches
+the surrogate and continues with the exact gradient;
+``surrogate_fallback_proposal`` records where (None if it never happened).
 """
 from __future__ import annotations
 
@@ -65,7 +73,8 @@ class HMC:
                autotuning: bool = False, target_acceptance_rate: float = 0.65,
                learning_rate: float = 0.75, queue=None, disable_progressbar=False, *,
                chains: int = 1, precision: str = "f64", engine: str = "auto", rng: str = None,
-               seeds=None, chunk: int = None):
+               seeds=None, chunk: int = None, surrogate=None, surrogate_fallback: float = 0.5,
+               surrogate_window: int = 20):
         """Sample ``distribution``; see the module docstring for the extras.
 
         ``samples_filename``: a ``.npy`` path (one chain), a list of paths (one
@@ -79,6 +88,7 @@ class HMC:
                    target_acceptance_rate, learning_rate, disable_progressbar, chains, precision,
                    engine, rng, seeds, chunk)
         try:
+            self._attach_surrogate(surrogate, surrogate_fallback, surrogate_window)
             self._loop()
         finally:
             self._close()
@@ -239,6 +249,28 @@ class HMC:
                     w.write_attribute(k, v)
 
     # ------------------------------------------------------------------ #
+    def _attach_surrogate(self, surrogate, fallback, window):
+        self.surrogate = surrogate
+        self.surrogate_fallback = float(fallback)
+        self.surrogate_fallback_proposal = None
+        self._sur_window = int(window)
+        self._sur_recent = []
+        if surrogate is not None:
+            assert 0.0 <= self.surrogate_fallback < 1.0, "surrogate_fallback must be in [0, 1)"
+            assert self._sur_window > 0, "surrogate_window must be positive"
+            self._chains.set_surrogate(surrogate)
+
+    def _surrogate_check(self, accepted_fraction, proposals_done):
+        """Fallback to the exact gradient when the acceptance rate is too low."""
+        if self.surrogate is None or self.surrogate_fallback_proposal is not None:
+            return
+        if accepted_fraction < self.surrogate_fallback:
+            self._chains.set_surrogate(None)
+            self.surrogate_fallback_proposal = int(proposals_done)
+            warnings.warn(f"surrogate acceptance {accepted_fraction:.3f} < {self.surrogate_fallback}: "
+                          f"continuing with the exact gradient after proposal {proposals_done}")
+
+    # ------------------------------------------------------------------ #
     def _write(self, X, chi):
         """Record one thinned sample per chain: X (B, d), chi (B,)."""
         cols = np.concatenate([X, chi[:, None]], axis=1)  # (B, d+1)
@@ -266,7 +298,11 @@ class HMC:
             if self.rng_mode == "host":
                 for i in range(self.proposals):
                     self.current_proposal = i
-                    self._host_proposal(i)
+                    acc_i = self._host_proposal(i)
+                    if self.surrogate is not None and self.surrogate_fallback_proposal is None:
+                        self._sur_recent.append(float(np.mean(acc_i)))
+                        if len(self._sur_recent) >= self._sur_window:
+                            self._surrogate_check(float(np.mean(self._sur_recent[-self._sur_window:])), i + 1)
                     if i % self.online_thinning == 0:
                         X, chi = self._chains.get_state()
                         self._write(X, chi)
@@ -289,6 +325,7 @@ class HMC:
                         self.autotuning, self.target_acceptance_rate, self.learning_rate,
                         self._device_seed, done, self.online_thinning)
                     self.accepted_proposals = self.accepted_proposals + (acc if B > 1 else int(acc[0]))
+                    self._surrogate_check(float(np.sum(acc)) / (n * B), done + n)
                     cnt = self._chains.trace_count()
                     if cnt:
                         X, chi = self._chains.read_trace(0, cnt)
@@ -344,6 +381,7 @@ class HMC:
             self.accepted_proposals += int(acc[0])
         else:
             self.accepted_proposals += acc.astype(np.int64)
+        return acc
 
     # ------------------------------------------------------------------ #
     def _close_files(self):

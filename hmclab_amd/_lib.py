@@ -80,6 +80,19 @@ SIGNATURES = {
     "hmc_chains_enable_timing": [_p, _i32],
     "hmc_chains_timing": [_p, _i32, _dp, _lp],
     "hmc_chains_sync": [_p],
+    "hmc_surrogate_create": [_i64, _i64, C.c_uint64, C.POINTER(_p)],
+    "hmc_surrogate_destroy": [_p],
+    "hmc_surrogate_set_mode": [_p, _i32],
+    "hmc_surrogate_set_normalization": [_p, _dp, _dp, C.c_double, C.c_double],
+    "hmc_surrogate_param_count": [_p, _lp],
+    "hmc_surrogate_get_params": [_p, C.POINTER(C.c_float)],
+    "hmc_surrogate_set_params": [_p, C.POINTER(C.c_float)],
+    "hmc_surrogate_eval": [_p, _dp, _i64, _dp, _dp],
+    "hmc_surrogate_loss_grad": [_p, _dp, _dp, _dp, _i64, C.c_double, C.c_double, _dp,
+                                C.POINTER(C.c_float)],
+    "hmc_surrogate_train": [_p, _dp, _dp, _dp, _i64, _i64, _i64, C.c_double, C.c_double,
+                            C.c_double, C.c_uint64, _dp],
+    "hmc_chains_set_surrogate": [_p, _p],
 }
 
 _LIB = None

@@ -9,13 +9,6 @@
 
 #include "hmc_internal.hpp"
 
-struct hmc_problem {
-  hmc::Problem p;
-};
-struct hmc_chains {
-  hmc::Chains c;
-};
-
 namespace hmc {
 
 static thread_local std::string g_last_error;
@@ -46,31 +39,10 @@ static void* upload(std::vector<void*>& owned, const void* host, size_t bytes, h
 
 // Fault injection for host-side tests of error paths: HMC_FAULT_INJECT=<name>
 // makes the named entry point fail with HMC_ERR_INJECTED before any work.
-static void maybe_inject(const char* name) {
+void maybe_inject(const char* name) {
   const char* f = std::getenv("HMC_FAULT_INJECT");
   if (f && std::strcmp(f, name) == 0)
     throw Error{HMC_ERR_INJECTED, std::string("fault injected in ") + name};
-}
-
-template <class F>
-static int guarded(const char* name, F&& f) {
-  try {
-    maybe_inject(name);
-    f();
-    return HMC_OK;
-  } catch (const Error& e) {
-    set_error(std::string(name) + ": " + e.msg);
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    set_error(std::string(name) + ": host allocation failed");
-    return HMC_ERR_NOMEM;
-  } catch (const std::exception& e) {
-    set_error(std::string(name) + ": " + e.what());
-    return HMC_ERR_STATE;
-  } catch (...) {
-    set_error(std::string(name) + ": unknown failure");
-    return HMC_ERR_STATE;
-  }
 }
 
 // ---- value conversion ------------------------------------------------------
@@ -217,7 +189,10 @@ static void gemm_proposal(Chains& c, bool host_p, int L, int integ, uint64_t see
   std::vector<Stage> st;
   integrator_schedule(integ, L, first, st);
   elem_refresh(c, host_p, first, seed, prop, !host_p, randomize);
-  for (const Stage& s : st) gemm_kick_drift(c, s.kick, s.drift, s.drift2);
+  for (const Stage& s : st) {
+    if (c.surrogate) surrogate_kick_drift(c, s.kick, s.drift, s.drift2);
+    else gemm_kick_drift(c, s.kick, s.drift, s.drift2);
+  }
   gemm_endpoint(c, c.xt, c.p->state_dtype);
   elem_metropolis(c, true, autotune, target, lr, prop, slot);
 }

@@ -1,6 +1,8 @@
 // Host-side objects behind the C-ABI handles and the engine entry points.
 #pragma once
 
+#include <new>
+#include <string>
 #include <vector>
 
 #include "hmc_common.hpp"
@@ -113,8 +115,37 @@ struct Chains {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   size_t ev_next = 0;
 
+  // learned surrogate driving the trajectory gradient (GEMM engine), or null
+  void* surrogate = nullptr;
+
   std::vector<void*> owned;
 };
+
+// error plumbing of the C-ABI: every entry point runs its body through
+// guarded(), which turns engine errors into status codes and the thread's
+// last-error message.
+void set_error(const std::string& msg);
+void maybe_inject(const char* name);
+template <class F>
+int guarded(const char* name, F&& f) {
+  try {
+    maybe_inject(name);
+    f();
+    return HMC_OK;
+  } catch (const Error& e) {
+    set_error(std::string(name) + ": " + e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(std::string(name) + ": host allocation failed");
+    return HMC_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(std::string(name) + ": " + e.what());
+    return HMC_ERR_STATE;
+  } catch (...) {
+    set_error(std::string(name) + ": unknown failure");
+    return HMC_ERR_STATE;
+  }
+}
 
 // allocation helpers
 void* dev_alloc(std::vector<void*>& owned, size_t bytes);
@@ -159,6 +190,11 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
 void fused_misfit(Chains& c, const void* X, double* out);
 void fused_gradient(Chains& c, const void* X, double* out);
 
+// ---- learned surrogate (hmc_surrogate.hip) ---------------------------------
+// Same contract as gemm_kick_drift with the likelihood gradient taken from
+// the surrogate network attached to the chains.
+void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2);
+
 // integrator schedules: coefficients of the first drift and of each
 // (kick, drift, drift2) stage; returned in units of the local step.
 struct Stage {
@@ -167,3 +203,10 @@ struct Stage {
 void integrator_schedule(int integrator, int L, double& first_drift, std::vector<Stage>& stages);
 
 }  // namespace hmc
+
+struct hmc_problem {
+  hmc::Problem p;
+};
+struct hmc_chains {
+  hmc::Chains c;
+};

@@ -213,6 +213,11 @@ class Chains:
     def sync(self):
         call("hmc_chains_sync", self._h)
 
+    def set_surrogate(self, surrogate=None):
+        """Drive the trajectory with a learned surrogate (None: exact gradient)."""
+        call("hmc_chains_set_surrogate", self._h, None if surrogate is None else surrogate._h)
+        self._surrogate = surrogate
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().hmc_chains_destroy(self._h)

@@ -198,6 +198,47 @@ int hmc_chains_timing(hmc_chains* c, int kind, double* total_ms, int64_t* launch
 /* Synchronise the engine's stream. */
 int hmc_chains_sync(hmc_chains* c);
 
+/* ---- learned surrogate potential ------------------------------------------
+ * No counterpart in the reference: hmclab's HMC always calls the target's
+ * gradient (Samplers.py:1524-1584, `self.target.gradient(position)` inside
+ * _propagate_leapfrog).  A surrogate is an MLP
+ *   f(x) = w3 . silu(W2 silu(W1 xn + b1) + b2) + b3,  xn = (x - x_mean)/x_scale
+ * of the likelihood misfit, chi_lik(x) ~ f_mean + f_scale f(x).  Attached to
+ * chains of a dense problem (GEMM engine), its input gradient replaces the
+ * likelihood gradient inside the trajectory; priors stay exact and the
+ * Metropolis test keeps the exact FP64 Hamiltonian.  Parameters are float32,
+ * packed as W1 (hidden x d, row-major) | b1 | W2 (hidden x hidden) | b2 | w3 | b3. */
+typedef struct hmc_surrogate hmc_surrogate;
+
+/* Glorot-uniform weights from `seed`; hidden must be a multiple of 64. */
+int hmc_surrogate_create(int64_t dimensions, int64_t hidden, uint64_t seed, hmc_surrogate** out);
+int hmc_surrogate_destroy(hmc_surrogate* s);
+/* 0: bf16 MFMA GEMMs with FP32 accumulation (default); 1: FP32 GEMMs. */
+int hmc_surrogate_set_mode(hmc_surrogate* s, int mode);
+int hmc_surrogate_set_normalization(hmc_surrogate* s, const double* x_mean, const double* x_scale,
+                                    double f_mean, double f_scale);
+int hmc_surrogate_param_count(hmc_surrogate* s, int64_t* n);
+int hmc_surrogate_get_params(hmc_surrogate* s, float* params);
+/* Replaces the parameters and resets the Adam moments. */
+int hmc_surrogate_set_params(hmc_surrogate* s, const float* params);
+/* Misfit approximation f_mean + f_scale f(x) [B] and its gradient [B x d]
+ * for rows X [B x d] (either output may be NULL). */
+int hmc_surrogate_eval(hmc_surrogate* s, const double* X, int64_t B, double* f, double* g);
+/* Mean training loss over a batch of NORMALISED rows Xn [B x d] with targets
+ * y [B] (of f) and t [B x d] (of df/dxn), and its packed parameter gradient
+ * (param_count floats; NULL to skip); B a multiple of 64. */
+int hmc_surrogate_loss_grad(hmc_surrogate* s, const double* Xn, const double* y, const double* t,
+                            int64_t B, double alpha, double beta, double* loss, float* grad);
+/* Adam training on N raw pairs (X [N x d], chi [N], grad [N x d]) of the
+ * likelihood misfit: `epochs` passes over shuffled batches of `batch` rows
+ * (a multiple of 64, <= N); loss_history[epochs] receives each epoch's mean
+ * loss (may be NULL).  Normalisation must be set first. */
+int hmc_surrogate_train(hmc_surrogate* s, const double* X, const double* chi, const double* grad,
+                        int64_t N, int64_t batch, int64_t epochs, double lr, double alpha,
+                        double beta, uint64_t seed, double* loss_history);
+/* Attach (or with NULL detach) a surrogate to chains of a dense problem. */
+int hmc_chains_set_surrogate(hmc_chains* c, hmc_surrogate* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,193 @@
+"""GPU parity of the learned surrogate (forward, input gradient, hand-written
+double backpropagation, Adam, surrogate-driven trajectories) against the
+NumPy oracle in oracle/mlp_oracle.py, through the C-ABI.
+
+Tolerances, relative to the oracle's float64 values and scaled by each
+array's largest magnitude: FP32 GEMM mode 1e-4; bf16 MFMA mode (operands
+rounded to bf16, FP32 accumulation) 3e-2.  Trajectories driven by the FP32
+surrogate: positions/momenta 1e-4, energies 1e-6 of the Hamiltonian."""
+import numpy as np
+import pytest
+
+from hmclab_amd import Distributions as D
+from hmclab_amd.Distributions import build_problem
+from hmclab_amd.engine import Chains
+from hmclab_amd.surrogate import Surrogate
+from oracle import hmc_oracle as O
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _params(rng, D_, H):
+    P = dict(W1=rng.normal(size=(H, D_)) / np.sqrt(D_), b1=rng.normal(size=H) * 0.1,
+             W2=rng.normal(size=(H, H)) / np.sqrt(H), b2=rng.normal(size=H) * 0.1,
+             w3=rng.normal(size=H) / np.sqrt(H), b3=np.float64(0.2))
+    # the device holds float32 weights: the oracle uses the same rounded values
+    return {k: np.asarray(v, dtype=np.float32).astype(np.float64) for k, v in P.items()}
+
+
+def _oracle_eval(P, X, xm, xs, fm, fs):
+    f, G, _ = M.forward(P, (X - xm) / xs)
+    return fm + fs * f, fs * G / xs
+
+
+@pytest.mark.parametrize("mode,tol", [("f32", 1e-4), ("bf16", 3e-2)])
+@pytest.mark.parametrize("D_,H,B", [(100, 64, 70), (257, 128, 64)])
+def test_eval_matches_oracle(mode, tol, D_, H, B):
+    rng = np.random.default_rng(D_ + H)
+    s = Surrogate(D_, H, seed=1, mode=mode)
+    P = _params(rng, D_, H)
+    s.set_params(P)
+    xm, xs = rng.normal(size=D_) * 0.1, rng.uniform(0.5, 2.0, size=D_)
+    s.set_normalization(xm, xs, 3.0, 2.5)
+    X = rng.normal(size=(B, D_))
+    f, g = s.evaluate(X)
+    fo, go = _oracle_eval(P, X, xm, xs, 3.0, 2.5)
+    assert _rel(f - 3.0, fo - 3.0) < tol
+    assert _rel(g, go) < tol
+    back = s.get_params()
+    for k in P:
+        assert np.array_equal(np.asarray(back[k], dtype=np.float64), P[k]), k
+
+
+@pytest.mark.parametrize("mode,tol", [("f32", 1e-4), ("bf16", 3e-2)])
+def test_loss_gradients_match_oracle(mode, tol):
+    rng = np.random.default_rng(5)
+    D_, H, B = 96, 64, 128
+    s = Surrogate(D_, H, seed=2, mode=mode)
+    P = _params(rng, D_, H)
+    s.set_params(P)
+    Xn = rng.normal(size=(B, D_))
+    y = rng.normal(size=B)
+    T = rng.normal(size=(B, D_)) * 0.3
+    for alpha, beta in [(1.0, 1.0), (0.0, 1.0), (1.0, 0.0)]:
+        loss, grads = s.loss_and_grads(Xn, y, T, alpha, beta)
+        lo, go = M.loss_and_grads(P, Xn, y, T, alpha, beta)
+        assert abs(loss - lo) <= tol * abs(lo)
+        for k in ("W1", "b1", "W2", "b2", "w3"):
+            assert _rel(grads[k], go[k]) < tol, (alpha, beta, k)
+        assert abs(float(grads["b3"]) - float(go["b3"])) <= tol * max(1.0, abs(float(go["b3"])))
+
+
+def test_adam_step_matches_oracle():
+    rng = np.random.default_rng(6)
+    D_, H, N = 64, 64, 64
+    s = Surrogate(D_, H, seed=3, mode="f32")
+    P = _params(rng, D_, H)
+    s.set_params(P)
+    s.set_normalization(np.zeros(D_), np.ones(D_), 0.0, 1.0)
+    X = rng.normal(size=(N, D_))
+    chi = rng.normal(size=N)
+    grad = rng.normal(size=(N, D_)) * 0.2
+    lr = 1e-3
+    s.train(X, chi, grad, epochs=1, batch=N, learning_rate=lr, normalize=False)
+    _, g = M.loss_and_grads(P, X, chi, grad, 1.0, 1.0)
+    ref = M.adam_step({k: np.array(v, copy=True) for k, v in P.items()}, g,
+                      {k: np.zeros_like(v) for k, v in P.items()}, {k: np.zeros_like(v) for k, v in P.items()},
+                      1, lr)
+    got = s.get_params()
+    for k in P:
+        gk = np.abs(np.atleast_1d(g[k]))
+        small = gk < 1e-5 * max(gk.max(), 1e-300)          # sign of a ~0 gradient is not determined
+        diff = np.abs(np.atleast_1d(np.asarray(got[k], dtype=np.float64) - ref[k]))
+        assert np.all(diff[~small] < 1e-6), k
+        assert np.all(diff[small] <= 2 * lr + 1e-6), k
+
+
+def test_training_learns_a_quadratic_misfit():
+    rng = np.random.default_rng(7)
+    D_, H, N = 32, 64, 2048
+    A = rng.normal(size=(D_, D_)) / np.sqrt(D_)
+    X = rng.normal(size=(N, D_))
+    R = X @ A.T
+    chi = 0.5 * np.sum(R * R, axis=1)
+    grad = R @ A
+    s = Surrogate(D_, H, seed=4, mode="bf16")
+    hist = s.train(X, chi, grad, epochs=40, batch=256, learning_rate=2e-3)
+    assert np.all(np.isfinite(hist))
+    assert hist[-1] < 0.1 * hist[0]
+    Xt = rng.normal(size=(256, D_))
+    Rt = Xt @ A.T
+    f, g = s.evaluate(Xt)
+    gt = Rt @ A
+    rel = np.linalg.norm(g - gt) / np.linalg.norm(gt)
+    assert rel < 0.5
+
+
+class _SurrogateTarget:
+    """Oracle trajectory target: exact prior + surrogate likelihood gradient."""
+
+    def __init__(self, exact, prior, P, xm, xs, fm, fs):
+        self.exact, self.prior, self.P = exact, prior, P
+        self.xm, self.xs, self.fm, self.fs = xm, xs, fm, fs
+
+    def gradient(self, x):
+        _, g = _oracle_eval(self.P, x[:, 0][None, :], self.xm, self.xs, self.fm, self.fs)
+        return self.prior.gradient(x) + g[0][:, None]
+
+    def corrector(self, x, p):
+        self.exact.corrector(x, p)
+
+
+def test_surrogate_trajectory_matches_oracle_with_exact_metropolis():
+    rng = np.random.default_rng(8)
+    n, d, B, L, H = 90, 40, 8, 12, 64
+    G = rng.normal(size=(n, d)) / np.sqrt(d)
+    dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+    mu, b = rng.normal(size=(d, 1)) * 0.1, rng.uniform(0.3, 1.0, size=(d, 1))
+    post = D.BayesRule([D.Laplace(mu.copy(), b.copy()), D.LinearMatrix(G, dd, 0.01)])
+    prior_o = O.Laplace(mu, b)
+    exact_o = O.BayesRule([prior_o, O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
+    prob = build_problem(post, precision="f64", engine="gemm")
+    ch = Chains(prob, B)
+    s = Surrogate(d, H, seed=5, mode="f32")
+    P = _params(rng, d, H)
+    s.set_params(P)
+    xm, xs = rng.normal(size=d) * 0.05, rng.uniform(0.2, 0.5, size=d)
+    s.set_normalization(xm, xs, 100.0, 40.0)
+    ch.set_surrogate(s)
+    X0 = rng.normal(size=(B, d)) * 0.3
+    ch.set_state(X0)
+    X0s, chi0 = ch.get_state()
+    P0 = rng.normal(size=(B, d))
+    e = 0.01 * rng.uniform(0.5, 1.5, size=B)
+    u = rng.uniform(size=B)
+    res = ch.propose(P0, e, u, L, "lf", trajectory=True)
+    sur_o = _SurrogateTarget(exact_o, prior_o, P, xm, xs, 100.0, 40.0)
+    unit = O.Unit(d, np.random.default_rng(0))
+    for k in range(B):
+        x0 = X0s[k].reshape(d, 1)
+        x1, p1 = O.leapfrog(sur_o, unit, x0, P0[k].reshape(d, 1), e[k], L)
+        h0 = exact_o.misfit(x0) + unit.kinetic_energy(P0[k].reshape(d, 1))
+        h1 = exact_o.misfit(x1) + unit.kinetic_energy(p1)
+        assert _rel(res["X1"][k], x1[:, 0]) < 1e-4
+        assert _rel(res["P1"][k], p1[:, 0]) < 1e-4
+        assert abs(res["H0"][k] - h0) <= 1e-10 * abs(h0)        # start: exact FP64
+        assert abs(res["H1"][k] - h1) <= 1e-6 * abs(h1)          # end: exact misfit at a 1e-4 position
+        if abs(np.log(u[k]) - (h0 - h1)) > 1e-3:
+            assert bool(res["accepted"][k]) == bool(np.exp(h0 - h1) > u[k])
+    # detaching restores the exact-gradient trajectory
+    ch.set_surrogate(None)
+    ch.set_state(X0s)
+    res2 = ch.propose(P0, e, u, L, "lf", trajectory=True)
+    for k in range(B):
+        ref = O.propose(exact_o, unit, X0s[k].reshape(d, 1), None, P0[k].reshape(d, 1), e[k], L, u[k])
+        assert _rel(res2["X1"][k], ref["x1"][:, 0]) < 1e-10
+
+
+def test_surrogate_requires_gemm_engine_and_matching_dimension():
+    rng = np.random.default_rng(9)
+    G = rng.normal(size=(30, 20))
+    post = D.BayesRule([D.Normal(np.zeros((20, 1)), 1.0), D.LinearMatrix(G, G @ np.ones((20, 1)), 0.01)])
+    ch = Chains(build_problem(post, precision="f64", engine="fused"), 4)
+    with pytest.raises(NotImplementedError):
+        ch.set_surrogate(Surrogate(20, 64))
+    ch2 = Chains(build_problem(post, precision="f64", engine="gemm"), 4)
+    with pytest.raises(ValueError):
+        ch2.set_surrogate(Surrogate(21, 64))
