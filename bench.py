@@ -3,13 +3,22 @@
 Default workload (the dense configuration the metric is quoted on):
 G 20000 x 8192 generated FP64 N(0, 1/8192) and rounded to bf16, m_true ~
 N(0, I), noise sigma 0.01, Normal prior std 1, Unit mass, 512 chains,
-25 leapfrog steps; trajectory GEMMs on bf16 MFMA (hi/lo split operands, FP32
-accumulation), endpoint Hamiltonian and Metropolis in FP64.  A "step" is
-one proposal for all 512 chains.  Chains start from exact posterior draws;
-W warm-up proposals run untimed, K proposals are timed; min-ESS is the
-rank-normalised split bulk ESS minimised over all 8192 parameters, pooled
-over chains.  Posterior mean/std errors are against the analytic FP64
-posterior of the bf16-rounded G.
+25 leapfrog steps.  Two legs run on the same target and the same exact
+posterior draws as starting states:
+
+* plain batched FP64 GPU HMC (the comparator): trajectory GEMMs on FP64 MFMA;
+* surrogate+batched (the reported value): trajectory GEMMs on bf16 MFMA
+  (hi/lo split operands, FP32 accumulation), endpoint Hamiltonian and
+  Metropolis in FP64; an MLP surrogate of the likelihood (8192 -> 512 -> 512
+  -> 1, SiLU) is trained on the GPU on (m, misfit, gradient) pairs from
+  warm-up proposals and drives the trajectory until a chunk's acceptance
+  falls below 0.5, after which the exact gradient takes over.
+
+A "step" is one proposal for all chains.  W warm-up proposals run untimed,
+K proposals are timed; min-ESS is the rank-normalised split bulk ESS
+minimised over all 8192 parameters, pooled over chains.  Posterior mean/std
+errors of both legs are against the analytic FP64 posterior of the
+bf16-rounded G ("matched posterior error").
 
 The roofline line is measured live with HIP events on the engine's stream
 around every trajectory GEMM launch (algorithmic FLOP per launch =
@@ -119,11 +128,88 @@ def trace_to_torch(torch, ch, K):
     return out
 
 
+def _posterior_errors(torch, ch, K, mu_np, sd_np, d):
+    from hmclab_amd import diagnostics
+
+    draws = trace_to_torch(torch, ch, K)
+    ess = diagnostics.ess_bulk(draws)
+    pooled = draws.to(torch.float64).reshape(-1, d)
+    mean = pooled.mean(dim=0).cpu().numpy()
+    std = pooled.std(dim=0).cpu().numpy()
+    del draws, pooled
+    err_mean = float(np.sqrt(np.mean(((mean - mu_np) / sd_np) ** 2)))
+    err_std = float(np.sqrt(np.mean((std / sd_np - 1.0) ** 2)))
+    return float(np.min(ess)), err_mean, err_std
+
+
+def _kernel_times(ch):
+    return {k: ch.timing(k) for k in range(4)}
+
+
+def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np):
+    """Comparator: plain batched FP64 GPU HMC (exact gradients on FP64 MFMA)."""
+    from hmclab_amd.Distributions import build_problem
+    from hmclab_amd.engine import Chains
+
+    B, L, d = args.chains, args.steps_per_traj, args.d
+    prob = build_problem(post, precision="f64", engine="gemm")
+    ch = Chains(prob, B)
+    ch.set_state(X0)
+    sub = argparse.Namespace(**vars(args))
+    sub.warmup = min(args.warmup, 5)
+    eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), sub, args.seed + 7, "fp64 ")
+    elapsed, acc = _timed(ch, B, L, eps, args.plain_steps, args.seed + 7, done, torch)
+    min_ess, em, es = _posterior_errors(torch, ch, args.plain_steps, mu_np, sd_np, d)
+    kt = _kernel_times(ch)
+    ch.close()
+    prob.close()
+    log(f"plain fp64: {args.plain_steps} proposals in {elapsed:.2f}s, acceptance {acc.mean() / args.plain_steps:.3f}, "
+        f"min ESS {min_ess:.0f} -> {min_ess / elapsed:.1f} ESS/s")
+    return dict(value=min_ess / elapsed, min_ess=min_ess, seconds=elapsed, proposals=args.plain_steps,
+                acceptance_rate=float(acc.mean() / args.plain_steps), stepsize=eps,
+                posterior_rms_mean_err_over_sd=em, posterior_rms_std_rel_err=es,
+                ms_per_proposal=elapsed / args.plain_steps * 1e3,
+                gemm_fp64_avg_us=(kt[0][0] + kt[3][0]) / max(kt[0][1] + kt[3][1], 1) * 1e3)
+
+
+def _train_surrogate(args, torch, ch, post, B, L, eps, done):
+    """Warm-up plain HMC samples -> exact (m, misfit, gradient) pairs -> Adam."""
+    from hmclab_amd.surrogate import Surrogate, training_pairs
+
+    d = args.d
+    t0 = time.perf_counter()
+    K = args.sur_warmup
+    ch.set_trace(K)
+    ch.run(K, L, np.full(B, eps), seed=args.seed + 3, first_proposal=done)
+    X, _ = ch.read_trace(0, K)
+    X = X.reshape(K * B, d)
+    chi, grad = training_pairs(post, X)
+    t_pairs = time.perf_counter() - t0
+    sur = Surrogate(d, args.sur_hidden, seed=args.seed, mode="bf16")
+    t1 = time.perf_counter()
+    hist = sur.train(X, chi, grad, epochs=args.sur_epochs, batch=256, learning_rate=args.sur_lr)
+    t_train = time.perf_counter() - t1
+    # held-out check at the chains' current states
+    Xc, _ = ch.get_state()
+    chi_c, g_c = training_pairs(post, Xc)
+    f_s, g_s = sur.evaluate(Xc)
+    gc = g_c - g_c.mean(axis=0)
+    grad_err = float(np.linalg.norm(g_s - g_c) / np.linalg.norm(gc))
+    misfit_err = float(np.sqrt(np.mean((f_s - chi_c) ** 2)) / max(np.std(chi_c), 1e-300))
+    del X, chi, grad
+    log(f"surrogate: {K * B} pairs in {t_pairs:.1f}s, {args.sur_epochs} epochs in {t_train:.1f}s, "
+        f"loss {hist[0]:.4g} -> {hist[-1]:.4g}; held-out gradient error {grad_err:.3f} (relative to "
+        f"the gradient's spread), misfit error {misfit_err:.3f} sd")
+    return sur, dict(pairs=K * B, warmup_and_pairs_s=t_pairs, train_s=t_train, epochs=args.sur_epochs,
+                     hidden=args.sur_hidden, loss_first=float(hist[0]), loss_last=float(hist[-1]),
+                     heldout_grad_rel_err=grad_err, heldout_misfit_err_over_sd=misfit_err,
+                     proposals_done=done + K)
+
+
 def run_dense(args):
     import torch
 
     from hmclab_amd import Distributions as D
-    from hmclab_amd import diagnostics
     from hmclab_amd.Distributions import build_problem
     from hmclab_amd.engine import Chains
 
@@ -140,63 +226,66 @@ def run_dense(args):
 
     post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0),
                         D.LinearMatrix(Gf.astype(np.float64), dd, sigma * sigma, dtype="bfloat16")])
+    plain = _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np) if args.plain_steps > 0 else None
+
     opts = {}
     if args.precision == "bf16":
         opts = {"bf16_split": args.bf16_split, "gemm_config": args.gemm_config}
+        if args.bf16_split_grad:
+            opts["bf16_split_grad"] = args.bf16_split_grad
     prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)
     log(f"device problem ready ({time.perf_counter() - t_setup:.1f}s)")
+    eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), args, args.seed)
 
-    # warm-up: tune one global step size towards acceptance 0.65 (multiplicative
-    # updates on the log scale), then run the remaining untimed proposals
-    eps = args.eps_scale / np.sqrt(lam_max)
-    done = 0
-    for _ in range(args.tune_rounds):
-        _, acc_t = ch.run(args.tune_props, L, np.full(B, eps), seed=args.seed, first_proposal=done)
-        done += args.tune_props
-        rate = acc_t.mean() / args.tune_props
-        eps *= float(np.exp(1.5 * (rate - 0.65)))
-        log(f"tune: acceptance {rate:.3f} -> eps {eps:.4g}")
-    step = np.full(B, eps)
-    if args.warmup:
-        ch.run(args.warmup, L, step, seed=args.seed, first_proposal=done)
-    done += args.warmup
+    sur, sur_info = None, None
+    if args.surrogate:
+        sur, sur_info = _train_surrogate(args, torch, ch, post, B, L, eps, done)
+        done = sur_info["proposals_done"]
+        ch.set_surrogate(sur)
+
+    # timed sampling phase: chunks of proposals; the surrogate is detached when
+    # a chunk's acceptance falls below the fallback threshold
     ch.set_trace(args.steps)
     ch.enable_timing(True)
     ch.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, acc = ch.run(args.steps, L, step, seed=args.seed, first_proposal=done)
+    acc = np.zeros(B, dtype=np.int64)
+    fallback_at, sur_acc, k = None, [], 0
+    while k < args.steps:
+        # while the surrogate drives the trajectories, short probe chunks keep a
+        # failing surrogate from costing more than sur_probe proposals
+        m = min(args.sur_probe if (sur is not None and fallback_at is None) else args.chunk, args.steps - k)
+        _, a = ch.run(m, L, np.full(B, eps), seed=args.seed, first_proposal=done + k)
+        acc += a
+        k += m
+        if sur is not None and fallback_at is None:
+            rate = float(a.sum()) / (m * B)
+            sur_acc.append(rate)
+            if rate < args.sur_fallback:
+                ch.set_surrogate(None)
+                fallback_at = k
     ch.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    res_ms, res_n = ch.timing(0)      # residual GEMM launches
-    leap_ms, leap_n = ch.timing(3)    # gradient GEMM launches with the kick/drift epilogue
-    end_ms, end_n = ch.timing(1)
-    ew_ms, ew_n = ch.timing(2)
+    if fallback_at is not None:
+        log(f"surrogate fallback after {fallback_at} proposals (chunk acceptance {sur_acc[-1]:.3f} < "
+            f"{args.sur_fallback})")
+    kt = _kernel_times(ch)
+    (res_ms, res_n), (end_ms, end_n), (ew_ms, ew_n), (leap_ms, leap_n) = kt[0], kt[1], kt[2], kt[3]
     log(f"timed {args.steps} proposals in {elapsed:.3f}s; residual gemm {res_ms:.1f} ms/{res_n}, "
         f"kick-drift gemm {leap_ms:.1f} ms/{leap_n}, endpoint {end_ms:.1f} ms/{end_n}, "
         f"elementwise {ew_ms:.1f} ms/{ew_n}")
-    # the dominant kernel is the slower of the two trajectory GEMMs
     if leap_ms >= res_ms:
         gemm_ms, gemm_n, gemm_name = leap_ms, leap_n, "k_gemm_bf16<...,EpiLeap> (gradient GEMM + kick/drift)"
     else:
         gemm_ms, gemm_n, gemm_name = res_ms, res_n, "k_gemm_bf16<...,EpiResid> (residual GEMM)"
-
-    draws = trace_to_torch(torch, ch, args.steps)
-    ess = diagnostics.ess_bulk(draws)
-    min_ess = float(np.min(ess))
-    pooled = draws.to(torch.float64).reshape(-1, d)
-    mean = pooled.mean(dim=0).cpu().numpy()
-    std = pooled.std(dim=0).cpu().numpy()
-    del draws, pooled
-    err_mean = float(np.sqrt(np.mean(((mean - mu_np) / sd_np) ** 2)))
-    err_std = float(np.sqrt(np.mean((std / sd_np - 1.0) ** 2)))
+    min_ess, err_mean, err_std = _posterior_errors(torch, ch, args.steps, mu_np, sd_np, d)
     acc_rate = float(acc.mean() / args.steps)
     value = min_ess / elapsed
 
-    # roofline of the dominant kernel (each trajectory GEMM launch = 2 n d B flops)
     flop_launch = 2.0 * n * d * B
     avg_s = gemm_ms / 1e3 / max(gemm_n, 1)
     achieved = flop_launch / avg_s
@@ -209,6 +298,22 @@ def run_dense(args):
         except Exception:
             traffic = None
 
+    cfg = {
+        "workload": f"dense linear-Gaussian {n}x{d}, G stored bf16, {B} chains, L={L}: surrogate+batched "
+                    f"({args.precision} trajectory GEMMs, fp64 endpoint Metropolis, MLP surrogate with "
+                    f"fallback) vs plain batched fp64 GPU HMC",
+        "chains": B, "n_data": n, "n_params": d, "leapfrog_steps": L, "stepsize": eps,
+        "min_ess": min_ess, "acceptance_rate": acc_rate,
+        "posterior_rms_mean_err_over_sd": err_mean, "posterior_rms_std_rel_err": err_std,
+        "proposals_per_s": args.steps / elapsed,
+    }
+    if sur_info is not None:
+        sur_info.update(chunk_acceptance=sur_acc, fallback_after_proposals=fallback_at,
+                        fallback_threshold=args.sur_fallback)
+        cfg["surrogate"] = sur_info
+    if plain is not None:
+        cfg["plain_fp64"] = plain
+        cfg["speedup_vs_plain_fp64"] = value / plain["value"]
     result = {
         "metric": METRIC,
         "value": value,
@@ -222,14 +327,7 @@ def run_dense(args):
         "vs_baseline": None,
         "dtype": args.precision,
         "data": "synthetic",
-        "config": {
-            "workload": f"dense linear-Gaussian {n}x{d}, G stored bf16, {B} chains, L={L}, "
-                        "plain batched HMC (no surrogate), bf16 trajectory + fp64 endpoint",
-            "chains": B, "n_data": n, "n_params": d, "leapfrog_steps": L, "stepsize": eps,
-            "min_ess": min_ess, "acceptance_rate": acc_rate,
-            "posterior_rms_mean_err_over_sd": err_mean, "posterior_rms_std_rel_err": err_std,
-            "proposals_per_s": args.steps / elapsed,
-        },
+        "config": cfg,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel": gemm_name, "avg_launch_us": avg_s * 1e6,
@@ -249,6 +347,8 @@ def run_dense(args):
         }
     ch.close()
     prob.close()
+    if sur is not None:
+        sur.close()
     return result
 
 
@@ -355,6 +455,92 @@ def run_tomography(args):
     return result
 
 
+def run_uneven(args):
+    """256 independent dense problems of uneven size (d log-uniform in
+    [50, 2000], n = 1.5 d, G float32), 16 chains each, leapfrog steps per
+    problem uniform in {5..30}; one hipGraph replay per proposal round."""
+    import torch
+
+    from hmclab_amd import diagnostics
+    from hmclab_amd.problem_set import CHAINS, ProblemSet, curvature_and_start, uneven_problems
+
+    t0 = time.perf_counter()
+    d, n, L, specs = uneven_problems(seed=args.seed, problems=args.problems)
+    ps = ProblemSet(d, n)
+    starts, steps = [], []
+    for p, sp in enumerate(specs):
+        ps.set_problem(p, sp["G"], sp["d_obs"], sp["variance"], prior=sp["prior"], scale=sp["scale"])
+        lam, x = curvature_and_start(sp)
+        starts.append(np.repeat(x[None, :], CHAINS, axis=0))
+        steps.append(np.full(CHAINS, args.eps_scale / np.sqrt(lam)))
+    ps.set_state(ps.join(starts))
+    step = np.concatenate(steps)
+    log(f"{args.problems} problems (sum n*d = {int(np.sum(n * d)):,}) ready in {time.perf_counter() - t0:.1f}s")
+    # tune each problem's step towards acceptance 0.65 (multiplicative, per problem)
+    done = 0
+    for _ in range(args.tune_rounds):
+        acc = ps.run(args.tune_props, L, step, seed=args.seed, first_proposal=done)
+        done += args.tune_props
+        rate = acc.reshape(-1, CHAINS).sum(axis=1) / (CHAINS * args.tune_props)
+        step *= np.repeat(np.exp(1.5 * (rate - 0.65)), CHAINS)
+    log(f"tuned: acceptance per problem {rate.min():.2f}..{rate.max():.2f}; tiles {ps.tile_config()}")
+    ps.run(args.warmup, L, step, seed=args.seed, first_proposal=done)
+    done += args.warmup
+    # kernel times from one directly launched round with events
+    ps.enable_timing(True)
+    ps.run(2, L, step, seed=args.seed, first_proposal=done, graph=False)
+    kt = {k: ps.timing(k) for k in range(4)}
+    ps.enable_timing(False)
+    done += 2
+    ps.set_trace(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    acc = ps.run(args.steps, L, step, seed=args.seed, first_proposal=done, graph=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    X, _ = ps.read_trace(0, args.steps)
+    ess_min = []
+    for p in range(args.problems):
+        a, b = ps.offsets[p], ps.offsets[p + 1]
+        draws = X[:, a:b].reshape(args.steps, CHAINS, int(d[p])).transpose(1, 0, 2)
+        ess_min.append(float(np.min(diagnostics.ess_bulk(draws))))
+    ess_min = np.array(ess_min)
+    rates = acc.reshape(-1, CHAINS).sum(axis=1) / (CHAINS * args.steps)
+    # algorithmic bytes per round (one pass of G per step, x and p read+write)
+    bytes_round = float(np.sum(L * (4.0 * n * d + 256.0 * d)) + np.sum(4.0 * n * d))
+    r_ms, r_n = kt[0]
+    g_ms, g_n = kt[1]
+    e_ms, e_n = kt[2]
+    c_ms, c_n = kt[3]
+    step_ms = (r_ms + g_ms) / 2.0                      # resid + grad launches of one round
+    traj_bytes = float(np.sum(L * (4.0 * n * d + 256.0 * d)))
+    achieved = traj_bytes / (step_ms / 1e3)
+    actual = float(np.sum(L * (8.0 * n * d + 256.0 * d + 2 * 64.0 * n))) / (step_ms / 1e3)
+    result = {
+        "metric": METRIC, "value": float(ess_min.min()) / elapsed, "unit": "ESS/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": done, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.problems} uneven dense problems (d in [50,2000] log-uniform, n=1.5d, "
+                               f"G fp32), {CHAINS} chains each, L in [5,30]; fp32 MFMA trajectories, fp64 "
+                               f"endpoint; one hipGraph replay per proposal round",
+                   "problems": args.problems, "chains": args.problems * CHAINS,
+                   "sum_nd": int(np.sum(n * d)), "value_is": "min over problems of min-ESS, per second",
+                   "median_problem_min_ess_per_s": float(np.median(ess_min)) / elapsed,
+                   "acceptance_min": float(rates.min()), "acceptance_median": float(np.median(rates)),
+                   "rounds_per_s": args.steps / elapsed, "tile_config": ps.tile_config(),
+                   "kernel_ms_per_round": {"resid": r_ms / 2, "grad": g_ms / 2, "endpoint": e_ms / 2,
+                                           "per_chain": c_ms / 2}},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK["hbm"], "traffic": None,
+                     "kernel": "k_set_resid + k_set_grad (all trajectory launches of one round)",
+                     "algorithmic_bytes_per_round": traj_bytes, "actual_stream_GBps": actual / 1e9,
+                     "avg_launch_us": {"resid": r_ms / max(r_n, 1) * 1e3, "grad": g_ms / max(g_n, 1) * 1e3}},
+    }
+    ps.close()
+    return result
+
+
 def run_small(args):
     """The 1-chain dense case (G 100x50): latency per proposal on the fused
     engine at the stated step 0.05 and ESS/s at a tuned step."""
@@ -406,7 +592,8 @@ def run_small(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small"])
+    ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small", "uneven"])
+    ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=20000)
@@ -420,11 +607,27 @@ def main():
     ap.add_argument("--cpu-eps", type=float, default=1e-3)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "f64", "f32"])
     ap.add_argument("--bf16-split", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--bf16-split-grad", type=int, default=None, choices=[1, 2],
+                    help="bf16 planes of the residual in the gradient GEMM (default: --bf16-split)")
     ap.add_argument("--gemm-config", type=int, default=1, choices=[0, 1])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-proposals", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="timed proposals of the plain fp64 comparator (default: --steps, so both "
+                         "legs' ESS estimates see chains of equal length; 0 skips it)")
+    ap.add_argument("--chunk", type=int, default=10, help="proposals per device run call")
+    ap.add_argument("--no-surrogate", dest="surrogate", action="store_false")
+    ap.add_argument("--sur-warmup", type=int, default=100, help="warm-up proposals giving training pairs")
+    ap.add_argument("--sur-hidden", type=int, default=512)
+    ap.add_argument("--sur-epochs", type=int, default=8)
+    ap.add_argument("--sur-lr", type=float, default=1e-3)
+    ap.add_argument("--sur-fallback", type=float, default=0.5)
+    ap.add_argument("--sur-probe", type=int, default=2,
+                    help="proposals per acceptance check while the surrogate is attached")
     args = ap.parse_args()
+    if args.plain_steps is None:
+        args.plain_steps = args.steps
     if args.cpu_baseline_child:
         cpu_baseline_child(args)
         return
@@ -438,6 +641,8 @@ def main():
         res = run_tomography(args)
     elif args.workload == "small":
         res = run_small(args)
+    elif args.workload == "uneven":
+        res = run_uneven(args)
     else:
         res = run_dense(args)
     print(json.dumps(res), flush=True)

@@ -93,6 +93,23 @@ SIGNATURES = {
     "hmc_surrogate_train": [_p, _dp, _dp, _dp, _i64, _i64, _i64, C.c_double, C.c_double,
                             C.c_double, C.c_uint64, _dp],
     "hmc_chains_set_surrogate": [_p, _p],
+    "hmc_set_create": [_i64, _lp, _lp, C.POINTER(_p)],
+    "hmc_set_destroy": [_p],
+    "hmc_set_problem_dense": [_p, _i64, C.POINTER(C.c_float), _dp, _dp, _i32],
+    "hmc_set_problem_prior": [_p, _i64, _i32, _dp, _dp],
+    "hmc_set_set_option": [_p, C.c_char_p, C.c_double],
+    "hmc_set_tile_config": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "hmc_set_set_state": [_p, _dp],
+    "hmc_set_get_state": [_p, _dp, _dp],
+    "hmc_set_misfit": [_p, _dp, _dp],
+    "hmc_set_gradient": [_p, _dp, _dp],
+    "hmc_set_propose": [_p, _dp, _dp, _dp, _ip, _ip, _dp, _dp, _dp],
+    "hmc_set_run": [_p, _i64, _ip, _dp, _i32, C.c_uint64, _i64, _i32, _lp],
+    "hmc_set_set_trace": [_p, _i64],
+    "hmc_set_trace_count": [_p, _lp],
+    "hmc_set_read_trace": [_p, _i64, _i64, _dp, _dp],
+    "hmc_set_enable_timing": [_p, _i32],
+    "hmc_set_timing": [_p, _i32, _dp, _lp],
 }
 
 _LIB = None

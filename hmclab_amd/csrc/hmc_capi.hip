@@ -431,6 +431,10 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
     if (key == "bf16_split") {
       HMC_REQUIRE(value == 1.0 || value == 2.0, HMC_ERR_ARG, "bf16_split must be 1 or 2");
       P.bf16_split = (int)value;
+      P.bf16_split_grad = (int)value;
+    } else if (key == "bf16_split_grad") {
+      HMC_REQUIRE(value == 1.0 || value == 2.0, HMC_ERR_ARG, "bf16_split_grad must be 1 or 2");
+      P.bf16_split_grad = (int)value;
     } else if (key == "gemm_config") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_config must be 0 or 1");
       P.gemm_cfg = (int)value;

@@ -499,13 +499,13 @@ static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const
 }
 
 template <class Epi>
-static void launch_bf16(const Problem& p, const uint16_t* A, int64_t lda, const uint16_t* B0,
+static void launch_bf16(const Problem& p, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
                         const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
   if (p.gemm_cfg == 0) {
-    if (p.bf16_split == 2) launch_bf16_cfg<64, 64, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+    if (split == 2) launch_bf16_cfg<64, 64, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
     else launch_bf16_cfg<64, 64, 1>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
   } else {
-    if (p.bf16_split == 2) launch_bf16_cfg<64, 32, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+    if (split == 2) launch_bf16_cfg<64, 32, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
     else launch_bf16_cfg<64, 32, 1>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
   }
 }
@@ -523,11 +523,11 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2) {
   if (p.precision == HMC_BF16) {
     EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
     time_begin(c, 0, ev1);
-    launch_bf16(p, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    launch_bf16(p, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
     time_end(c, 0, ev1);
     EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, c.xl, c.ldx, c.eps, kick, drift, drift2, md, d, B};
     time_begin(c, 3, ev2);
-    launch_bf16(p, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_bf16(p, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
@@ -578,9 +578,9 @@ void gemm_gradient(Chains& c, const void* X, double* out) {
   if (p.precision == HMC_BF16) {
     // hi/lo planes of X were prepared by the caller into c.xh / c.xl
     EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
-    launch_bf16(p, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    launch_bf16(p, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
     EpiGrad<float> e2{(const float*)X, out, c.ldx, c.oob, md, d, B};
-    launch_bf16(p, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_bf16(p, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
     launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e1);

@@ -30,7 +30,8 @@ struct Problem {
   int state_dtype = HMC_F64;// storage of positions/momenta (F64 or F32)
   int op = 0;               // 0 none, 1 dense, 2 csr
   int engine = HMC_ENGINE_AUTO;
-  int bf16_split = 2;       // bf16 planes of the FP32 GEMM operand (1 or 2)
+  int bf16_split = 2;       // bf16 planes of the positions in the residual GEMM (1 or 2)
+  int bf16_split_grad = 2;  // bf16 planes of the residuals in the gradient GEMM (1 or 2)
   int gemm_cfg = 1;         // bf16 GEMM tile configuration
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 

@@ -1,0 +1,1090 @@
+// Many independent dense linear problems of uneven size, sampled together.
+//
+// hmclab samples one target per process (ParallelSampleSMP runs one
+// HMC.sample per chain, Samplers.py:1757-1977; the trajectory is
+// HMC._propagate_leapfrog, Samplers.py:1524-1584, and the accept test
+// HMC._evaluate_acceptance, Samplers.py:1471-1492).  Here P problems x C=16
+// chains share every launch:
+//
+//   refresh      one workgroup per chain: momentum, first half drift, K0
+//   resid        tiles (problem, 64*RW rows): w = (G x - d) / var       (fp32 MFMA)
+//   grad_leap    tiles (problem, 64 columns): g = G^T w + prior'(x);
+//                p -= eps g; x += c eps p                               (fp32 MFMA)
+//   endpoint     tiles (problem, 64 rows): sum (G x - d)^2 / var         (fp64 MFMA)
+//   metropolis   one workgroup per chain: exact FP64 H0/H1, accept, trace
+//
+// Step s of the trajectory launches only the tiles of problems with L > s:
+// problems are ordered by descending L so that those tiles form a prefix of
+// one tile list.  A whole proposal round is captured once as a hipGraph and
+// replayed; the RNG counter and the run parameters live in device memory so
+// replays need no new arguments.  Two kernel shapes (rows per residual tile,
+// waves per gradient tile) are chosen per set by timing every combination
+// on the set itself.
+//
+// Layouts in HBM (fp32 unless noted), per problem p with d, n:
+//   G   [round_up(n,16)][ldg = round_up(d,16)]  row-major, zero padded
+//   x0, xt, pt  [C][ldg]   chain-major states (fp32), zero padded
+//   w   [C][ldr = round_up(n,16)]   scaled residuals
+//   dobs, ivar  [n] fp64;   prior a, b  [d] fp64
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "hmc_internal.hpp"
+
+namespace hmc {
+
+constexpr int SC = 16;   // chains per problem: one MFMA tile width
+
+struct SetDesc {
+  int32_t d, n, ldg, ldr;
+  int64_t g_off, x_off, w_off, v_off, a_off, h_off;   // h_off: offset in host-layout rows
+  int32_t prior_kind, L;
+  int32_t ptile0, n_ptiles;                           // endpoint tiles of this problem
+};
+
+struct SetRun {
+  uint64_t seed;
+  int64_t prop;        // proposal index of the next refresh
+  int64_t trace_base;  // proposal index recorded in trace slot 0
+  int64_t trace_cap;
+  int32_t randomize;
+  int32_t host_p;      // momenta/eps/u supplied by the host
+};
+
+__device__ inline float set_prior_grad(int kind, double a, double b, float x) {
+  if (kind == HMC_PRIOR_NORMAL) return -(float)b * ((float)a - x);
+  if (kind == HMC_PRIOR_LAPLACE) {
+    const float af = (float)a;
+    const float s = (x > af) ? 1.f : ((x < af) ? -1.f : 0.f);
+    return s * (float)b;
+  }
+  return 0.f;
+}
+
+__device__ inline double set_prior_misfit(int kind, double a, double b, double x) {
+  if (kind == HMC_PRIOR_NORMAL) {
+    const double r = a - x;
+    return 0.5 * (r * (b * r));
+  }
+  if (kind == HMC_PRIOR_LAPLACE) return fabs(x - a) * b;
+  return 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// residual: w[c][row] = (sum_k G[row][k] x[c][k] - dobs[row]) * ivar[row]
+// 4 waves; wave wv owns RW MFMA row tiles of 16 rows.  MFMA 16x16x4 f32:
+// lane (i = l%16, q = l/16) feeds A = G[row i][k], B = x[chain i][k] with
+// k = k0 + 4q + e in instruction e (one float4 load per operand feeds four
+// instructions); the accumulator of lane (q, i) holds rows 4q+v of chain i.
+// ---------------------------------------------------------------------------
+template <int RW>
+__global__ __launch_bounds__(256) void k_set_resid(const SetDesc* __restrict__ desc,
+                                                   const int2* __restrict__ tiles,
+                                                   const float* __restrict__ G,
+                                                   const float* __restrict__ X,
+                                                   const double* __restrict__ dobs,
+                                                   const double* __restrict__ ivar,
+                                                   float* __restrict__ W) {
+  const int2 t = tiles[blockIdx.x];
+  const SetDesc D = desc[t.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = lane & 15, q = lane >> 4;
+  const int rbase = t.y * (64 * RW) + wv * (16 * RW);
+  const float* xrow = X + D.x_off + (int64_t)i * D.ldg + 4 * q;
+  floatx4 acc[RW];
+  const float* grow[RW];
+  bool act[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    acc[r] = floatx4{0.f, 0.f, 0.f, 0.f};
+    act[r] = rbase + 16 * r < D.n;
+    grow[r] = G + D.g_off + (int64_t)(rbase + 16 * r + i) * D.ldg + 4 * q;
+  }
+  for (int k0 = 0; k0 < D.ldg; k0 += 16) {
+    const floatx4 b = *(const floatx4*)(xrow + k0);
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (!act[r]) continue;
+      const floatx4 a = *(const floatx4*)(grow[r] + k0);
+      acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], acc[r], 0, 0, 0);
+      acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], b[1], acc[r], 0, 0, 0);
+      acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], b[2], acc[r], 0, 0, 0);
+      acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], acc[r], 0, 0, 0);
+    }
+  }
+  const double* dp = dobs + D.v_off;
+  const double* vp = ivar + D.v_off;
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    if (!act[r]) continue;
+    const int row0 = rbase + 16 * r + 4 * q;
+    floatx4 o;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = row0 + v;
+      o[v] = (row < D.n) ? (acc[r][v] - (float)dp[row]) * (float)vp[row] : 0.f;
+    }
+    *(floatx4*)(W + D.w_off + (int64_t)i * D.ldr + row0) = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gradient + kick/drift on 64 columns of one problem.  NW waves split the row
+// groups of 16; lane (i, q) loads G[row][col0 + 4i .. 4i+3] (one float4 per
+// row, 256 contiguous bytes per wave) and feeds component t to MFMA tile t,
+// so tile t's A row i is column col0 + 4i + t; B = w[chain i][row].  The
+// accumulator of lane (q, i) then holds columns col0 + 16q + 4v + t of chain i.
+// MODE 0: kick/drift; MODE 1: store the gradient (f64) for gradient queries.
+// ---------------------------------------------------------------------------
+template <int NW, int MODE>
+__global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict__ desc,
+                                                      const int2* __restrict__ tiles,
+                                                      const float* __restrict__ G,
+                                                      const float* __restrict__ W,
+                                                      float* __restrict__ X, float* __restrict__ P,
+                                                      const double* __restrict__ pa,
+                                                      const double* __restrict__ pb,
+                                                      const double* __restrict__ eps, int step,
+                                                      double* __restrict__ gout) {
+  __shared__ float red[NW][SC][64];
+  const int2 t = tiles[blockIdx.x];
+  const SetDesc D = desc[t.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = lane & 15, q = lane >> 4;
+  const int col0 = t.y * 64;
+  const bool cin = col0 + 4 * i < D.ldg;
+  floatx4 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const float* wrow = W + D.w_off + (int64_t)i * D.ldr + 4 * q;
+  const float* gcol = G + D.g_off + col0 + 4 * i;
+  const int ngroups = (D.n + 15) >> 4;
+  for (int g = wv; g < ngroups; g += NW) {
+    const int r0 = g * 16;
+    const floatx4 b = *(const floatx4*)(wrow + r0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const floatx4 a = cin ? *(const floatx4*)(gcol + (int64_t)(r0 + 4 * q + e) * D.ldg)
+                            : floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[e], acc[u], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[wv][i][16 * q + 4 * v + u] = acc[u][v];
+  __syncthreads();
+  for (int e = threadIdx.x; e < SC * 64; e += NW * 64) {
+    const int c = e >> 6, col = e & 63;
+    const int j = col0 + col;
+    if (j >= D.d) continue;
+    float gs = 0.f;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) gs += red[w][c][col];
+    const int64_t o = D.x_off + (int64_t)c * D.ldg + j;
+    const float xv0 = X[o];
+    const float gv = gs + set_prior_grad(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], xv0);
+    if (MODE == 1) {
+      gout[D.h_off + (int64_t)c * D.d + j] = (double)gv;
+    } else {
+      if (step >= D.L) continue;
+      const float ec = (float)eps[(int64_t)t.x * SC + c];
+      const float drift = (step == D.L - 1) ? 0.5f : 1.f;
+      float pv = P[o];
+      pv -= ec * gv;
+      X[o] = xv0 + (drift * ec) * pv;
+      P[o] = pv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// endpoint data misfit in FP64: partial[tile][c] = sum over the tile's 64 rows
+// of (G x - d)^2 / var.  MFMA 16x16x4 f64 with the same k order as resid; the
+// accumulator of lane (q, i) holds rows q + 4v of chain i.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_set_endpoint(const SetDesc* __restrict__ desc,
+                                                      const int2* __restrict__ tiles,
+                                                      const float* __restrict__ G,
+                                                      const float* __restrict__ X,
+                                                      const double* __restrict__ dobs,
+                                                      const double* __restrict__ ivar,
+                                                      double* __restrict__ partial) {
+  __shared__ double part[4][SC];
+  const int2 t = tiles[blockIdx.x];
+  const SetDesc D = desc[t.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = lane & 15, q = lane >> 4;
+  const int rbase = t.y * 64 + wv * 16;
+  double s = 0.0;
+  if (rbase < D.n) {
+    const float* grow = G + D.g_off + (int64_t)(rbase + i) * D.ldg + 4 * q;
+    const float* xrow = X + D.x_off + (int64_t)i * D.ldg + 4 * q;
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < D.ldg; k0 += 16) {
+      const floatx4 a = *(const floatx4*)(grow + k0);
+      const floatx4 b = *(const floatx4*)(xrow + k0);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[e], (double)b[e], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rbase + q + 4 * v;
+      if (row < D.n) {
+        const double r = acc[v] - dobs[D.v_off + row];
+        s += r * (r * ivar[D.v_off + row]);
+      }
+    }
+  }
+  s += __shfl_xor(s, 16, 64);
+  s += __shfl_xor(s, 32, 64);
+  if (q == 0) part[wv][i] = s;
+  __syncthreads();
+  if (threadIdx.x < SC) {
+    const double v = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                     part[3][threadIdx.x];
+    partial[(int64_t)blockIdx.x * SC + threadIdx.x] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-chain kernels (one workgroup of 256 per chain gc = problem * 16 + c)
+// ---------------------------------------------------------------------------
+constexpr int SW = 256;
+
+__global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ desc,
+                                                    const float* __restrict__ X0, float* __restrict__ XT,
+                                                    float* __restrict__ PT,
+                                                    const double* __restrict__ p_host,
+                                                    double* __restrict__ k0, double* __restrict__ eps,
+                                                    double* __restrict__ u,
+                                                    const double* __restrict__ stepv,
+                                                    const SetRun* __restrict__ run) {
+  __shared__ double scratch[SW / 64 + 1];
+  const int64_t gc = blockIdx.x;
+  const SetDesc D = desc[gc / SC];
+  const int c = (int)(gc % SC);
+  const SetRun R = *run;
+  double e;
+  if (!R.host_p) {
+    double ue, ua;
+    philox_scalars(R.seed, (uint32_t)gc, (uint64_t)R.prop, ue, ua);
+    e = R.randomize ? ue * stepv[gc] : stepv[gc];
+    if (threadIdx.x == 0) {
+      eps[gc] = e;
+      u[gc] = ua;
+    }
+  } else {
+    e = eps[gc];
+  }
+  const float half = (float)(0.5 * e);
+  double ks = 0.0;
+  const int64_t base = D.x_off + (int64_t)c * D.ldg;
+  const int npairs = (D.d + 1) / 2;
+  for (int qp = threadIdx.x; qp < npairs; qp += SW) {
+    double z[2] = {0.0, 0.0};
+    if (!R.host_p) philox_normal2(R.seed, (uint32_t)gc, (uint64_t)R.prop, (uint32_t)qp, z[0], z[1]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = 2 * qp + h;
+      if (j >= D.d) break;
+      const float pv = R.host_p ? (float)p_host[D.h_off + (int64_t)c * D.d + j] : (float)z[h];
+      ks += (double)pv * (double)pv;
+      XT[base + j] = X0[base + j] + half * pv;
+      PT[base + j] = pv;
+    }
+  }
+  const double s = block_sum<SW>(ks, scratch);
+  if (threadIdx.x == 0) k0[gc] = 0.5 * s;
+}
+
+__device__ inline double set_chain_misfit(const SetDesc& D, const float* x, const double* partial,
+                                          int c, const double* pa, const double* pb, double* scratch,
+                                          const float* p, double* kin) {
+  double pm = 0.0, kk = 0.0;
+  for (int j = threadIdx.x; j < D.d; j += SW) {
+    pm += set_prior_misfit(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], (double)x[j]);
+    if (p) kk += (double)p[j] * (double)p[j];
+  }
+  pm = block_sum<SW>(pm, scratch);
+  if (p) kk = block_sum<SW>(kk, scratch);
+  double lik = 0.0;
+  for (int k = 0; k < D.n_ptiles; ++k) lik += partial[(int64_t)(D.ptile0 + k) * SC + c];
+  if (kin) *kin = 0.5 * kk;
+  return 0.5 * lik + pm;
+}
+
+__global__ __launch_bounds__(SW) void k_set_metropolis(
+    const SetDesc* __restrict__ desc, float* __restrict__ X0, const float* __restrict__ XT,
+    const float* __restrict__ PT, const double* __restrict__ partial, const double* __restrict__ pa,
+    const double* __restrict__ pb, double* __restrict__ chi0, const double* __restrict__ k0,
+    const double* __restrict__ u, double* __restrict__ h0o, double* __restrict__ h1o,
+    int32_t* __restrict__ acc, int64_t* __restrict__ acc_count, const SetRun* __restrict__ run,
+    float* __restrict__ tr_x, double* __restrict__ tr_chi, int64_t tr_row, int64_t nchains) {
+  __shared__ double scratch[SW / 64 + 1];
+  __shared__ int accepted;
+  __shared__ double chi_new;
+  const int64_t gc = blockIdx.x;
+  const SetDesc D = desc[gc / SC];
+  const int c = (int)(gc % SC);
+  const int64_t base = D.x_off + (int64_t)c * D.ldg;
+  double k1;
+  const double chi1 = set_chain_misfit(D, XT + base, partial, c, pa, pb, scratch, PT + base, &k1);
+  if (threadIdx.x == 0) {
+    const double H0 = chi0[gc] + k0[gc];
+    const double H1 = chi1 + k1;
+    const double a = exp(H0 - H1);
+    const int ok = (a > u[gc]) ? 1 : 0;   // strict, Samplers.py:1486
+    accepted = ok;
+    chi_new = ok ? chi1 : chi0[gc];
+    if (h0o) h0o[gc] = H0;
+    if (h1o) h1o[gc] = H1;
+    if (acc) acc[gc] = ok;
+    acc_count[gc] += ok;
+    chi0[gc] = chi_new;
+  }
+  __syncthreads();
+  if (accepted)
+    for (int j = threadIdx.x; j < D.d; j += SW) X0[base + j] = XT[base + j];
+  const SetRun R = *run;
+  const int64_t slot = R.prop - R.trace_base;
+  if (tr_x && slot >= 0 && slot < R.trace_cap) {
+    const float* src = accepted ? XT : X0;
+    float* dst = tr_x + slot * tr_row + D.h_off + (int64_t)c * D.d;
+    for (int j = threadIdx.x; j < D.d; j += SW) dst[j] = src[base + j];
+    if (threadIdx.x == 0) tr_chi[slot * nchains + gc] = chi_new;
+  }
+}
+
+__global__ void k_set_advance(SetRun* run) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) run->prop += 1;
+}
+
+__global__ __launch_bounds__(SW) void k_set_finish(const SetDesc* __restrict__ desc,
+                                                   const float* __restrict__ X,
+                                                   const double* __restrict__ partial,
+                                                   const double* __restrict__ pa,
+                                                   const double* __restrict__ pb,
+                                                   double* __restrict__ out) {
+  __shared__ double scratch[SW / 64 + 1];
+  const int64_t gc = blockIdx.x;
+  const SetDesc D = desc[gc / SC];
+  const int c = (int)(gc % SC);
+  const double chi = set_chain_misfit(D, X + D.x_off + (int64_t)c * D.ldg, partial, c, pa, pb, scratch,
+                                      nullptr, nullptr);
+  if (threadIdx.x == 0) out[gc] = chi;
+}
+
+// host-layout f64 rows <-> padded device fp32 states
+__global__ void k_set_load(const SetDesc* __restrict__ desc, const double* __restrict__ src,
+                           float* __restrict__ dst, int64_t total_h) {
+  const int64_t gc = blockIdx.y;
+  const SetDesc D = desc[gc / SC];
+  const int c = (int)(gc % SC);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < D.d; j += gridDim.x * blockDim.x)
+    dst[D.x_off + (int64_t)c * D.ldg + j] = (float)src[D.h_off + (int64_t)c * D.d + j];
+}
+
+__global__ void k_set_store(const SetDesc* __restrict__ desc, const float* __restrict__ src,
+                            double* __restrict__ dst) {
+  const int64_t gc = blockIdx.y;
+  const SetDesc D = desc[gc / SC];
+  const int c = (int)(gc % SC);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < D.d; j += gridDim.x * blockDim.x)
+    dst[D.h_off + (int64_t)c * D.d + j] = (double)src[D.x_off + (int64_t)c * D.ldg + j];
+}
+
+// ---------------------------------------------------------------------------
+// host object
+// ---------------------------------------------------------------------------
+struct SetProblemHost {
+  int64_t d = 0, n = 0;
+  bool has_data = false;
+  int prior_kind = 0;
+};
+
+struct ProblemSet {
+  int64_t P = 0;
+  std::vector<SetProblemHost> hp;
+  std::vector<SetDesc> hd;         // host copy of descriptors
+  SetDesc* desc = nullptr;
+  int64_t g_total = 0, x_total = 0, w_total = 0, v_total = 0, a_total = 0, h_total = 0;
+  float* G = nullptr;
+  double* dobs = nullptr;
+  double* ivar = nullptr;
+  double* pa = nullptr;
+  double* pb = nullptr;
+  float *x0 = nullptr, *xt = nullptr, *pt = nullptr, *w = nullptr;
+  double *chi0 = nullptr, *k0 = nullptr, *eps = nullptr, *u = nullptr, *h0 = nullptr, *h1 = nullptr,
+         *step = nullptr, *p_host = nullptr, *partial = nullptr, *gout = nullptr, *xin = nullptr;
+  int32_t* acc = nullptr;
+  int64_t* acc_count = nullptr;
+  SetRun* run = nullptr;
+  // tiles
+  std::vector<int32_t> order;     // problems by descending L (ties: index)
+  int rw = 1, nw = 4;             // resid rows per wave (x16), grad waves per tile
+  bool tuned = false;
+  int autotune = 1;
+  int2 *rtiles = nullptr, *gtiles = nullptr, *ptiles = nullptr;
+  std::vector<int64_t> rprefix, gprefix;   // tiles of problems with L > s, per step s
+  int64_t n_ptiles = 0;
+  int Lmax = 0;
+  std::vector<int32_t> Lcur;
+  bool finalized = false, state_set = false;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  // trace
+  float* tr_x = nullptr;
+  double* tr_chi = nullptr;
+  int64_t tr_cap = 0, tr_n = 0;
+  // timing: 0 resid, 1 grad, 2 endpoint, 3 per-chain kernels
+  bool timing = false;
+  double t_ms[4] = {0, 0, 0, 0};
+  int64_t t_n[4] = {0, 0, 0, 0};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  hipStream_t stream = nullptr;
+  std::vector<void*> owned;
+  std::vector<void*> tile_mem;
+
+  int64_t chains() const { return P * SC; }
+};
+
+static void set_time(ProblemSet& s, int kind, bool begin, hipEvent_t& e0) {
+  if (!s.timing) return;
+  if (begin) {
+    HMC_HIP(hipEventCreate(&e0));
+    HMC_HIP(hipEventRecord(e0, s.stream));
+  } else {
+    hipEvent_t e1;
+    HMC_HIP(hipEventCreate(&e1));
+    HMC_HIP(hipEventRecord(e1, s.stream));
+    s.ev.push_back({kind, {e0, e1}});
+  }
+}
+
+static void set_collect(ProblemSet& s) {
+  if (s.ev.empty()) return;
+  HMC_HIP(hipStreamSynchronize(s.stream));
+  for (auto& e : s.ev) {
+    float ms = 0.f;
+    HMC_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+    s.t_ms[e.first] += ms;
+    s.t_n[e.first] += 1;
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  s.ev.clear();
+}
+
+static void free_tiles(ProblemSet& s) {
+  for (void* p : s.tile_mem) (void)hipFree(p);
+  s.tile_mem.clear();
+  s.rtiles = s.gtiles = nullptr;
+}
+
+static void drop_graph(ProblemSet& s) {
+  if (s.gexec) (void)hipGraphExecDestroy(s.gexec);
+  if (s.graph) (void)hipGraphDestroy(s.graph);
+  s.gexec = nullptr;
+  s.graph = nullptr;
+}
+
+// Tile lists for the current L and tile shapes: problems by descending L, so
+// the tiles active at step s are a prefix.
+static void build_tiles(ProblemSet& s) {
+  free_tiles(s);
+  s.order.resize(s.P);
+  std::iota(s.order.begin(), s.order.end(), 0);
+  std::stable_sort(s.order.begin(), s.order.end(),
+                   [&](int32_t a, int32_t b) { return s.Lcur[a] > s.Lcur[b]; });
+  std::vector<int2> rt, gt;
+  std::vector<int64_t> rend(s.P), gend(s.P);
+  const int rows = 64 * s.rw;
+  for (int64_t k = 0; k < s.P; ++k) {
+    const int32_t p = s.order[k];
+    const int64_t nr = (s.hp[p].n + rows - 1) / rows, nc = (s.hp[p].d + 63) / 64;
+    for (int64_t b = 0; b < nr; ++b) rt.push_back(make_int2(p, (int)b));
+    for (int64_t b = 0; b < nc; ++b) gt.push_back(make_int2(p, (int)b));
+    rend[k] = (int64_t)rt.size();
+    gend[k] = (int64_t)gt.size();
+  }
+  s.Lmax = s.P ? s.Lcur[s.order[0]] : 0;
+  s.rprefix.assign(s.Lmax, 0);
+  s.gprefix.assign(s.Lmax, 0);
+  for (int st = 0; st < s.Lmax; ++st) {
+    int64_t cnt = 0;
+    while (cnt < s.P && s.Lcur[s.order[cnt]] > st) ++cnt;
+    s.rprefix[st] = cnt ? rend[cnt - 1] : 0;
+    s.gprefix[st] = cnt ? gend[cnt - 1] : 0;
+  }
+  s.rtiles = (int2*)dev_alloc(s.tile_mem, std::max<size_t>(rt.size(), 1) * sizeof(int2));
+  s.gtiles = (int2*)dev_alloc(s.tile_mem, std::max<size_t>(gt.size(), 1) * sizeof(int2));
+  HMC_HIP(hipMemcpy(s.rtiles, rt.data(), rt.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HMC_HIP(hipMemcpy(s.gtiles, gt.data(), gt.size() * sizeof(int2), hipMemcpyHostToDevice));
+  for (int64_t p = 0; p < s.P; ++p) s.hd[p].L = s.Lcur[p];
+  HMC_HIP(hipMemcpy(s.desc, s.hd.data(), s.P * sizeof(SetDesc), hipMemcpyHostToDevice));
+  drop_graph(s);
+}
+
+static void launch_resid(ProblemSet& s, int64_t ntiles, const float* X) {
+  if (ntiles <= 0) return;
+  hipEvent_t e;
+  set_time(s, 0, true, e);
+  if (s.rw == 1)
+    hipLaunchKernelGGL(k_set_resid<1>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+                       s.G, X, s.dobs, s.ivar, s.w);
+  else if (s.rw == 2)
+    hipLaunchKernelGGL(k_set_resid<2>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+                       s.G, X, s.dobs, s.ivar, s.w);
+  else
+    hipLaunchKernelGGL(k_set_resid<4>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+                       s.G, X, s.dobs, s.ivar, s.w);
+  HMC_HIP(hipGetLastError());
+  set_time(s, 0, false, e);
+}
+
+static void launch_grad(ProblemSet& s, int64_t ntiles, float* X, int step, int mode) {
+  if (ntiles <= 0) return;
+  hipEvent_t e;
+  set_time(s, 1, true, e);
+  if (mode == 0) {
+    if (s.nw == 4)
+      hipLaunchKernelGGL((k_set_grad<4, 0>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
+                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr);
+    else
+      hipLaunchKernelGGL((k_set_grad<8, 0>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, s.gtiles,
+                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr);
+  } else {
+    hipLaunchKernelGGL((k_set_grad<4, 1>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
+                       s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, s.gout);
+  }
+  HMC_HIP(hipGetLastError());
+  set_time(s, 1, false, e);
+}
+
+static void launch_endpoint(ProblemSet& s, const float* X) {
+  hipEvent_t e;
+  set_time(s, 2, true, e);
+  hipLaunchKernelGGL(k_set_endpoint, dim3((unsigned)s.n_ptiles), dim3(256), 0, s.stream, s.desc, s.ptiles,
+                     s.G, X, s.dobs, s.ivar, s.partial);
+  HMC_HIP(hipGetLastError());
+  set_time(s, 2, false, e);
+}
+
+// One proposal round on the stream (captured into a graph by run).
+static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
+  const int64_t B = s.chains();
+  hipEvent_t e;
+  set_time(s, 3, true, e);
+  hipLaunchKernelGGL(k_set_refresh, dim3((unsigned)B), dim3(SW), 0, s.stream, s.desc, s.x0, s.xt, s.pt,
+                     s.p_host, s.k0, s.eps, s.u, s.step, s.run);
+  HMC_HIP(hipGetLastError());
+  set_time(s, 3, false, e);
+  for (int st = 0; st < s.Lmax; ++st) {
+    launch_resid(s, s.rprefix[st], s.xt);
+    launch_grad(s, s.gprefix[st], s.xt, st, 0);
+  }
+  launch_endpoint(s, s.xt);
+  set_time(s, 3, true, e);
+  hipLaunchKernelGGL(k_set_metropolis, dim3((unsigned)B), dim3(SW), 0, s.stream, s.desc, s.x0, s.xt, s.pt,
+                     s.partial, s.pa, s.pb, s.chi0, s.k0, s.u, want_energies ? s.h0 : nullptr,
+                     want_energies ? s.h1 : nullptr, s.acc, s.acc_count, s.run, trace ? s.tr_x : nullptr,
+                     s.tr_chi, s.h_total, B);
+  hipLaunchKernelGGL(k_set_advance, dim3(1), dim3(64), 0, s.stream, s.run);
+  HMC_HIP(hipGetLastError());
+  set_time(s, 3, false, e);
+}
+
+static void set_misfit_of(ProblemSet& s, const float* X, double* out) {
+  launch_endpoint(s, X);
+  hipLaunchKernelGGL(k_set_finish, dim3((unsigned)s.chains()), dim3(SW), 0, s.stream, s.desc, X, s.partial,
+                     s.pa, s.pb, out);
+  HMC_HIP(hipGetLastError());
+}
+
+static void finalize(ProblemSet& s) {
+  if (s.finalized) return;
+  for (int64_t p = 0; p < s.P; ++p)
+    HMC_REQUIRE(s.hp[p].has_data, HMC_ERR_STATE, "problem " + std::to_string(p) + " has no data");
+  // endpoint tiles: 64 rows each, problem order
+  std::vector<int2> pt;
+  for (int64_t p = 0; p < s.P; ++p) {
+    s.hd[p].ptile0 = (int32_t)pt.size();
+    const int64_t nr = (s.hp[p].n + 63) / 64;
+    for (int64_t b = 0; b < nr; ++b) pt.push_back(make_int2((int)p, (int)b));
+    s.hd[p].n_ptiles = (int32_t)nr;
+    s.hd[p].prior_kind = s.hp[p].prior_kind;
+  }
+  s.n_ptiles = (int64_t)pt.size();
+  s.ptiles = (int2*)dev_alloc(s.owned, pt.size() * sizeof(int2));
+  HMC_HIP(hipMemcpy(s.ptiles, pt.data(), pt.size() * sizeof(int2), hipMemcpyHostToDevice));
+  s.partial = (double*)dev_alloc(s.owned, (size_t)s.n_ptiles * SC * 8);
+  HMC_HIP(hipMemcpy(s.desc, s.hd.data(), s.P * sizeof(SetDesc), hipMemcpyHostToDevice));
+  s.finalized = true;
+}
+
+static void set_L(ProblemSet& s, const int32_t* L) {
+  bool changed = s.rtiles == nullptr;
+  for (int64_t p = 0; p < s.P; ++p) {
+    HMC_REQUIRE(L[p] >= 1, HMC_ERR_ARG, "amount_of_steps must be >= 1 for every problem");
+    if (s.Lcur[p] != L[p]) changed = true;
+  }
+  if (changed) {
+    std::copy(L, L + s.P, s.Lcur.begin());
+    build_tiles(s);
+  }
+}
+
+// Time every (rw, nw) combination on one trajectory step of this set.
+static void autotune_tiles(ProblemSet& s) {
+  if (s.tuned || !s.autotune) return;
+  const bool saved = s.timing;
+  s.timing = false;
+  std::vector<float> xs;   // scratch states: use xt/pt copies of x0
+  HMC_HIP(hipMemcpyAsync(s.xt, s.x0, s.x_total * 4, hipMemcpyDeviceToDevice, s.stream));
+  HMC_HIP(hipMemsetAsync(s.pt, 0, s.x_total * 4, s.stream));
+  HMC_HIP(hipMemsetAsync(s.eps, 0, s.chains() * 8, s.stream));   // zero step: states unchanged
+  hipEvent_t a, b;
+  HMC_HIP(hipEventCreate(&a));
+  HMC_HIP(hipEventCreate(&b));
+  float best = 1e30f;
+  int brw = 1, bnw = 4;
+  for (int rw : {1, 2, 4})
+    for (int nw : {4, 8}) {
+      s.rw = rw;
+      s.nw = nw;
+      build_tiles(s);
+      for (int rep = 0; rep < 2; ++rep) {   // first pass warms caches and code
+        HMC_HIP(hipEventRecord(a, s.stream));
+        for (int k = 0; k < 3; ++k) {
+          launch_resid(s, s.rprefix[0], s.xt);
+          launch_grad(s, s.gprefix[0], s.xt, 0, 0);
+        }
+        HMC_HIP(hipEventRecord(b, s.stream));
+      }
+      HMC_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      HMC_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best) {
+        best = ms;
+        brw = rw;
+        bnw = nw;
+      }
+    }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  s.rw = brw;
+  s.nw = bnw;
+  build_tiles(s);
+  s.tuned = true;
+  s.timing = saved;
+}
+
+}  // namespace hmc
+
+struct hmc_set {
+  hmc::ProblemSet s;
+};
+
+using namespace hmc;
+
+extern "C" {
+
+int hmc_set_create(int64_t problems, const int64_t* dimensions, const int64_t* data, hmc_set** out) {
+  return guarded("hmc_set_create", [&] {
+    HMC_REQUIRE(out && dimensions && data && problems > 0, HMC_ERR_ARG, "invalid arguments");
+    for (int64_t p = 0; p < problems; ++p)
+      HMC_REQUIRE(dimensions[p] > 0 && data[p] > 0 && dimensions[p] < (1 << 24) && data[p] < (1 << 24),
+                  HMC_ERR_ARG, "problem sizes must be positive (and below 2^24)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      throw Error{HMC_ERR_HIP, "no HIP device is visible"};
+    }
+    auto* h = new hmc_set();
+    ProblemSet& s = h->s;
+    try {
+      HMC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+      s.P = problems;
+      s.hp.resize(problems);
+      s.hd.resize(problems);
+      s.Lcur.assign(problems, 0);
+      for (int64_t p = 0; p < problems; ++p) {
+        SetProblemHost& q = s.hp[p];
+        q.d = dimensions[p];
+        q.n = data[p];
+        SetDesc& D = s.hd[p];
+        std::memset(&D, 0, sizeof(D));
+        D.d = (int32_t)q.d;
+        D.n = (int32_t)q.n;
+        D.ldg = (int32_t)round_up(q.d, 16);
+        D.ldr = (int32_t)round_up(q.n, 16);
+        D.g_off = s.g_total;
+        s.g_total += round_up(q.n, 16) * D.ldg;
+        D.x_off = s.x_total;
+        s.x_total += (int64_t)SC * D.ldg;
+        D.w_off = s.w_total;
+        s.w_total += (int64_t)SC * D.ldr;
+        D.v_off = s.v_total;
+        s.v_total += q.n;
+        D.a_off = s.a_total;
+        s.a_total += q.d;
+        D.h_off = s.h_total;
+        s.h_total += (int64_t)SC * q.d;
+      }
+      const int64_t B = s.chains();
+      // G gets 64 floats of slack: the last gradient tile's guarded float4 loads stay inside
+      s.G = (float*)dev_alloc(s.owned, (s.g_total + 64) * 4);
+      s.dobs = (double*)dev_alloc(s.owned, s.v_total * 8);
+      s.ivar = (double*)dev_alloc(s.owned, s.v_total * 8);
+      s.pa = (double*)dev_alloc(s.owned, s.a_total * 8);
+      s.pb = (double*)dev_alloc(s.owned, s.a_total * 8);
+      s.x0 = (float*)dev_alloc(s.owned, s.x_total * 4);
+      s.xt = (float*)dev_alloc(s.owned, s.x_total * 4);
+      s.pt = (float*)dev_alloc(s.owned, s.x_total * 4);
+      s.w = (float*)dev_alloc(s.owned, s.w_total * 4);
+      s.chi0 = (double*)dev_alloc(s.owned, B * 8);
+      s.k0 = (double*)dev_alloc(s.owned, B * 8);
+      s.eps = (double*)dev_alloc(s.owned, B * 8);
+      s.u = (double*)dev_alloc(s.owned, B * 8);
+      s.h0 = (double*)dev_alloc(s.owned, B * 8);
+      s.h1 = (double*)dev_alloc(s.owned, B * 8);
+      s.step = (double*)dev_alloc(s.owned, B * 8);
+      s.acc = (int32_t*)dev_alloc(s.owned, B * 4);
+      s.acc_count = (int64_t*)dev_alloc(s.owned, B * 8);
+      s.p_host = (double*)dev_alloc(s.owned, s.h_total * 8);
+      s.xin = (double*)dev_alloc(s.owned, s.h_total * 8);
+      s.gout = (double*)dev_alloc(s.owned, s.h_total * 8);
+      s.run = (SetRun*)dev_alloc(s.owned, sizeof(SetRun));
+      s.desc = (SetDesc*)dev_alloc(s.owned, problems * sizeof(SetDesc));
+    } catch (...) {
+      for (void* p : s.owned) (void)hipFree(p);
+      if (s.stream) (void)hipStreamDestroy(s.stream);
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int hmc_set_destroy(hmc_set* h) {
+  return guarded("hmc_set_destroy", [&] {
+    if (!h) return;
+    ProblemSet& s = h->s;
+    (void)hipStreamSynchronize(s.stream);
+    drop_graph(s);
+    free_tiles(s);
+    for (auto& e : s.ev) {
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+    for (void* p : s.owned) (void)hipFree(p);
+    (void)hipStreamDestroy(s.stream);
+    delete h;
+  });
+}
+
+int hmc_set_problem_dense(hmc_set* h, int64_t i, const float* G, const double* d_obs, const double* variance,
+                          int scalar_variance) {
+  return guarded("hmc_set_problem_dense", [&] {
+    HMC_REQUIRE(h && G && d_obs && variance, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(i >= 0 && i < s.P, HMC_ERR_ARG, "problem index out of range");
+    HMC_REQUIRE(!s.finalized, HMC_ERR_STATE, "the set is already in use; data cannot change");
+    const SetDesc& D = s.hd[i];
+    const int64_t n = s.hp[i].n, d = s.hp[i].d;
+    std::vector<float> g((size_t)round_up(n, 16) * D.ldg, 0.f);
+    for (int64_t r = 0; r < n; ++r) std::memcpy(&g[r * D.ldg], G + r * d, d * 4);
+    std::vector<double> iv(n);
+    for (int64_t r = 0; r < n; ++r) {
+      const double v = scalar_variance ? variance[0] : variance[r];
+      HMC_REQUIRE(v > 0.0 && std::isfinite(v), HMC_ERR_ARG, "data variances must be positive and finite");
+      iv[r] = 1.0 / v;
+    }
+    HMC_HIP(hipMemcpy(s.G + D.g_off, g.data(), g.size() * 4, hipMemcpyHostToDevice));
+    HMC_HIP(hipMemcpy(s.dobs + D.v_off, d_obs, n * 8, hipMemcpyHostToDevice));
+    HMC_HIP(hipMemcpy(s.ivar + D.v_off, iv.data(), n * 8, hipMemcpyHostToDevice));
+    s.hp[i].has_data = true;
+  });
+}
+
+// kind: HMC_PRIOR_NORMAL (a = means, b = inverse variances) or
+// HMC_PRIOR_LAPLACE (a = means, b = inverse dispersions); one prior per problem.
+int hmc_set_problem_prior(hmc_set* h, int64_t i, int kind, const double* a, const double* b) {
+  return guarded("hmc_set_problem_prior", [&] {
+    HMC_REQUIRE(h && a && b, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(i >= 0 && i < s.P, HMC_ERR_ARG, "problem index out of range");
+    HMC_REQUIRE(kind == HMC_PRIOR_NORMAL || kind == HMC_PRIOR_LAPLACE, HMC_ERR_ARG, "unknown prior kind");
+    HMC_REQUIRE(!s.finalized, HMC_ERR_STATE, "the set is already in use; priors cannot change");
+    const int64_t d = s.hp[i].d;
+    HMC_HIP(hipMemcpy(s.pa + s.hd[i].a_off, a, d * 8, hipMemcpyHostToDevice));
+    HMC_HIP(hipMemcpy(s.pb + s.hd[i].a_off, b, d * 8, hipMemcpyHostToDevice));
+    s.hp[i].prior_kind = kind;
+  });
+}
+
+// "autotune" (1: time the tile shapes on first use, 0: keep the set ones),
+// "resid_rows" (64, 128, 256 rows per residual tile), "grad_waves" (4 or 8).
+int hmc_set_set_option(hmc_set* h, const char* name, double value) {
+  return guarded("hmc_set_set_option", [&] {
+    HMC_REQUIRE(h && name, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    const std::string k(name);
+    if (k == "autotune") {
+      s.autotune = value != 0.0;
+    } else if (k == "resid_rows") {
+      HMC_REQUIRE(value == 64 || value == 128 || value == 256, HMC_ERR_ARG, "resid_rows must be 64, 128 or 256");
+      s.rw = (int)value / 64;
+      s.autotune = 0;
+    } else if (k == "grad_waves") {
+      HMC_REQUIRE(value == 4 || value == 8, HMC_ERR_ARG, "grad_waves must be 4 or 8");
+      s.nw = (int)value;
+      s.autotune = 0;
+    } else {
+      throw Error{HMC_ERR_ARG, "unknown option '" + k + "'"};
+    }
+    if (s.rtiles) build_tiles(s);
+  });
+}
+
+int hmc_set_tile_config(hmc_set* h, int* resid_rows, int* grad_waves) {
+  return guarded("hmc_set_tile_config", [&] {
+    HMC_REQUIRE(h && resid_rows && grad_waves, HMC_ERR_ARG, "NULL argument");
+    *resid_rows = 64 * h->s.rw;
+    *grad_waves = h->s.nw;
+  });
+}
+
+// X: host layout, problem after problem, 16 rows (chains) of d_p values each.
+int hmc_set_set_state(hmc_set* h, const double* X) {
+  return guarded("hmc_set_set_state", [&] {
+    HMC_REQUIRE(h && X, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    finalize(s);
+    HMC_HIP(hipMemcpyAsync(s.xin, X, s.h_total * 8, hipMemcpyHostToDevice, s.stream));
+    hipLaunchKernelGGL(k_set_load, dim3(8, (unsigned)s.chains()), dim3(256), 0, s.stream, s.desc, s.xin, s.x0,
+                       s.h_total);
+    HMC_HIP(hipGetLastError());
+    set_misfit_of(s, s.x0, s.chi0);
+    std::vector<double> chi(s.chains());
+    HMC_HIP(hipMemcpyAsync(chi.data(), s.chi0, chi.size() * 8, hipMemcpyDeviceToHost, s.stream));
+    HMC_HIP(hipStreamSynchronize(s.stream));
+    for (double c : chi)
+      HMC_REQUIRE(std::isfinite(c), HMC_ERR_ARG, "initial model has a non-finite misfit");
+    s.state_set = true;
+  });
+}
+
+int hmc_set_get_state(hmc_set* h, double* X, double* chi) {
+  return guarded("hmc_set_get_state", [&] {
+    HMC_REQUIRE(h, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(s.state_set, HMC_ERR_STATE, "state not set");
+    if (X) {
+      hipLaunchKernelGGL(k_set_store, dim3(8, (unsigned)s.chains()), dim3(256), 0, s.stream, s.desc, s.x0,
+                         s.xin);
+      HMC_HIP(hipGetLastError());
+      HMC_HIP(hipMemcpyAsync(X, s.xin, s.h_total * 8, hipMemcpyDeviceToHost, s.stream));
+    }
+    if (chi) HMC_HIP(hipMemcpyAsync(chi, s.chi0, s.chains() * 8, hipMemcpyDeviceToHost, s.stream));
+    HMC_HIP(hipStreamSynchronize(s.stream));
+  });
+}
+
+// misfit (P*16) and gradient (host layout) at positions X (host layout);
+// positions are rounded to the fp32 state first.
+int hmc_set_misfit(hmc_set* h, const double* X, double* out) {
+  return guarded("hmc_set_misfit", [&] {
+    HMC_REQUIRE(h && X && out, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    finalize(s);
+    HMC_HIP(hipMemcpyAsync(s.xin, X, s.h_total * 8, hipMemcpyHostToDevice, s.stream));
+    hipLaunchKernelGGL(k_set_load, dim3(8, (unsigned)s.chains()), dim3(256), 0, s.stream, s.desc, s.xin, s.xt,
+                       s.h_total);
+    set_misfit_of(s, s.xt, s.h0);
+    HMC_HIP(hipMemcpyAsync(out, s.h0, s.chains() * 8, hipMemcpyDeviceToHost, s.stream));
+    HMC_HIP(hipStreamSynchronize(s.stream));
+  });
+}
+
+int hmc_set_gradient(hmc_set* h, const double* X, double* out) {
+  return guarded("hmc_set_gradient", [&] {
+    HMC_REQUIRE(h && X && out, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    finalize(s);
+    if (!s.rtiles) {
+      std::vector<int32_t> ones(s.P, 1);
+      set_L(s, ones.data());
+    }
+    HMC_HIP(hipMemcpyAsync(s.xin, X, s.h_total * 8, hipMemcpyHostToDevice, s.stream));
+    hipLaunchKernelGGL(k_set_load, dim3(8, (unsigned)s.chains()), dim3(256), 0, s.stream, s.desc, s.xin, s.xt,
+                       s.h_total);
+    const int64_t nr = s.rprefix.empty() ? 0 : s.rprefix[0], ng = s.gprefix.empty() ? 0 : s.gprefix[0];
+    launch_resid(s, nr, s.xt);
+    launch_grad(s, ng, s.xt, 0, 1);
+    HMC_HIP(hipMemcpyAsync(out, s.gout, s.h_total * 8, hipMemcpyDeviceToHost, s.stream));
+    HMC_HIP(hipStreamSynchronize(s.stream));
+  });
+}
+
+// One proposal with host randoms (parity with the single-chain reference loop):
+// P0 host layout, eps/u per chain, L per problem.
+int hmc_set_propose(hmc_set* h, const double* P0, const double* eps, const double* u, const int32_t* L,
+                    int32_t* accepted, double* H0, double* H1, double* X1) {
+  return guarded("hmc_set_propose", [&] {
+    HMC_REQUIRE(h && P0 && eps && u && L, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(s.state_set, HMC_ERR_STATE, "state not set");
+    set_L(s, L);
+    const int64_t B = s.chains();
+    SetRun R{};
+    R.host_p = 1;
+    R.trace_cap = 0;
+    R.prop = 0;
+    HMC_HIP(hipMemcpyAsync(s.run, &R, sizeof(R), hipMemcpyHostToDevice, s.stream));
+    HMC_HIP(hipMemcpyAsync(s.p_host, P0, s.h_total * 8, hipMemcpyHostToDevice, s.stream));
+    HMC_HIP(hipMemcpyAsync(s.eps, eps, B * 8, hipMemcpyHostToDevice, s.stream));
+    HMC_HIP(hipMemcpyAsync(s.u, u, B * 8, hipMemcpyHostToDevice, s.stream));
+    enqueue_proposal(s, true, false);
+    if (X1) {
+      hipLaunchKernelGGL(k_set_store, dim3(8, (unsigned)B), dim3(256), 0, s.stream, s.desc, s.xt, s.xin);
+      HMC_HIP(hipGetLastError());
+      HMC_HIP(hipMemcpyAsync(X1, s.xin, s.h_total * 8, hipMemcpyDeviceToHost, s.stream));
+    }
+    if (accepted) HMC_HIP(hipMemcpyAsync(accepted, s.acc, B * 4, hipMemcpyDeviceToHost, s.stream));
+    if (H0) HMC_HIP(hipMemcpyAsync(H0, s.h0, B * 8, hipMemcpyDeviceToHost, s.stream));
+    if (H1) HMC_HIP(hipMemcpyAsync(H1, s.h1, B * 8, hipMemcpyDeviceToHost, s.stream));
+    HMC_HIP(hipStreamSynchronize(s.stream));
+    set_collect(s);
+  });
+}
+
+int hmc_set_set_trace(hmc_set* h, int64_t capacity) {
+  return guarded("hmc_set_set_trace", [&] {
+    HMC_REQUIRE(h && capacity >= 0, HMC_ERR_ARG, "invalid argument");
+    ProblemSet& s = h->s;
+    HMC_HIP(hipStreamSynchronize(s.stream));
+    if (s.tr_x) {
+      auto it = std::find(s.owned.begin(), s.owned.end(), (void*)s.tr_x);
+      if (it != s.owned.end()) s.owned.erase(it);
+      (void)hipFree(s.tr_x);
+      it = std::find(s.owned.begin(), s.owned.end(), (void*)s.tr_chi);
+      if (it != s.owned.end()) s.owned.erase(it);
+      (void)hipFree(s.tr_chi);
+      s.tr_x = nullptr;
+      s.tr_chi = nullptr;
+    }
+    s.tr_cap = capacity;
+    s.tr_n = 0;
+    if (capacity > 0) {
+      s.tr_x = (float*)dev_alloc(s.owned, (size_t)capacity * s.h_total * 4);
+      s.tr_chi = (double*)dev_alloc(s.owned, (size_t)capacity * s.chains() * 8);
+    }
+    drop_graph(s);
+  });
+}
+
+int hmc_set_trace_count(hmc_set* h, int64_t* n) {
+  return guarded("hmc_set_trace_count", [&] {
+    HMC_REQUIRE(h && n, HMC_ERR_ARG, "NULL argument");
+    *n = h->s.tr_n;
+  });
+}
+
+// X [count][h_total] (f64, host layout), chi [count][P*16]
+int hmc_set_read_trace(hmc_set* h, int64_t first, int64_t count, double* X, double* chi) {
+  return guarded("hmc_set_read_trace", [&] {
+    HMC_REQUIRE(h, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(first >= 0 && count >= 0 && first + count <= s.tr_n, HMC_ERR_ARG, "trace range out of bounds");
+    HMC_HIP(hipStreamSynchronize(s.stream));
+    if (X) {
+      std::vector<float> buf((size_t)s.h_total);
+      for (int64_t k = 0; k < count; ++k) {
+        HMC_HIP(hipMemcpy(buf.data(), s.tr_x + (first + k) * s.h_total, s.h_total * 4, hipMemcpyDeviceToHost));
+        for (int64_t j = 0; j < s.h_total; ++j) X[k * s.h_total + j] = (double)buf[j];
+      }
+    }
+    if (chi)
+      HMC_HIP(hipMemcpy(chi, s.tr_chi + first * s.chains(), count * s.chains() * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+// Device-RNG proposals: every proposal is one replay of a captured graph
+// (use_graph = 1) or the same launches issued directly (0).  stepsize [P*16]
+// per chain; L [P]; accepted [P*16] receives the accepted count of the call.
+int hmc_set_run(hmc_set* h, int64_t proposals, const int32_t* L, const double* stepsize, int randomize,
+                uint64_t seed, int64_t first_proposal, int use_graph, int64_t* accepted) {
+  return guarded("hmc_set_run", [&] {
+    HMC_REQUIRE(h && L && stepsize && proposals >= 0, HMC_ERR_ARG, "invalid arguments");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(s.state_set, HMC_ERR_STATE, "state not set");
+    set_L(s, L);
+    autotune_tiles(s);
+    const int64_t B = s.chains();
+    SetRun R{};
+    R.seed = seed;
+    R.prop = first_proposal;
+    R.trace_base = first_proposal;
+    R.trace_cap = s.tr_x ? s.tr_cap : 0;
+    R.randomize = randomize ? 1 : 0;
+    R.host_p = 0;
+    HMC_HIP(hipMemcpyAsync(s.run, &R, sizeof(R), hipMemcpyHostToDevice, s.stream));
+    HMC_HIP(hipMemcpyAsync(s.step, stepsize, B * 8, hipMemcpyHostToDevice, s.stream));
+    HMC_HIP(hipMemsetAsync(s.acc_count, 0, B * 8, s.stream));
+    const bool trace = s.tr_x != nullptr;
+    if (use_graph && !s.timing) {
+      if (!s.gexec) {
+        HMC_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_proposal(s, false, trace);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s.stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HMC_HIP(hipStreamEndCapture(s.stream, &s.graph));
+        HMC_HIP(hipGraphInstantiate(&s.gexec, s.graph, nullptr, nullptr, 0));
+      }
+      for (int64_t k = 0; k < proposals; ++k) HMC_HIP(hipGraphLaunch(s.gexec, s.stream));
+    } else {
+      for (int64_t k = 0; k < proposals; ++k) enqueue_proposal(s, false, trace);
+    }
+    if (accepted) HMC_HIP(hipMemcpyAsync(accepted, s.acc_count, B * 8, hipMemcpyDeviceToHost, s.stream));
+    HMC_HIP(hipStreamSynchronize(s.stream));
+    set_collect(s);
+    if (trace) s.tr_n = std::min(s.tr_cap, proposals);
+  });
+}
+
+int hmc_set_enable_timing(hmc_set* h, int on) {
+  return guarded("hmc_set_enable_timing", [&] {
+    HMC_REQUIRE(h, HMC_ERR_ARG, "NULL argument");
+    h->s.timing = on != 0;
+    for (int k = 0; k < 4; ++k) {
+      h->s.t_ms[k] = 0.0;
+      h->s.t_n[k] = 0;
+    }
+  });
+}
+
+// kind 0: residual launches, 1: gradient/kick-drift launches, 2: endpoint, 3: per-chain kernels
+int hmc_set_timing(hmc_set* h, int kind, double* total_ms, int64_t* launches) {
+  return guarded("hmc_set_timing", [&] {
+    HMC_REQUIRE(h && total_ms && launches && kind >= 0 && kind < 4, HMC_ERR_ARG, "invalid argument");
+    set_collect(h->s);
+    *total_ms = h->s.t_ms[kind];
+    *launches = h->s.t_n[kind];
+  });
+}
+
+}  // extern "C"

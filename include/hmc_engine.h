@@ -116,8 +116,10 @@ int hmc_problem_set_mass(hmc_problem* p, int kind, const double* diagonal);
 int hmc_problem_set_engine(hmc_problem* p, int engine);
 
 /* Engine tuning knobs (no counterpart in the reference):
- *   "bf16_split"  1 or 2: bf16 planes of the FP32 operand of the trajectory
+ *   "bf16_split"  1 or 2: bf16 planes of the FP32 operand of both trajectory
  *                 GEMMs (2 = hi/lo split, ~16-bit operand precision; default 2)
+ *   "bf16_split_grad" 1 or 2: the same for the gradient GEMM only (its operand
+ *                 is the scaled residual)
  *   "gemm_config" 0 or 1: bf16 GEMM tile (0 = 128x128, 4 waves of 64x64;
  *                 1 = 128x128, 8 waves of 64x32; default 1)
  * Must be set before hmc_chains_create. */
@@ -238,6 +240,51 @@ int hmc_surrogate_train(hmc_surrogate* s, const double* X, const double* chi, co
                         double beta, uint64_t seed, double* loss_history);
 /* Attach (or with NULL detach) a surrogate to chains of a dense problem. */
 int hmc_chains_set_surrogate(hmc_chains* c, hmc_surrogate* s);
+
+/* ---- sets of independent problems -----------------------------------------
+ * Replaces running one HMC.sample per problem and per chain in separate
+ * processes (ParallelSampleSMP, Samplers.py:1757-1977, around
+ * HMC._propagate_leapfrog :1524-1584 and HMC._evaluate_acceptance
+ * :1471-1492): P dense linear problems of uneven size (G float32, n_p x d_p),
+ * each with one Normal or Laplace prior (base.py:451-574, 653-710), Unit
+ * mass (MassMatrices.py:82-156), leapfrog with per-problem step counts, and
+ * 16 chains per problem, advance together.  Trajectories run in float32 on
+ * MFMA; the endpoint misfit and Metropolis test are FP64.
+ * Host layout of states, momenta and gradients: problem after problem, 16
+ * rows (chains) of d_p doubles each.  Chain index gc = p * 16 + c. */
+typedef struct hmc_set hmc_set;
+
+int hmc_set_create(int64_t problems, const int64_t* dimensions, const int64_t* data, hmc_set** out);
+int hmc_set_destroy(hmc_set* s);
+/* G row-major n x d float32; variance scalar (scalar_variance=1) or [n]. */
+int hmc_set_problem_dense(hmc_set* s, int64_t problem, const float* G, const double* d_obs,
+                          const double* variance, int scalar_variance);
+/* HMC_PRIOR_NORMAL (a = means, b = inverse variances) or HMC_PRIOR_LAPLACE
+ * (a = means, b = inverse dispersions), [d] each. */
+int hmc_set_problem_prior(hmc_set* s, int64_t problem, int kind, const double* a, const double* b);
+/* "autotune" 1|0 (time the tile shapes on this set at first run; default 1),
+ * "resid_rows" 64|128|256, "grad_waves" 4|8 (fix a shape, no autotune). */
+int hmc_set_set_option(hmc_set* s, const char* name, double value);
+int hmc_set_tile_config(hmc_set* s, int* resid_rows, int* grad_waves);
+int hmc_set_set_state(hmc_set* s, const double* X);
+int hmc_set_get_state(hmc_set* s, double* X, double* chi);
+/* misfit [P*16] / gradient (host layout) at X (host layout, rounded to float32) */
+int hmc_set_misfit(hmc_set* s, const double* X, double* out);
+int hmc_set_gradient(hmc_set* s, const double* X, double* out);
+/* One proposal with host momenta P0, local steps eps [P*16], accept uniforms
+ * u [P*16] and step counts L [P]; outputs may be NULL. */
+int hmc_set_propose(hmc_set* s, const double* P0, const double* eps, const double* u, const int32_t* L,
+                    int32_t* accepted, double* H0, double* H1, double* X1);
+/* Device-RNG proposals; each one a replay of a captured hipGraph when
+ * use_graph != 0.  stepsize [P*16], L [P]; accepted [P*16] counts accepts. */
+int hmc_set_run(hmc_set* s, int64_t proposals, const int32_t* L, const double* stepsize, int randomize,
+                uint64_t seed, int64_t first_proposal, int use_graph, int64_t* accepted);
+int hmc_set_set_trace(hmc_set* s, int64_t capacity);
+int hmc_set_trace_count(hmc_set* s, int64_t* n);
+int hmc_set_read_trace(hmc_set* s, int64_t first, int64_t count, double* X, double* chi);
+int hmc_set_enable_timing(hmc_set* s, int on);
+/* kind 0 residual, 1 gradient + kick/drift, 2 endpoint misfit, 3 per-chain kernels */
+int hmc_set_timing(hmc_set* s, int kind, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,180 @@
+"""GPU parity of the problem-set engine (many uneven dense problems, 16
+chains each) against the oracle, problem by problem, through the C-ABI.
+
+Tolerances, relative to the oracle's float64 values and scaled by each
+array's largest magnitude: misfits 1e-10 (FP64 endpoint on the float32
+state and float32 G, which the oracle holds exactly); gradients, trajectory
+end points and end energies 1e-4 (float32 trajectory arithmetic).  Accept
+decisions must agree except for near-ties |log u - (H0 - H1)| < 1e-3."""
+import numpy as np
+import pytest
+
+from hmclab_amd.problem_set import CHAINS, ProblemSet, uneven_problems
+from oracle import hmc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _build(seed, dims, ratios=None, noise=0.05):
+    rng = np.random.default_rng(seed)
+    dims = np.asarray(dims)
+    ratios = np.full(len(dims), 1.5) if ratios is None else np.asarray(ratios)
+    n = np.maximum(1, np.round(ratios * dims)).astype(int)
+    ps = ProblemSet(dims, n)
+    orc, specs = [], []
+    for p, (d, nn) in enumerate(zip(dims, n)):
+        G = (rng.standard_normal((nn, d)) / np.sqrt(d)).astype(np.float32)
+        dd = G.astype(np.float64) @ rng.normal(size=d) + rng.normal(0, noise, size=nn)
+        mu = rng.normal(size=d) * 0.1
+        if p % 2 == 0:
+            ps.set_problem(p, G, dd, noise ** 2, prior="normal", means=mu, scale=1.0)
+            prior = O.Normal(mu.reshape(d, 1), 1.0)
+            specs.append(dict(G=G, d_obs=dd, variance=noise ** 2, prior="normal", scale=1.0))
+        else:
+            ps.set_problem(p, G, dd, noise ** 2, prior="laplace", means=mu, scale=0.5)
+            prior = O.Laplace(mu.reshape(d, 1), np.full((d, 1), 0.5))
+            specs.append(dict(G=G, d_obs=dd, variance=noise ** 2, prior="laplace", scale=0.5))
+        lik = O.LinearMatrixSimple(G.astype(np.float64), dd.reshape(nn, 1), noise ** 2, dtype=np.float64,
+                                   premultiplication=False)
+        orc.append(O.BayesRule([prior, lik]))
+    ps.specs = specs
+    return ps, orc, rng
+
+
+def _start_and_steps(ps):
+    """Ridge start (16 copies per problem) and per-chain steps 0.5/sqrt(lam_max)."""
+    from hmclab_amd.problem_set import curvature_and_start
+
+    starts, steps = [], []
+    for spec in ps.specs:
+        lam, x = curvature_and_start(spec)
+        starts.append(np.repeat(x[None, :], CHAINS, axis=0))
+        steps.append(np.full(CHAINS, 0.5 / np.sqrt(lam)))
+    return ps.join(starts), np.concatenate(steps)
+
+
+DIMS = [3, 17, 50, 130, 257, 64]
+RATIOS = [1.5, 1.5, 0.4, 1.5, 1.5, 3.0]
+
+
+def test_misfit_and_gradient_match_oracle():
+    ps, orc, rng = _build(1, DIMS, RATIOS)
+    X = ps.join([rng.normal(size=(CHAINS, d)) * 0.3 for d in DIMS])
+    Xf = X.astype(np.float32).astype(np.float64)
+    chi = ps.misfit(X)
+    g = ps.split(ps.gradient(X))
+    xs = ps.split(Xf)
+    for p, o in enumerate(orc):
+        for c in range(CHAINS):
+            x = xs[p][c].reshape(-1, 1)
+            assert _rel(chi[p * CHAINS + c], o.misfit(x)) < 1e-10, (p, c)
+            assert _rel(g[p][c], o.gradient(x)[:, 0]) < 1e-4, (p, c)
+
+
+@pytest.mark.parametrize("Ls", [[1, 1, 1, 1, 1, 1], [3, 12, 1, 7, 5, 9]])
+def test_teacher_forced_proposal_matches_oracle(Ls):
+    ps, orc, rng = _build(2, DIMS, RATIOS)
+    ps.set_state(ps.join([rng.normal(size=(CHAINS, d)) * 0.2 for d in DIMS]))
+    X0, chi0 = ps.get_state()
+    # momenta live in float32 on the device: give the oracle the same values
+    P0 = ps.join([rng.normal(size=(CHAINS, d)).astype(np.float32).astype(np.float64) for d in DIMS])
+    eps = 0.01 * rng.uniform(0.5, 1.5, size=ps.chains)
+    u = rng.uniform(size=ps.chains)
+    res = ps.propose(P0, eps, u, Ls)
+    xs, pz, x1 = ps.split(X0), ps.split(P0), ps.split(res["X1"])
+    near = 0
+    for p, o in enumerate(orc):
+        d = DIMS[p]
+        unit = O.Unit(d, np.random.default_rng(0))
+        for c in range(CHAINS):
+            gc = p * CHAINS + c
+            ref = O.propose(o, unit, xs[p][c].reshape(d, 1), None, pz[p][c].reshape(d, 1), eps[gc], Ls[p], u[gc])
+            assert _rel(chi0[gc], ref["chi0"]) < 1e-10
+            assert _rel(res["H0"][gc], ref["h0"]) < 1e-10
+            assert _rel(x1[p][c], ref["x1"][:, 0]) < 1e-4, (p, c)
+            assert _rel(res["H1"][gc], ref["h1"]) < 1e-4, (p, c)
+            if abs(np.log(u[gc]) - (ref["h0"] - ref["h1"])) < 1e-3:
+                near += 1
+            else:
+                assert bool(res["accepted"][gc]) == ref["accepted"], (p, c)
+    assert near <= 4
+
+
+def test_graph_replays_match_direct_launches_and_tile_shapes():
+    dims = [40, 200, 75, 310]
+    out = []
+    for graph, rows, waves in [(True, 64, 4), (False, 64, 4), (True, 128, 8), (True, 256, 4)]:
+        ps, _, rng = _build(3, dims)
+        ps.set_option("resid_rows", rows)
+        ps.set_option("grad_waves", waves)
+        X0, step = _start_and_steps(ps)
+        ps.set_state(X0)
+        acc = ps.run(5, [6, 11, 3, 8], step, seed=9, graph=graph)
+        X1, _ = ps.get_state()                   # eager work between the replays
+        acc2 = ps.run(5, [6, 11, 3, 8], step, seed=9, first_proposal=5, graph=graph)
+        X2, chi2 = ps.get_state()
+        out.append((acc, acc2, X1, X2, chi2, ps.tile_config()))
+        ps.close()
+    ref = out[0]
+    assert ref[0].sum() > 0.3 * 5 * len(ref[0]) and ref[1].sum() > 0.3 * 5 * len(ref[1])
+    # graph replays and direct launches of the same shapes are bitwise identical
+    assert np.array_equal(out[1][2], ref[2]) and np.array_equal(out[1][3], ref[3])
+    assert np.array_equal(out[1][0], ref[0]) and np.array_equal(out[1][1], ref[1])
+    # other tile shapes change only the summation order of the gradient
+    for o in out[2:]:
+        assert _rel(o[3], ref[3]) < 1e-3
+        assert np.abs(o[1] - ref[1]).sum() <= 4
+
+
+def test_device_rng_posterior_matches_analytic_for_normal_problems():
+    dims = [20, 48]
+    ps, orc, rng = _build(4, dims, noise=0.3)
+    # start at the analytic posterior means of the Normal-prior problem 0; problem 1 (Laplace) near 0
+    ps.set_state(ps.join([np.zeros((CHAINS, d)) for d in dims]))
+    o = orc[0]
+    lik = o.separate_distributions[1]
+    G = lik.G.astype(np.float64)
+    prior_mu = o.separate_distributions[0].means[:, 0]
+    H = G.T @ G / 0.3 ** 2 + np.eye(dims[0])
+    Sigma = np.linalg.inv(H)
+    mu = Sigma @ (G.T @ lik.d[:, 0] / 0.3 ** 2 + prior_mu)
+    eps = 0.5 / np.sqrt(np.linalg.eigvalsh(H).max())
+    ps.set_trace(400)
+    step = np.full(ps.chains, eps)
+    ps.run(50, [10, 10], step, seed=11)
+    ps.set_trace(400)
+    acc = ps.run(400, [10, 10], step, seed=11, first_proposal=50)
+    assert acc[:CHAINS].mean() / 400 > 0.5
+    X, _ = ps.read_trace(0, 400)
+    s0 = np.stack([ps.split(x)[0] for x in X])          # (400, 16, d0)
+    flat = s0.reshape(-1, dims[0])
+    sd = np.sqrt(np.diag(Sigma))
+    err = np.abs(flat.mean(axis=0) - mu) / sd
+    assert np.sqrt(np.mean(err ** 2)) < 0.15            # 6400 correlated draws
+    assert np.all(np.abs(flat.std(axis=0) / sd - 1) < 0.15)
+
+
+def test_uneven_workload_builder_and_autotune():
+    d, n, L, specs = uneven_problems(seed=0, problems=12, d_max=300)
+    assert np.all((d >= 50) & (d <= 300)) and np.all(n == np.round(1.5 * d))
+    assert np.all((L >= 5) & (L <= 30))
+    from hmclab_amd.problem_set import curvature_and_start
+
+    ps = ProblemSet(d, n)
+    starts, steps = [], []
+    for p, s in enumerate(specs):
+        ps.set_problem(p, s["G"], s["d_obs"], s["variance"], prior=s["prior"], scale=s["scale"])
+        lam, x = curvature_and_start(s)
+        starts.append(np.repeat(x[None, :], CHAINS, axis=0))
+        steps.append(np.full(CHAINS, 0.5 / np.sqrt(lam)))
+    ps.set_state(ps.join(starts))
+    acc = ps.run(6, L, np.concatenate(steps), seed=1)
+    cfg = ps.tile_config()
+    assert cfg["resid_rows"] in (64, 128, 256) and cfg["grad_waves"] in (4, 8)
+    rate = acc.reshape(12, CHAINS).sum(axis=1) / (6 * CHAINS)
+    assert np.all(rate > 0.3), rate

@@ -155,20 +155,25 @@ def test_surrogate_trajectory_matches_oracle_with_exact_metropolis():
     X0 = rng.normal(size=(B, d)) * 0.3
     ch.set_state(X0)
     X0s, chi0 = ch.get_state()
+    unit = O.Unit(d, np.random.default_rng(0))
+    for k in range(B):
+        assert abs(chi0[k] - exact_o.misfit(X0s[k].reshape(d, 1))) <= 1e-10 * abs(chi0[k]), k
     P0 = rng.normal(size=(B, d))
     e = 0.01 * rng.uniform(0.5, 1.5, size=B)
     u = rng.uniform(size=B)
     res = ch.propose(P0, e, u, L, "lf", trajectory=True)
+    _, chi0_after = ch.get_state()
     sur_o = _SurrogateTarget(exact_o, prior_o, P, xm, xs, 100.0, 40.0)
-    unit = O.Unit(d, np.random.default_rng(0))
     for k in range(B):
         x0 = X0s[k].reshape(d, 1)
         x1, p1 = O.leapfrog(sur_o, unit, x0, P0[k].reshape(d, 1), e[k], L)
-        h0 = exact_o.misfit(x0) + unit.kinetic_energy(P0[k].reshape(d, 1))
+        k0 = unit.kinetic_energy(P0[k].reshape(d, 1))
+        h0 = exact_o.misfit(x0) + k0
         h1 = exact_o.misfit(x1) + unit.kinetic_energy(p1)
         assert _rel(res["X1"][k], x1[:, 0]) < 1e-4
         assert _rel(res["P1"][k], p1[:, 0]) < 1e-4
-        assert abs(res["H0"][k] - h0) <= 1e-10 * abs(h0)        # start: exact FP64
+        assert abs(res["H0"][k] - h0) <= 1e-10 * abs(h0), (k, res["H0"][k], h0, chi0[k], k0, res["H1"][k], h1,
+                                                            chi0_after[k])
         assert abs(res["H1"][k] - h1) <= 1e-6 * abs(h1)          # end: exact misfit at a 1e-4 position
         if abs(np.log(u[k]) - (h0 - h1)) > 1e-3:
             assert bool(res["accepted"][k]) == bool(np.exp(h0 - h1) > u[k])
