@@ -25,9 +25,29 @@ void* dev_alloc(std::vector<void*>& owned, size_t bytes) {
     throw Error{HMC_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) +
                                    " bytes failed: " + hipGetErrorString(e)};
   }
-  HMC_HIP(hipMemset(ptr, 0, bytes));
+  const hipError_t z = hipMemset(ptr, 0, bytes);
+  const hipError_t w = (z == hipSuccess) ? hipStreamSynchronize(nullptr) : z;
+  if (w != hipSuccess) {
+    (void)hipFree(ptr);
+    HMC_HIP(w);
+  }
   owned.push_back(ptr);
   return ptr;
+}
+
+void h2d(void* dst, const void* src, size_t bytes) {
+  HMC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  HMC_HIP(hipStreamSynchronize(nullptr));
+}
+
+void d2h(void* dst, const void* src, size_t bytes) {
+  HMC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  HMC_HIP(hipStreamSynchronize(nullptr));
+}
+
+void dzero(void* dst, size_t bytes) {
+  HMC_HIP(hipMemset(dst, 0, bytes));
+  HMC_HIP(hipStreamSynchronize(nullptr));
 }
 
 static void* upload(std::vector<void*>& owned, const void* host, size_t bytes, hipStream_t s) {

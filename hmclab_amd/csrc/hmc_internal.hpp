@@ -148,8 +148,17 @@ int guarded(const char* name, F&& f) {
   }
 }
 
-// allocation helpers
+// allocation helpers (zero-filled; the fill has completed on return)
 void* dev_alloc(std::vector<void*>& owned, size_t bytes);
+
+// Set-up transfers and fills on the legacy null stream.  A device fill or a
+// pageable host-to-device copy there can return before the device has
+// finished it, and the engines' non-blocking streams are not ordered after
+// null-stream work, so each helper waits for the null stream: the data is in
+// place before any later launch on any stream.
+void h2d(void* dst, const void* src, size_t bytes);
+void d2h(void* dst, const void* src, size_t bytes);
+void dzero(void* dst, size_t bytes);
 size_t dtype_size(int dt);
 
 // kernel timing brackets (no-ops unless c.timing)

@@ -526,10 +526,10 @@ static void build_tiles(ProblemSet& s) {
   }
   s.rtiles = (int2*)dev_alloc(s.tile_mem, std::max<size_t>(rt.size(), 1) * sizeof(int2));
   s.gtiles = (int2*)dev_alloc(s.tile_mem, std::max<size_t>(gt.size(), 1) * sizeof(int2));
-  HMC_HIP(hipMemcpy(s.rtiles, rt.data(), rt.size() * sizeof(int2), hipMemcpyHostToDevice));
-  HMC_HIP(hipMemcpy(s.gtiles, gt.data(), gt.size() * sizeof(int2), hipMemcpyHostToDevice));
+  h2d(s.rtiles, rt.data(), rt.size() * sizeof(int2));
+  h2d(s.gtiles, gt.data(), gt.size() * sizeof(int2));
   for (int64_t p = 0; p < s.P; ++p) s.hd[p].L = s.Lcur[p];
-  HMC_HIP(hipMemcpy(s.desc, s.hd.data(), s.P * sizeof(SetDesc), hipMemcpyHostToDevice));
+  h2d(s.desc, s.hd.data(), s.P * sizeof(SetDesc));
   drop_graph(s);
 }
 
@@ -624,9 +624,9 @@ static void finalize(ProblemSet& s) {
   }
   s.n_ptiles = (int64_t)pt.size();
   s.ptiles = (int2*)dev_alloc(s.owned, pt.size() * sizeof(int2));
-  HMC_HIP(hipMemcpy(s.ptiles, pt.data(), pt.size() * sizeof(int2), hipMemcpyHostToDevice));
+  h2d(s.ptiles, pt.data(), pt.size() * sizeof(int2));
   s.partial = (double*)dev_alloc(s.owned, (size_t)s.n_ptiles * SC * 8);
-  HMC_HIP(hipMemcpy(s.desc, s.hd.data(), s.P * sizeof(SetDesc), hipMemcpyHostToDevice));
+  h2d(s.desc, s.hd.data(), s.P * sizeof(SetDesc));
   s.finalized = true;
 }
 
@@ -808,9 +808,9 @@ int hmc_set_problem_dense(hmc_set* h, int64_t i, const float* G, const double* d
       HMC_REQUIRE(v > 0.0 && std::isfinite(v), HMC_ERR_ARG, "data variances must be positive and finite");
       iv[r] = 1.0 / v;
     }
-    HMC_HIP(hipMemcpy(s.G + D.g_off, g.data(), g.size() * 4, hipMemcpyHostToDevice));
-    HMC_HIP(hipMemcpy(s.dobs + D.v_off, d_obs, n * 8, hipMemcpyHostToDevice));
-    HMC_HIP(hipMemcpy(s.ivar + D.v_off, iv.data(), n * 8, hipMemcpyHostToDevice));
+    h2d(s.G + D.g_off, g.data(), g.size() * 4);
+    h2d(s.dobs + D.v_off, d_obs, n * 8);
+    h2d(s.ivar + D.v_off, iv.data(), n * 8);
     s.hp[i].has_data = true;
   });
 }
@@ -825,8 +825,8 @@ int hmc_set_problem_prior(hmc_set* h, int64_t i, int kind, const double* a, cons
     HMC_REQUIRE(kind == HMC_PRIOR_NORMAL || kind == HMC_PRIOR_LAPLACE, HMC_ERR_ARG, "unknown prior kind");
     HMC_REQUIRE(!s.finalized, HMC_ERR_STATE, "the set is already in use; priors cannot change");
     const int64_t d = s.hp[i].d;
-    HMC_HIP(hipMemcpy(s.pa + s.hd[i].a_off, a, d * 8, hipMemcpyHostToDevice));
-    HMC_HIP(hipMemcpy(s.pb + s.hd[i].a_off, b, d * 8, hipMemcpyHostToDevice));
+    h2d(s.pa + s.hd[i].a_off, a, d * 8);
+    h2d(s.pb + s.hd[i].a_off, b, d * 8);
     s.hp[i].prior_kind = kind;
   });
 }
@@ -1009,12 +1009,12 @@ int hmc_set_read_trace(hmc_set* h, int64_t first, int64_t count, double* X, doub
     if (X) {
       std::vector<float> buf((size_t)s.h_total);
       for (int64_t k = 0; k < count; ++k) {
-        HMC_HIP(hipMemcpy(buf.data(), s.tr_x + (first + k) * s.h_total, s.h_total * 4, hipMemcpyDeviceToHost));
+        d2h(buf.data(), s.tr_x + (first + k) * s.h_total, s.h_total * 4);
         for (int64_t j = 0; j < s.h_total; ++j) X[k * s.h_total + j] = (double)buf[j];
       }
     }
     if (chi)
-      HMC_HIP(hipMemcpy(chi, s.tr_chi + first * s.chains(), count * s.chains() * 8, hipMemcpyDeviceToHost));
+      d2h(chi, s.tr_chi + first * s.chains(), count * s.chains() * 8);
   });
 }
 

@@ -496,13 +496,13 @@ int hmc_surrogate_create(int64_t dimensions, int64_t hidden, uint64_t seed, hmc_
       fill(s.oW1, s.H, s.D, s.Dp, s.D, s.H);
       fill(s.oW2, s.H, s.H, s.H, s.H, s.H);
       fill(s.ow3, 1, s.H, s.H, s.H, 1);
-      HMC_HIP(hipMemcpy(s.P, P0.data(), s.nP * 4, hipMemcpyHostToDevice));
+      h2d(s.P, P0.data(), s.nP * 4);
       std::vector<float> ones(s.Dp, 1.f), zeros(s.Dp, 0.f);
       s.xm = (float*)dev_alloc(s.owned, s.Dp * 4);
       s.xs = (float*)dev_alloc(s.owned, s.Dp * 4);
       s.gscale = (float*)dev_alloc(s.owned, s.Dp * 4);
-      HMC_HIP(hipMemcpy(s.xs, ones.data(), s.Dp * 4, hipMemcpyHostToDevice));
-      HMC_HIP(hipMemcpy(s.gscale, ones.data(), s.Dp * 4, hipMemcpyHostToDevice));
+      h2d(s.xs, ones.data(), s.Dp * 4);
+      h2d(s.gscale, ones.data(), s.Dp * 4);
     } catch (...) {
       for (void* p : s.owned) (void)hipFree(p);
       if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -542,9 +542,9 @@ int hmc_surrogate_set_normalization(hmc_surrogate* h, const double* x_mean, cons
       sc[j] = (float)x_scale[j];
       gs[j] = (float)(f_scale / x_scale[j]);
     }
-    HMC_HIP(hipMemcpy(s.xm, m.data(), s.Dp * 4, hipMemcpyHostToDevice));
-    HMC_HIP(hipMemcpy(s.xs, sc.data(), s.Dp * 4, hipMemcpyHostToDevice));
-    HMC_HIP(hipMemcpy(s.gscale, gs.data(), s.Dp * 4, hipMemcpyHostToDevice));
+    h2d(s.xm, m.data(), s.Dp * 4);
+    h2d(s.xs, sc.data(), s.Dp * 4);
+    h2d(s.gscale, gs.data(), s.Dp * 4);
     s.f_mean = f_mean;
     s.f_scale = f_scale;
   });
@@ -562,7 +562,7 @@ int hmc_surrogate_param_count(hmc_surrogate* h, int64_t* n) {
 static void pack_unpack(Surrogate& s, float* host, bool to_device, float* dev) {
   // host layout is unpadded (W1 H x D); device rows are padded to Dp
   std::vector<float> buf(s.nP, 0.f);
-  if (!to_device) HMC_HIP(hipMemcpy(buf.data(), dev, s.nP * 4, hipMemcpyDeviceToHost));
+  if (!to_device) d2h(buf.data(), dev, s.nP * 4);
   int64_t k = 0;
   for (int64_t r = 0; r < s.H; ++r)
     for (int64_t c = 0; c < s.D; ++c, ++k) {
@@ -574,7 +574,7 @@ static void pack_unpack(Surrogate& s, float* host, bool to_device, float* dev) {
     if (to_device) buf[s.ob1 + i] = host[k];
     else host[k] = buf[s.ob1 + i];
   }
-  if (to_device) HMC_HIP(hipMemcpy(dev, buf.data(), s.nP * 4, hipMemcpyHostToDevice));
+  if (to_device) h2d(dev, buf.data(), s.nP * 4);
 }
 
 int hmc_surrogate_get_params(hmc_surrogate* h, float* params) {
@@ -591,8 +591,8 @@ int hmc_surrogate_set_params(hmc_surrogate* h, const float* params) {
     HMC_HIP(hipStreamSynchronize(h->s.stream));
     pack_unpack(h->s, const_cast<float*>(params), true, h->s.P);
     h->s.adam_t = 0;
-    HMC_HIP(hipMemset(h->s.Pm, 0, h->s.nP * 4));
-    HMC_HIP(hipMemset(h->s.Pv, 0, h->s.nP * 4));
+    dzero(h->s.Pm, h->s.nP * 4);
+    dzero(h->s.Pv, h->s.nP * 4);
   });
 }
 
@@ -601,11 +601,11 @@ static int64_t upload_norm(Surrogate& s, const double* X, int64_t B) {
   const int64_t rows = round_up(B, 64);
   ensure_rows(s, rows);
   std::vector<float> hx(rows * s.Dp, 0.f), xm(s.Dp), xs(s.Dp);
-  HMC_HIP(hipMemcpy(xm.data(), s.xm, s.Dp * 4, hipMemcpyDeviceToHost));
-  HMC_HIP(hipMemcpy(xs.data(), s.xs, s.Dp * 4, hipMemcpyDeviceToHost));
+  d2h(xm.data(), s.xm, s.Dp * 4);
+  d2h(xs.data(), s.xs, s.Dp * 4);
   for (int64_t b = 0; b < B; ++b)
     for (int64_t j = 0; j < s.D; ++j) hx[b * s.Dp + j] = ((float)X[b * s.D + j] - xm[j]) / xs[j];
-  HMC_HIP(hipMemcpy(s.Xn, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
+  h2d(s.Xn, hx.data(), hx.size() * 4);
   return rows;
 }
 
@@ -644,9 +644,9 @@ int hmc_surrogate_loss_grad(hmc_surrogate* h, const double* Xn, const double* y,
         ht[b * s.Dp + j] = (float)t[b * s.D + j];
       }
     }
-    HMC_HIP(hipMemcpy(s.Xn, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
-    HMC_HIP(hipMemcpy(s.Tn, ht.data(), ht.size() * 4, hipMemcpyHostToDevice));
-    HMC_HIP(hipMemcpy(s.yb, hy.data(), B * 4, hipMemcpyHostToDevice));
+    h2d(s.Xn, hx.data(), hx.size() * 4);
+    h2d(s.Tn, ht.data(), ht.size() * 4);
+    h2d(s.yb, hy.data(), B * 4);
     forward_grad(s, B);
     backward(s, B, (float)alpha, (float)beta);
     float l = 0.f;
@@ -669,8 +669,8 @@ int hmc_surrogate_train(hmc_surrogate* h, const double* X, const double* chi, co
     ensure_rows(s, batch);
     // normalised training set on the device
     std::vector<float> xm(s.Dp), xs(s.Dp);
-    HMC_HIP(hipMemcpy(xm.data(), s.xm, s.Dp * 4, hipMemcpyDeviceToHost));
-    HMC_HIP(hipMemcpy(xs.data(), s.xs, s.Dp * 4, hipMemcpyDeviceToHost));
+    d2h(xm.data(), s.xm, s.Dp * 4);
+    d2h(xs.data(), s.xs, s.Dp * 4);
     std::vector<void*> tmp;
     float* dX = (float*)dev_alloc(tmp, (size_t)N * s.Dp * 4);
     float* dT = (float*)dev_alloc(tmp, (size_t)N * s.Dp * 4);
@@ -684,9 +684,9 @@ int hmc_surrogate_train(hmc_surrogate* h, const double* X, const double* chi, co
         }
         hy[i] = (float)((chi[i] - s.f_mean) / s.f_scale);
       }
-      HMC_HIP(hipMemcpy(dX, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
-      HMC_HIP(hipMemcpy(dT, ht.data(), ht.size() * 4, hipMemcpyHostToDevice));
-      HMC_HIP(hipMemcpy(dy, hy.data(), N * 4, hipMemcpyHostToDevice));
+      h2d(dX, hx.data(), hx.size() * 4);
+      h2d(dT, ht.data(), ht.size() * 4);
+      h2d(dy, hy.data(), N * 4);
       std::vector<int32_t> perm(N);
       std::iota(perm.begin(), perm.end(), 0);
       std::mt19937_64 g(seed);
