@@ -146,6 +146,22 @@ def _kernel_times(ch):
     return {k: ch.timing(k) for k in range(4)}
 
 
+def _trajectory_gradient_error(post, prob, X, precision):
+    """Relative error (max-norm per row, worst row) of the trajectory gradient
+    in `precision` against the FP64 gradient of the same target."""
+    from hmclab_amd.Distributions import build_problem
+
+    if precision == "f64":
+        return 0.0
+    ref = build_problem(post, precision="f64", engine="gemm")
+    try:
+        g64 = ref.gradient_batch(X)
+    finally:
+        ref.close()
+    g = prob.gradient_batch(X)
+    return float(np.max(np.max(np.abs(g - g64), axis=1) / np.max(np.abs(g64), axis=1)))
+
+
 def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np):
     """Comparator: plain batched FP64 GPU HMC (exact gradients on FP64 MFMA)."""
     from hmclab_amd.Distributions import build_problem
@@ -233,11 +249,15 @@ def run_dense(args):
         opts = {"bf16_split": args.bf16_split, "gemm_config": args.gemm_config}
         if args.bf16_split_grad:
             opts["bf16_split_grad"] = args.bf16_split_grad
+        if args.gemm_config_grad is not None:
+            opts["gemm_config_grad"] = args.gemm_config_grad
+        opts["incremental_residual"] = args.incremental_residual
     prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)
     log(f"device problem ready ({time.perf_counter() - t_setup:.1f}s)")
     eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), args, args.seed)
+    grad_err = _trajectory_gradient_error(post, prob, X0[:8], args.precision)
 
     sur, sur_info = None, None
     if args.surrogate:
@@ -306,6 +326,10 @@ def run_dense(args):
         "min_ess": min_ess, "acceptance_rate": acc_rate,
         "posterior_rms_mean_err_over_sd": err_mean, "posterior_rms_std_rel_err": err_std,
         "proposals_per_s": args.steps / elapsed,
+        "trajectory_gradient_rel_err_vs_fp64": grad_err,
+        "gemm": {"config": args.gemm_config, "config_grad": args.gemm_config_grad,
+                 "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
+                 "incremental_residual": args.incremental_residual},
     }
     if sur_info is not None:
         sur_info.update(chunk_acceptance=sur_acc, fallback_after_proposals=fallback_at,
@@ -541,6 +565,29 @@ def run_uneven(args):
     return result
 
 
+def run_copy(args):
+    """Calibration for HBM byte counters: a plain device copy of a known size
+    (torch's copy kernel, 1 GiB read + 1 GiB written per launch)."""
+    import torch
+
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"metric": "device copy bandwidth (counter calibration)", "value": 2 * nbytes / dt / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 2, "ms_per_step": dt * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "copy 1 GiB -> 1 GiB", "bytes_read_per_launch": nbytes,
+                                            "bytes_written_per_launch": nbytes}}
+
+
 def run_small(args):
     """The 1-chain dense case (G 100x50): latency per proposal on the fused
     engine at the stated step 0.05 and ESS/s at a tuned step."""
@@ -592,7 +639,7 @@ def run_small(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small", "uneven"])
+    ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small", "uneven", "copy"])
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
@@ -607,9 +654,14 @@ def main():
     ap.add_argument("--cpu-eps", type=float, default=1e-3)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "f64", "f32"])
     ap.add_argument("--bf16-split", type=int, default=2, choices=[1, 2])
-    ap.add_argument("--bf16-split-grad", type=int, default=None, choices=[1, 2],
-                    help="bf16 planes of the residual in the gradient GEMM (default: --bf16-split)")
-    ap.add_argument("--gemm-config", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--bf16-split-grad", type=int, default=1, choices=[1, 2],
+                    help="bf16 planes of the scaled residual in the gradient GEMM (1: one plane, "
+                         "measured to leave acceptance unchanged; 2: hi/lo)")
+    ap.add_argument("--gemm-config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--incremental-residual", type=int, default=0, choices=[0, 1],
+                    help="bf16: advance a float32 residual by G dx after the first stage")
+    ap.add_argument("--gemm-config-grad", type=int, default=None, choices=[0, 1, 2],
+                    help="tile configuration of the gradient GEMM (default: --gemm-config)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-proposals", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
@@ -643,6 +695,8 @@ def main():
         res = run_small(args)
     elif args.workload == "uneven":
         res = run_uneven(args)
+    elif args.workload == "copy":
+        res = run_copy(args)
     else:
         res = run_dense(args)
     print(json.dumps(res), flush=True)

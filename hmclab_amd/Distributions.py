@@ -58,25 +58,30 @@ class _AbstractDistribution:
     def _invalidate(self):
         self.__dict__.pop("_gpu_cache", None)
 
+    # Tempering: the distribution is exp(-chi(m) / temperature), as the
+    # reference's tempered targets divide misfit and gradient by their
+    # temperature (e.g. StandardNormal1D, base.py:392-414).  1 = untempered.
+    temperature: float = 1.0
+
     def misfit(self, coordinates: np.ndarray) -> float:
-        """chi(m) for a (dimensions, 1) column vector, on the GPU."""
+        """chi(m) / temperature for a (dimensions, 1) column vector, on the GPU."""
         m = np.asarray(coordinates, dtype=np.float64)
-        return float(self._gpu().misfit_batch(m.reshape(1, -1))[0])
+        return float(self._gpu().misfit_batch(m.reshape(1, -1))[0]) / self.temperature
 
     def gradient(self, coordinates: np.ndarray) -> np.ndarray:
-        """d chi / dm as a (dimensions, 1) column vector, on the GPU."""
+        """d chi / dm / temperature as a (dimensions, 1) column vector, on the GPU."""
         m = np.asarray(coordinates, dtype=np.float64)
-        return self._gpu().gradient_batch(m.reshape(1, -1)).reshape(-1, 1)
+        return self._gpu().gradient_batch(m.reshape(1, -1)).reshape(-1, 1) / self.temperature
 
     def misfit_batch(self, M: np.ndarray, precision: str = "f64") -> np.ndarray:
         """Misfits of the columns of M (dimensions, B) -> (B,)."""
         M = np.asarray(M, dtype=np.float64).reshape(self.dimensions, -1)
-        return self._gpu(precision).misfit_batch(M.T)
+        return self._gpu(precision).misfit_batch(M.T) / self.temperature
 
     def gradient_batch(self, M: np.ndarray, precision: str = "f64") -> np.ndarray:
         """Gradients of the columns of M (dimensions, B) -> (dimensions, B)."""
         M = np.asarray(M, dtype=np.float64).reshape(self.dimensions, -1)
-        return self._gpu(precision).gradient_batch(M.T).T
+        return self._gpu(precision).gradient_batch(M.T).T / self.temperature
 
     # ---- bounds (host bookkeeping, same rules and messages as base.py) ------
     def corrector(self, coordinates: np.ndarray, momentum: np.ndarray):

@@ -74,7 +74,8 @@ class HMC:
                learning_rate: float = 0.75, queue=None, disable_progressbar=False, *,
                chains: int = 1, precision: str = "f64", engine: str = "auto", rng: str = None,
                seeds=None, chunk: int = None, surrogate=None, surrogate_fallback: float = 0.5,
-               surrogate_window: int = 20):
+               surrogate_window: int = 20, temperatures=None, exchange_schedule=None,
+               exchange_interval: int = 1, rngs=None):
         """Sample ``distribution``; see the module docstring for the extras.
 
         ``samples_filename``: a ``.npy`` path (one chain), a list of paths (one
@@ -82,12 +83,15 @@ class HMC:
         chains, or None to keep the samples in memory (``self.results``,
         shape (chains, d+1, N)).
         """
+        self._temperatures = temperatures
+        self._given_rngs = rngs
         self._init(samples_filename, distribution, stepsize, randomize_stepsize, amount_of_steps,
                    mass_matrix, integrator, initial_model, proposals, online_thinning,
                    diagnostic_mode, overwrite_existing_file, max_time, autotuning,
                    target_acceptance_rate, learning_rate, disable_progressbar, chains, precision,
                    engine, rng, seeds, chunk)
         try:
+            self._setup_exchange(exchange_schedule, exchange_interval)
             self._attach_surrogate(surrogate, surrogate_fallback, surrogate_window)
             self._loop()
         finally:
@@ -205,6 +209,11 @@ class HMC:
         self.problem = build_problem(distribution, precision=precision, engine=engine,
                                      mass_diagonal=mass_matrix._device_diagonal())
         self._chains = Chains(self.problem, B)
+        T = self._temperatures if self._temperatures is not None else distribution.temperature
+        T = np.broadcast_to(np.asarray(T, dtype=np.float64), (B,))
+        if np.any(T != 1.0):
+            self._chains.set_temperatures(T)
+        self.temperatures = T.copy()
         try:
             self._chains.set_state(X0)
         except ValueError as e:
@@ -219,7 +228,10 @@ class HMC:
 
         # per-chain randoms for host mode
         if self.rng_mode == "host":
-            if B == 1 and seeds is None:
+            if self._given_rngs is not None:
+                assert len(self._given_rngs) == B, "one random generator per chain is required"
+                self._rngs = list(self._given_rngs)
+            elif B == 1 and seeds is None:
                 self._rngs = [self.rng]
             else:
                 if seeds is None:
@@ -249,6 +261,31 @@ class HMC:
                     w.write_attribute(k, v)
 
     # ------------------------------------------------------------------ #
+    def _setup_exchange(self, schedule, interval):
+        """Replica exchange between chains (ParallelSampleSMP, Samplers.py:589-669):
+        after proposal i with i % interval == 0, row i // interval of the schedule
+        pairs its entries (2k, 2k+1); the second of a pair draws the uniform."""
+        self._ex_schedule = None
+        if schedule is None:
+            return
+        sched = np.asarray(schedule, dtype=np.int32)
+        assert sched.ndim == 2 and sched.shape[1] % 2 == 0, "exchange schedule must be (events, 2 * pairs)"
+        assert int(interval) > 0, "exchange_interval must be positive"
+        self._ex_schedule = sched
+        self._ex_interval = int(interval)
+        if self.rng_mode == "device":
+            self._chains.set_exchange(self._ex_interval, sched)
+
+    def _host_exchange(self, i):
+        if self._ex_schedule is None or i % self._ex_interval != 0:
+            return
+        e = i // self._ex_interval
+        if e >= self._ex_schedule.shape[0]:
+            return
+        pairs = self._ex_schedule[e].reshape(-1, 2)
+        u = np.array([self._rngs[int(b)].uniform(0, 1) for _, b in pairs])
+        self._chains.exchange(pairs, u, i)
+
     def _attach_surrogate(self, surrogate, fallback, window):
         self.surrogate = surrogate
         self.surrogate_fallback = float(fallback)
@@ -299,6 +336,7 @@ class HMC:
                 for i in range(self.proposals):
                     self.current_proposal = i
                     acc_i = self._host_proposal(i)
+                    self._host_exchange(i)
                     if self.surrogate is not None and self.surrogate_fallback_proposal is None:
                         self._sur_recent.append(float(np.mean(acc_i)))
                         if len(self._sur_recent) >= self._sur_window:
@@ -391,6 +429,7 @@ class HMC:
     def _close(self):
         self.end_time = datetime.now()
         B, d = self.chains, self.dimensions
+        self._chains_exchange_counts = self._chains.exchange_counts()
         X, chi = self._chains.get_state()
         self.current_model = X[0].reshape(d, 1) if B == 1 else X.T.copy()
         self.current_x = float(chi[0]) if B == 1 else chi
@@ -441,3 +480,120 @@ class HMC:
             return self.results[0][:, burn_in:]
         with Samples(self._filenames[0], burn_in=burn_in) as s:
             return np.array(s.numpy)
+
+
+class ParallelSampleSMP:
+    """Several chains with (possibly tempered) posteriors and replica exchange,
+    mirroring ``hmclab.Samplers.ParallelSampleSMP`` (Samplers.py:1804-1977).
+
+    The reference runs one process per chain and exchanges models through
+    pipes; here all chains run as one batch on the GPU.  The posteriors must
+    share one target up to their ``temperature`` (chain c samples
+    exp(-chi / T_c)), which is how tempered ladders are built; the exchange
+    schedule is drawn exactly as the reference draws it
+    (``rng.choice(N, 2 * (N // 2), replace=False)`` per exchange event), and
+    the second chain of each pair decides with a uniform from its own sampler's
+    generator, so a seeded run reproduces the reference's decisions.
+    Deviation: after an accepted swap the misfit moves with the model (the
+    reference keeps the old ``current_x``, Samplers.py:640-668, so its samples
+    file records a stale misfit and later exchange decisions use it).
+    """
+
+    def __init__(self, seed=None):
+        self.rng = np.random.default_rng() if seed is None else np.random.default_rng(seed)
+
+    def sample(self, samplers, filenames, posteriors, overwrite_existing_files=False, proposals: int = 100,
+               exchange: bool = True, exchange_interval: int = 1, initial_model=None, kwargs=None):
+        assert overwrite_existing_files, (
+            "You have to manually enable overwriting samples. This is for safety. The "
+            "existing file dialog doesn't work in the parallel case. "
+            "Set `overwrite_existing_files=True`."
+        )
+        number_of_chains = len(samplers)
+        assert len(filenames) == number_of_chains, (
+            f"The number of supplied initial models ({len(filenames)}) "
+            f"is not equal to the amount of chains ({number_of_chains}). "
+            f"Supply {number_of_chains} models."
+        )
+        assert len(posteriors) == number_of_chains, (
+            f"The number of supplied initial models ({len(posteriors)}) "
+            f"is not equal to the amount of chains ({number_of_chains}). "
+            f"Supply {number_of_chains} posteriors."
+        )
+        if type(initial_model) == list:
+            assert len(initial_model) == number_of_chains, (
+                f"The number of supplied initial models ({len(initial_model)}) "
+                f"is not equal to the amount of chains ({number_of_chains}). "
+                f"Supply either 1 or {number_of_chains} models."
+            )
+        if type(kwargs) == list:
+            assert len(kwargs) == number_of_chains, (
+                f"The number of supplied kwargs dictionaries ({len(kwargs)}) "
+                f"is not equal to the amount of chains ({number_of_chains}). "
+                f"Supply either 1 or {number_of_chains} kwargs. Note that even if you "
+                f"don't want to pass arguments to one of the chains, and empty "
+                f"dictionary still has to be included."
+            )
+        # Samplers.py:1880-1893: one row of distinct chain indices per exchange event
+        schedule = None
+        if exchange:
+            exchanges_per_proposal = int(np.floor(number_of_chains / 2))
+            schedule = np.vstack([
+                self.rng.choice(number_of_chains, exchanges_per_proposal * 2, replace=False)
+                for _ in range(int(proposals / exchange_interval))
+            ])
+        self.exchange_schedule = schedule
+        _check_same_target(posteriors)
+        temperatures = [float(getattr(p, "temperature", 1.0)) for p in posteriors]
+        d = posteriors[0].dimensions
+        if type(initial_model) == list:
+            m0 = np.hstack([np.asarray(m, dtype=np.float64).reshape(d, 1) for m in initial_model])
+        elif initial_model is None:
+            m0 = None
+        else:
+            m0 = np.repeat(np.asarray(initial_model, dtype=np.float64).reshape(d, 1), number_of_chains, axis=1)
+        kw = dict(kwargs[0] if type(kwargs) == list else (kwargs or {}))
+        if type(kwargs) == list:
+            for other in kwargs[1:]:
+                assert other == kwargs[0], "all chains of one batch must share their sampler settings"
+        for key in ("proposals", "overwrite_existing_file", "queue"):
+            kw.pop(key, None)
+        self.sampler = HMC(seed=getattr(samplers[0], "seed", None))
+        names = None if all(f is None for f in filenames) else list(filenames)   # None: keep in memory
+        self.sampler.sample(names, posteriors[0], proposals=proposals, initial_model=m0,
+                            overwrite_existing_file=True, chains=number_of_chains, rng="host",
+                            rngs=[s.rng for s in samplers], temperatures=temperatures,
+                            exchange_schedule=schedule, exchange_interval=exchange_interval, **kw)
+        self.exchange_counts = self.sampler._chains_exchange_counts
+        return self
+
+
+def _check_same_target(posteriors):
+    """All posteriors of one batch must compile to the same device problem."""
+    from .Distributions import _Spec
+
+    def spec_of(p):
+        s = _Spec()
+        p._contribute(s)
+        return s
+
+    first = spec_of(posteriors[0])
+    for p in posteriors[1:]:
+        assert p.dimensions == posteriors[0].dimensions, "posteriors differ in dimension"
+        s = spec_of(p)
+        same = len(s.priors) == len(first.priors) and all(
+            a[0] == b[0] and np.array_equal(np.asarray(a[1]), np.asarray(b[1]))
+            and np.array_equal(np.asarray(a[2]), np.asarray(b[2])) for a, b in zip(s.priors, first.priors))
+        if (s.likelihood is None) != (first.likelihood is None):
+            same = False
+        elif s.likelihood is not None and s.likelihood is not first.likelihood:
+            ga, da, va = s.likelihood._whitened()
+            gb, db, vb = first.likelihood._whitened()
+            same = same and (ga.shape == gb.shape) and np.array_equal(np.asarray(da), np.asarray(db)) and (
+                np.array_equal(ga.toarray() if hasattr(ga, "toarray") else ga,
+                               gb.toarray() if hasattr(gb, "toarray") else gb))
+            same = same and np.array_equal(np.asarray(va), np.asarray(vb))
+        if not same:
+            raise NotImplementedError(
+                "the GPU batch needs posteriors that differ only by their temperature")
+

@@ -8,6 +8,6 @@ hand-written gfx950 HIP kernels (``libhmc.so``, C-ABI in
 """
 from . import Distributions, MassMatrices, Samplers, Samples, diagnostics, tomography  # noqa: F401
 from ._lib import device_count  # noqa: F401
-from .Samplers import HMC  # noqa: F401
+from .Samplers import HMC, ParallelSampleSMP  # noqa: F401
 
 __version__ = "0.1.0"

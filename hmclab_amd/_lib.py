@@ -93,6 +93,10 @@ SIGNATURES = {
     "hmc_surrogate_train": [_p, _dp, _dp, _dp, _i64, _i64, _i64, C.c_double, C.c_double,
                             C.c_double, C.c_uint64, _dp],
     "hmc_chains_set_surrogate": [_p, _p],
+    "hmc_chains_set_temperatures": [_p, _dp],
+    "hmc_chains_exchange": [_p, _ip, _i64, _dp, _i64, _ip],
+    "hmc_chains_set_exchange": [_p, _i64, _ip, _i64, _i64],
+    "hmc_chains_exchange_counts": [_p, _lp],
     "hmc_set_create": [_i64, _lp, _lp, C.POINTER(_p)],
     "hmc_set_destroy": [_p],
     "hmc_set_problem_dense": [_p, _i64, C.POINTER(C.c_float), _dp, _dp, _i32],

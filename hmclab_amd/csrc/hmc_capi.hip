@@ -209,9 +209,10 @@ static void gemm_proposal(Chains& c, bool host_p, int L, int integ, uint64_t see
   std::vector<Stage> st;
   integrator_schedule(integ, L, first, st);
   elem_refresh(c, host_p, first, seed, prop, !host_p, randomize);
-  for (const Stage& s : st) {
+  for (size_t k = 0; k < st.size(); ++k) {
+    const Stage& s = st[k];
     if (c.surrogate) surrogate_kick_drift(c, s.kick, s.drift, s.drift2);
-    else gemm_kick_drift(c, s.kick, s.drift, s.drift2);
+    else gemm_kick_drift(c, s.kick, s.drift, s.drift2, k == 0);
   }
   gemm_endpoint(c, c.xt, c.p->state_dtype);
   elem_metropolis(c, true, autotune, target, lr, prop, slot);
@@ -295,8 +296,8 @@ int hmc_problem_set_dense(hmc_problem* hp, const void* G, int g_dtype, int64_t n
     HMC_REQUIRE(n_data > 0, HMC_ERR_ARG, "n_data must be positive");
     HMC_REQUIRE(g_dtype == HMC_F64 || g_dtype == HMC_F32 || g_dtype == HMC_BF16, HMC_ERR_ARG, "bad G dtype");
     const int64_t n = n_data, d = P.d;
-    P.n_pad = round_up(n, 128);
-    P.d_pad = round_up(d, 128);
+    P.n_pad = round_up(n, gemm_row_pad());
+    P.d_pad = round_up(d, gemm_row_pad());
     P.ldg = round_up(d, 64);
     P.ldgt = round_up(n, 64);
     const int traj = P.precision;
@@ -455,9 +456,17 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
     } else if (key == "bf16_split_grad") {
       HMC_REQUIRE(value == 1.0 || value == 2.0, HMC_ERR_ARG, "bf16_split_grad must be 1 or 2");
       P.bf16_split_grad = (int)value;
+    } else if (key == "incremental_residual") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "incremental_residual must be 0 or 1");
+      P.resid_update = (int)value;
     } else if (key == "gemm_config") {
-      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_config must be 0 or 1");
+      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0, HMC_ERR_ARG, "gemm_config must be 0, 1 or 2");
       P.gemm_cfg = (int)value;
+      P.gemm_cfg_grad = (int)value;
+    } else if (key == "gemm_config_grad") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0, HMC_ERR_ARG,
+                  "gemm_config_grad must be 0, 1 or 2");
+      P.gemm_cfg_grad = (int)value;
     } else {
       throw Error{HMC_ERR_ARG, "unknown option " + key};
     }
@@ -504,6 +513,9 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
       c.acc = (int32_t*)dev_alloc(c.owned, c.B_pad * 4);
       c.oob = (int32_t*)dev_alloc(c.owned, c.B_pad * 4);
       c.acc_count = (int64_t*)dev_alloc(c.owned, c.B_pad * 8);
+      c.ex_count = (int64_t*)dev_alloc(c.owned, c.B_pad * 8);
+      c.ex_tmp = (int32_t*)dev_alloc(c.owned, (size_t)c.B_pad * 2 * 4);
+      c.ex_u = (double*)dev_alloc(c.owned, c.B_pad * 8);
       if (P.op == 1) {
         c.n_mblocks = P.n_pad / 128;
         c.partial = (double*)dev_alloc(c.owned, (size_t)c.n_mblocks * c.B_pad * 8);
@@ -511,6 +523,7 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
           c.xh = (uint16_t*)dev_alloc(c.owned, rows * 2);
           c.xl = (uint16_t*)dev_alloc(c.owned, rows * 2);
           c.rh = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
+          if (P.resid_update) c.rf = (float*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 4);
           c.rl = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
         } else if (P.resolved_engine() == HMC_ENGINE_GEMM) {
           c.r = dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * es);
@@ -566,7 +579,99 @@ int hmc_chains_get_state(hmc_chains* hc, double* X, double* chi) {
     if (chi) {
       HMC_HIP(hipMemcpyAsync(chi, c.chi0, c.B * 8, hipMemcpyDeviceToHost, P.stream));
       HMC_HIP(hipStreamSynchronize(P.stream));
+      if (c.beta) {
+        std::vector<double> bt(c.B);
+        HMC_HIP(hipMemcpy(bt.data(), c.beta, c.B * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < c.B; ++i) chi[i] *= bt[i];
+      }
     }
+  });
+}
+
+int hmc_chains_set_temperatures(hmc_chains* hc, const double* temperatures) {
+  return guarded("hmc_chains_set_temperatures", [&] {
+    HMC_REQUIRE(hc, HMC_ERR_ARG, "NULL chains");
+    Chains& c = hc->c;
+    HMC_HIP(hipStreamSynchronize(c.p->stream));
+    if (!temperatures) {
+      if (c.beta) {
+        (void)hipFree(c.beta);
+        c.owned.erase(std::remove(c.owned.begin(), c.owned.end(), (void*)c.beta), c.owned.end());
+        c.beta = nullptr;
+      }
+      return;
+    }
+    std::vector<double> bt(c.B_pad, 1.0);
+    for (int64_t i = 0; i < c.B; ++i) {
+      HMC_REQUIRE(temperatures[i] > 0.0 && std::isfinite(temperatures[i]), HMC_ERR_ARG,
+                  "temperatures must be positive and finite");
+      bt[i] = 1.0 / temperatures[i];
+    }
+    if (!c.beta) c.beta = (double*)dev_alloc(c.owned, c.B_pad * 8);
+    h2d(c.beta, bt.data(), c.B_pad * 8);
+  });
+}
+
+int hmc_chains_exchange(hmc_chains* hc, const int32_t* pairs, int64_t npairs, const double* u,
+                        int64_t proposal, int32_t* swapped) {
+  return guarded("hmc_chains_exchange", [&] {
+    HMC_REQUIRE(hc && pairs && u && npairs >= 0, HMC_ERR_ARG, "invalid argument");
+    Chains& c = hc->c;
+    HMC_REQUIRE(2 * npairs <= c.B, HMC_ERR_ARG, "more pairs than chains");
+    std::vector<char> used(c.B, 0);
+    for (int64_t k = 0; k < 2 * npairs; ++k) {
+      HMC_REQUIRE(pairs[k] >= 0 && pairs[k] < c.B, HMC_ERR_ARG, "pair index out of range");
+      HMC_REQUIRE(!used[pairs[k]], HMC_ERR_ARG, "a chain appears twice in one exchange event");
+      used[pairs[k]] = 1;
+    }
+    if (npairs == 0) return;
+    Problem& P = *c.p;
+    HMC_HIP(hipMemcpyAsync(c.ex_tmp, pairs, npairs * 2 * 4, hipMemcpyHostToDevice, P.stream));
+    HMC_HIP(hipMemcpyAsync(c.ex_u, u, npairs * 8, hipMemcpyHostToDevice, P.stream));
+    int32_t* flags = c.ex_tmp + c.B_pad;
+    elem_exchange(c, c.ex_tmp, npairs, c.ex_u, 0, proposal, flags, -1);
+    if (swapped) HMC_HIP(hipMemcpyAsync(swapped, flags, npairs * 4, hipMemcpyDeviceToHost, P.stream));
+    HMC_HIP(hipStreamSynchronize(P.stream));
+  });
+}
+
+int hmc_chains_set_exchange(hmc_chains* hc, int64_t interval, const int32_t* schedule, int64_t events,
+                            int64_t npairs) {
+  return guarded("hmc_chains_set_exchange", [&] {
+    HMC_REQUIRE(hc, HMC_ERR_ARG, "NULL chains");
+    Chains& c = hc->c;
+    HMC_HIP(hipStreamSynchronize(c.p->stream));
+    if (c.ex_pairs) {
+      (void)hipFree(c.ex_pairs);
+      c.owned.erase(std::remove(c.owned.begin(), c.owned.end(), (void*)c.ex_pairs), c.owned.end());
+      c.ex_pairs = nullptr;
+    }
+    c.ex_events = c.ex_npairs = c.ex_interval = 0;
+    if (!schedule || events <= 0 || npairs <= 0) return;
+    HMC_REQUIRE(interval > 0 && 2 * npairs <= c.B, HMC_ERR_ARG, "invalid exchange schedule");
+    for (int64_t e = 0; e < events; ++e) {
+      std::vector<char> used(c.B, 0);
+      for (int64_t k = 0; k < 2 * npairs; ++k) {
+        const int32_t v = schedule[e * 2 * npairs + k];
+        HMC_REQUIRE(v >= 0 && v < c.B && !used[v], HMC_ERR_ARG,
+                    "exchange schedule rows must hold distinct chain indices");
+        used[v] = 1;
+      }
+    }
+    c.ex_pairs = (int32_t*)dev_alloc(c.owned, (size_t)events * 2 * npairs * 4);
+    h2d(c.ex_pairs, schedule, (size_t)events * 2 * npairs * 4);
+    c.ex_events = events;
+    c.ex_npairs = npairs;
+    c.ex_interval = interval;
+  });
+}
+
+int hmc_chains_exchange_counts(hmc_chains* hc, int64_t* counts) {
+  return guarded("hmc_chains_exchange_counts", [&] {
+    HMC_REQUIRE(hc && counts, HMC_ERR_ARG, "NULL argument");
+    Chains& c = hc->c;
+    HMC_HIP(hipMemcpyAsync(counts, c.ex_count, c.B * 8, hipMemcpyDeviceToHost, c.p->stream));
+    HMC_HIP(hipStreamSynchronize(c.p->stream));
   });
 }
 
@@ -666,6 +771,10 @@ int hmc_chains_run(hmc_chains* hc, int64_t proposals, int L, int integrator, dou
         gemm_proposal(c, false, L, integrator, seed, prop, randomize, autotune, target, learning_rate, slot);
       else
         fused_propose(c, false, L, integrator, seed, prop, randomize, autotune, target, learning_rate, slot);
+      // replica exchange after the proposal, as in the reference's loop (Samplers.py:589-595)
+      if (c.ex_pairs && prop % c.ex_interval == 0 && prop / c.ex_interval < c.ex_events)
+        elem_exchange(c, c.ex_pairs + (prop / c.ex_interval) * 2 * c.ex_npairs, c.ex_npairs, nullptr, seed, prop,
+                      nullptr, slot);
       if (c.timing && c.ev_pending.size() > 2048) time_collect(c);
     }
     HMC_HIP(hipMemcpyAsync(stepsize, c.step, c.B * 8, hipMemcpyDeviceToHost, P.stream));

@@ -111,7 +111,7 @@ __device__ __host__ inline double u53(uint32_t a, uint32_t b) {
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
-enum : uint32_t { STREAM_MOMENTUM = 0, STREAM_SCALARS = 1 };
+enum : uint32_t { STREAM_MOMENTUM = 0, STREAM_SCALARS = 1, STREAM_EXCHANGE = 2 };
 
 // Two standard normals (Box-Muller) for element pair `pair` of `chain` at
 // proposal `prop`.
@@ -135,6 +135,13 @@ __device__ inline void philox_scalars(uint64_t seed, uint32_t chain, uint64_t pr
   U32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   u_eps = 0.5 + u53(r.v[0], r.v[1]);
   u_acc = u53(r.v[2], r.v[3]);
+}
+
+// Accept uniform in [0, 1) of a replica exchange decided by `chain` after proposal `prop`.
+__device__ inline double philox_exchange(uint64_t seed, uint32_t chain, uint64_t prop) {
+  U32x4 c = {{0u, chain, (uint32_t)prop, (STREAM_EXCHANGE << 24) | ((uint32_t)(prop >> 32) & 0xffffffu)}};
+  U32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return u53(r.v[0], r.v[1]);
 }
 
 // ----------------------------------------------------------------------------

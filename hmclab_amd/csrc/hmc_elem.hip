@@ -116,7 +116,7 @@ __global__ __launch_bounds__(EW_NT) void k_metropolis(
     int64_t ldx, int64_t d, const double* partial, int64_t nmb, int64_t ldp, double mconst,
     const double* k0, const double* u, double* h0o, double* h1o, int32_t* acc, int64_t* acc_count,
     double* step, int autotune, double target, double lr, int64_t prop_index, T* tr_x,
-    double* tr_chi, int64_t slot, int64_t B, DevModel md) {
+    double* tr_chi, int64_t slot, int64_t B, const double* __restrict__ beta, DevModel md) {
   __shared__ double scratch[EW_NT / 64 + 1];
   __shared__ int accepted;
   __shared__ double chi_new;
@@ -124,9 +124,10 @@ __global__ __launch_bounds__(EW_NT) void k_metropolis(
   double k1;
   const double chi1 = chain_misfit<T>(xt + c * ldx, d, partial, nmb, ldp, c, mconst, md, scratch,
                                       &k1, pt + c * ldx);
+  const double bt = beta ? beta[c] : 1.0;   // tempered target exp(-beta chi)
   if (threadIdx.x == 0) {
-    const double H0 = chi0[c] + k0[c];
-    const double H1 = chi1 + k1;
+    const double H0 = bt * chi0[c] + k0[c];
+    const double H1 = bt * chi1 + k1;
     const double a = exp(H0 - H1);
     if (autotune) {
       // Samplers.py:1502-1522
@@ -152,7 +153,57 @@ __global__ __launch_bounds__(EW_NT) void k_metropolis(
   if (slot >= 0) {
     const T* src = accepted ? xt : x0;
     for (int64_t j = threadIdx.x; j < d; j += EW_NT) tr_x[(slot * B + c) * d + j] = src[c * ldx + j];
-    if (threadIdx.x == 0) tr_chi[slot * B + c] = chi_new;
+    if (threadIdx.x == 0) tr_chi[slot * B + c] = bt * chi_new;
+  }
+}
+
+// Replica exchange (ParallelSampleSMP, Samplers.py:589-669): pair k = (a, b),
+// b decides; the swap is accepted when
+//   exp(beta_b (chi_b - chi(x_a)) + beta_a (chi_a - chi(x_b))) > u,
+// i.e. exp((beta_a - beta_b)(chi_a - chi_b)) > u for one shared target; the
+// positions and misfits of a and b are then exchanged.  One workgroup per pair.
+template <typename T>
+__global__ __launch_bounds__(EW_NT) void k_exchange(T* __restrict__ x0, double* __restrict__ chi0,
+                                                   int64_t ldx, int64_t d, const int32_t* __restrict__ pairs,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ beta, int64_t* ex_count,
+                                                   int32_t* swapped, uint64_t seed, int64_t prop, T* tr_x,
+                                                   double* tr_chi, int64_t slot, int64_t B) {
+  __shared__ int swap;
+  const int64_t k = blockIdx.x;
+  const int a = pairs[2 * k], b = pairs[2 * k + 1];
+  if (threadIdx.x == 0) {
+    const double ba = beta ? beta[a] : 1.0, bb = beta ? beta[b] : 1.0;
+    const double ua = u ? u[k] : philox_exchange(seed, (uint32_t)b, (uint64_t)prop);
+    const double improvement = bb * (chi0[b] - chi0[a]) + ba * (chi0[a] - chi0[b]);
+    swap = exp(improvement) > ua ? 1 : 0;
+    if (swapped) swapped[k] = swap;
+  }
+  __syncthreads();
+  if (!swap) return;
+  for (int64_t j = threadIdx.x; j < d; j += EW_NT) {
+    const T t = x0[(int64_t)a * ldx + j];
+    x0[(int64_t)a * ldx + j] = x0[(int64_t)b * ldx + j];
+    x0[(int64_t)b * ldx + j] = t;
+  }
+  if (slot >= 0) {   // the sample recorded for this proposal is the post-exchange state
+    for (int64_t j = threadIdx.x; j < d; j += EW_NT) {
+      tr_x[(slot * B + a) * d + j] = x0[(int64_t)a * ldx + j];
+      tr_x[(slot * B + b) * d + j] = x0[(int64_t)b * ldx + j];
+    }
+  }
+  if (threadIdx.x == 0) {
+    const double t = chi0[a];
+    chi0[a] = chi0[b];
+    chi0[b] = t;
+    if (ex_count) {
+      ex_count[a] += 1;
+      ex_count[b] += 1;
+    }
+    if (slot >= 0) {
+      tr_chi[slot * B + a] = (beta ? beta[a] : 1.0) * chi0[a];
+      tr_chi[slot * B + b] = (beta ? beta[b] : 1.0) * chi0[b];
+    }
   }
 }
 
@@ -246,15 +297,30 @@ void elem_metropolis(Chains& c, bool have_partials, int autotune, double target,
                        c.chi0, (const double*)c.xt, (const double*)c.pt, c.ldx, p.d, c.partial, nmb,
                        c.B_pad, p.misfit_const, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
                        autotune, target, learning_rate, prop_index, (double*)c.tr_x, c.tr_chi,
-                       trace_slot, c.B, p.model());
+                       trace_slot, c.B, (const double*)c.beta, p.model());
   else
     hipLaunchKernelGGL(k_metropolis<float>, dim3(c.B), dim3(EW_NT), 0, p.stream, (float*)c.x0,
                        c.chi0, (const float*)c.xt, (const float*)c.pt, c.ldx, p.d, c.partial, nmb,
                        c.B_pad, p.misfit_const, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
                        autotune, target, learning_rate, prop_index, (float*)c.tr_x, c.tr_chi,
-                       trace_slot, c.B, p.model());
+                       trace_slot, c.B, (const double*)c.beta, p.model());
   HMC_HIP(hipGetLastError());
   time_end(c, 2, ev);
+}
+
+void elem_exchange(Chains& c, const int32_t* pairs, int64_t npairs, const double* u, uint64_t seed,
+                   int64_t prop, int32_t* swapped, int64_t slot) {
+  if (npairs <= 0) return;
+  Problem& p = *c.p;
+  if (p.state_dtype == HMC_F64)
+    hipLaunchKernelGGL(k_exchange<double>, dim3((unsigned)npairs), dim3(EW_NT), 0, p.stream, (double*)c.x0,
+                       c.chi0, c.ldx, p.d, pairs, u, (const double*)c.beta, c.ex_count, swapped, seed, prop,
+                       (double*)c.tr_x, c.tr_chi, slot, c.B);
+  else
+    hipLaunchKernelGGL(k_exchange<float>, dim3((unsigned)npairs), dim3(EW_NT), 0, p.stream, (float*)c.x0,
+                       c.chi0, c.ldx, p.d, pairs, u, (const double*)c.beta, c.ex_count, swapped, seed, prop,
+                       (float*)c.tr_x, c.tr_chi, slot, c.B);
+  HMC_HIP(hipGetLastError());
 }
 
 void elem_finish_misfit(Chains& c, const void* X, bool have_partials, double* out) {

@@ -142,6 +142,7 @@ struct FusedArgs {
   uint64_t seed;
   int64_t prop;
   int device_scalars, randomize, autotune;
+  const double* beta;   // per-chain inverse temperature (null: 1)
   double target, lr;
   // schedule (units of the local step): L steps of P stages; every step but
   // the last uses pat, the last uses pat_last; each stage = (kick, drift, drift2)
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   T* Xs = reinterpret_cast<T*>(smem_raw);                         // [d][CT]
   T* Rs = Xs + round_up(op.d, 2) * CT;                            // [n][CT]
   __shared__ double scratch[FU_NW + 1];
-  __shared__ double s_k0[CT], s_eps[CT], s_chi1[CT], s_k1[CT];
+  __shared__ double s_k0[CT], s_eps[CT], s_chi1[CT], s_k1[CT], s_beta[CT];
   __shared__ int s_ok[CT];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -182,6 +183,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
       }
     }
     s_eps[tid] = e;
+    s_beta[tid] = (a.beta && c < a.B) ? a.beta[c] : 1.0;
   }
   // ---- load positions of the tile
   for (int64_t j = tid; j < op.d; j += FU_NT)
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
             T xv = Xs[j * CT + c], pv = preg[ks][c];
             T gg = g[c] + prior_grad<T>(md, j, xv);
             if (out_of_bounds(md, j, (double)xv)) gg = (T)INFINITY;
-            pv -= (T)(kick * e) * gg;
+            pv -= (T)(a.beta ? kick * e * s_beta[c] : kick * e) * gg;
             xv += (T)(drift * e) * kgrad<T>(md, j, pv);
             corrector<T>(md, j, xv, pv);
             if (drift2 != 0.0) {
@@ -345,8 +347,8 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     const int64_t ch = c0 + c;
     int ok = 0;
     if (ch < a.B) {
-      const double H0 = a.chi0[ch] + s_k0[c];
-      const double H1 = s_chi1[c] + s_k1[c];
+      const double H0 = s_beta[c] * a.chi0[ch] + s_k0[c];
+      const double H1 = s_beta[c] * s_chi1[c] + s_k1[c];
       const double av = exp(H0 - H1);
       if (a.autotune) {
         const double wgt = pow((double)(a.prop + 1), -a.lr);
@@ -361,7 +363,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
       if (a.acc) a.acc[ch] = ok;
       if (a.acc_count) a.acc_count[ch] += ok;
       if (ok) a.chi0[ch] = s_chi1[c];
-      if (a.slot >= 0) a.tr_chi[a.slot * a.B + ch] = a.chi0[ch];
+      if (a.slot >= 0) a.tr_chi[a.slot * a.B + ch] = s_beta[c] * a.chi0[ch];
     }
     s_ok[c] = ok;
   }
@@ -639,6 +641,7 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
   a.eps = c.eps;
   a.u = c.u;
   a.step = c.step;
+  a.beta = c.beta;
   a.h0 = c.h0;
   a.h1 = c.h1;
   a.acc = c.acc;

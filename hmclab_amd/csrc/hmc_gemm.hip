@@ -361,7 +361,7 @@ struct EpiResid {
   const double* var;
   int M, N;
   template <typename V>
-  __device__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -380,6 +380,36 @@ struct EpiResid {
   }
 };
 
+// residual with a float32 running copy: raw = acc - d (full product G x) or
+// raw += acc (increment G dx); writes raw and the bf16 hi/lo planes of raw/var.
+struct EpiResidRaw {
+  static constexpr bool kReduce = false;
+  float* raw;       // [N_pad][ld]
+  uint16_t* oh;
+  uint16_t* ol;
+  int64_t ld;
+  const double* dobs;
+  const double* var;
+  int accumulate;
+  int M, N;
+  template <typename V>
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t m = m0 + (int64_t)e * ms;
+      if (m >= M) return;
+      const int64_t o = n * ld + m;
+      const float r = accumulate ? raw[o] + (float)v[e] : (float)v[e] - (float)dobs[m];
+      raw[o] = r;
+      const float f = r / (float)var[m];
+      const uint16_t h = f32_to_bf16_bits(f);
+      oh[o] = h;
+      ol[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
+    }
+  }
+};
+
 // trajectory step: g = acc + prior'(x); p -= kick*eps*g; x += drift*eps*kgrad(p);
 // corrector; optional second drift + corrector.
 template <typename T>
@@ -391,20 +421,23 @@ struct EpiLeap {
   uint16_t* xl;
   int64_t ld;
   const double* eps;
+  const double* beta;   // per-chain inverse temperature scaling the kick (null: 1)
   double kick, drift, drift2;
   DevModel md;
   int M, N;
+  int delta_planes;     // planes hold x_new - x_old (residual increments) instead of x
   template <typename V>
-  __device__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
     const double e = eps[n];
-    const T kk = (T)(kick * e), dd = (T)(drift * e), dd2 = (T)(drift2 * e);
+    const T kk = (T)(beta ? kick * e * beta[n] : kick * e), dd = (T)(drift * e), dd2 = (T)(drift2 * e);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int64_t j = m0 + (int64_t)k * ms;
       if (j >= M) return;
       const int64_t o = n * ld + j;
       T xv = x[o], pv = p[o];
+      const T xold = xv;
       T g = (T)v[k] + prior_grad<T>(md, j, xv);
       if (out_of_bounds(md, j, (double)xv)) g = (T)INFINITY;
       pv -= kk * g;
@@ -417,7 +450,7 @@ struct EpiLeap {
       x[o] = xv;
       p[o] = pv;
       if (xh) {
-        const float f = (float)xv;
+        const float f = delta_planes ? (float)(xv - xold) : (float)xv;
         const uint16_t h = f32_to_bf16_bits(f);
         xh[o] = h;
         xl[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
@@ -437,7 +470,7 @@ struct EpiGrad {
   DevModel md;
   int M, N;
   template <typename V>
-  __device__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -488,53 +521,75 @@ static void launch_f64(hipStream_t s, const TA* A, int64_t lda, const TB* Bt, in
   HMC_HIP(hipGetLastError());
 }
 
-template <int WM, int WN, int NSPLIT, class Epi>
+template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
 static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const uint16_t* B0,
                             const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
-  const int gm = (M + BF_BM - 1) / BF_BM, gn = (N + BF_BN - 1) / BF_BN;
-  constexpr int NT = (BF_BM / WM) * (BF_BN / WN) * 64;
-  hipLaunchKernelGGL((k_gemm_bf16<BF_BM, BF_BN, WM, WN, NSPLIT, BF_STAGES, Epi>),
-                     dim3(grid_for(gm, gn)), dim3(NT), 0, s, A, lda, B0, B1, ldb, M, N, K, epi);
+  const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN;
+  constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  hipLaunchKernelGGL((k_gemm_bf16<BM, BN, WM, WN, NSPLIT, S, Epi>), dim3(grid_for(gm, gn)), dim3(NT), 0, s, A,
+                     lda, B0, B1, ldb, M, N, K, epi);
   HMC_HIP(hipGetLastError());
 }
 
-template <class Epi>
-static void launch_bf16(const Problem& p, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
-                        const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
-  if (p.gemm_cfg == 0) {
-    if (split == 2) launch_bf16_cfg<64, 64, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
-    else launch_bf16_cfg<64, 64, 1>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
-  } else {
-    if (split == 2) launch_bf16_cfg<64, 32, 2>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
-    else launch_bf16_cfg<64, 32, 1>(p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
-  }
+// Tile configurations: 0 = 128x128 block, 4 waves of 64x64, 3-stage ring;
+// 1 = 128x128, 8 waves of 64x32 (two per SIMD), 3 stages;
+// 2 = 256x128, 4 waves of 128x64, 2 stages (half the LDS fragment traffic per
+//     MFMA of config 1, a quarter of the tiles).
+template <int NSPLIT, class Epi>
+static void launch_bf16_split(int cfg, hipStream_t st, const uint16_t* A, int64_t lda, const uint16_t* B0,
+                              const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+  if (cfg == 0)
+    launch_bf16_cfg<BF_BM, BF_BN, 64, 64, NSPLIT, BF_STAGES>(st, A, lda, B0, B1, ldb, M, N, K, epi);
+  else if (cfg == 1)
+    launch_bf16_cfg<BF_BM, BF_BN, 64, 32, NSPLIT, BF_STAGES>(st, A, lda, B0, B1, ldb, M, N, K, epi);
+  else
+    launch_bf16_cfg<256, 128, 128, 64, NSPLIT, 2>(st, A, lda, B0, B1, ldb, M, N, K, epi);
 }
 
-int64_t gemm_row_pad() { return 128; }   // M-tile of every GEMM config
+template <class Epi>
+static void launch_bf16(const Problem& p, int cfg, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
+                        const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+  if (split == 2) launch_bf16_split<2>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+  else launch_bf16_split<1>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+}
+
+int64_t gemm_row_pad() { return 256; }   // largest M-tile of the GEMM configs
 int64_t gemm_k_pad() { return 64; }      // k-tile multiple of every GEMM config
 int64_t gemm_chain_pad() { return 128; } // N-tile of every GEMM config
 
 // Trajectory: residual GEMM then gradient GEMM with kick/drift epilogue.
-void gemm_kick_drift(Chains& c, double kick, double drift, double drift2) {
+void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first) {
   Problem& p = *c.p;
   const int n = (int)p.n, d = (int)p.d, B = (int)c.B;
   const DevModel md = p.model();
   hipEvent_t ev1, ev2;
   if (p.precision == HMC_BF16) {
-    EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
+    const bool inc = p.resid_update && c.rf;
     time_begin(c, 0, ev1);
-    launch_bf16(p, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    if (inc) {
+      // first stage: full product G x (hi/lo planes of x); later stages: the
+      // float32 residual advances by G dx with one plane of dx
+      EpiResidRaw e1{c.rf, c.rh, c.rl, c.ldr, p.dobs, p.var, first ? 0 : 1, n, B};
+      launch_bf16(p, p.gemm_cfg, first ? p.bf16_split : 1, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B,
+                  (int)p.ldg, e1);
+    } else {
+      EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
+      launch_bf16(p, p.gemm_cfg, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg,
+                  e1);
+    }
     time_end(c, 0, ev1);
-    EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, c.xl, c.ldx, c.eps, kick, drift, drift2, md, d, B};
+    EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, c.xl, c.ldx, c.eps, c.beta, kick, drift, drift2, md, d, B,
+                      inc ? 1 : 0};
     time_begin(c, 3, ev2);
-    launch_bf16(p, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_bf16(p, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
     time_begin(c, 0, ev1);
     launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)c.xt, c.ldx, n, B, (int)p.ldg, e1);
     time_end(c, 0, ev1);
-    EpiLeap<double> e2{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, kick, drift, drift2, md, d, B};
+    EpiLeap<double> e2{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
+                       md, d, B, 0};
     time_begin(c, 3, ev2);
     launch_f64(p.stream, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
@@ -578,9 +633,9 @@ void gemm_gradient(Chains& c, const void* X, double* out) {
   if (p.precision == HMC_BF16) {
     // hi/lo planes of X were prepared by the caller into c.xh / c.xl
     EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
-    launch_bf16(p, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    launch_bf16(p, p.gemm_cfg, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
     EpiGrad<float> e2{(const float*)X, out, c.ldx, c.oob, md, d, B};
-    launch_bf16(p, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_bf16(p, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
     launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e1);

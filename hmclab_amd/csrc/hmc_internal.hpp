@@ -32,7 +32,9 @@ struct Problem {
   int engine = HMC_ENGINE_AUTO;
   int bf16_split = 2;       // bf16 planes of the positions in the residual GEMM (1 or 2)
   int bf16_split_grad = 2;  // bf16 planes of the residuals in the gradient GEMM (1 or 2)
-  int gemm_cfg = 1;         // bf16 GEMM tile configuration
+  int resid_update = 0;     // bf16: advance a float32 residual by G dx after the first stage
+  int gemm_cfg = 1;         // bf16 GEMM tile configuration of the residual GEMM
+  int gemm_cfg_grad = 1;    // ... of the gradient GEMM
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 
   // dense (row-major, zero padded): G [n_pad][ldg], Gt [d_pad][ldgt]
@@ -91,6 +93,7 @@ struct Chains {
   uint16_t* rh = nullptr;   // bf16 hi/lo planes of the residual
   uint16_t* rl = nullptr;
   void* r = nullptr;        // residual [B_pad][ldr] (state dtype, F64/F32 GEMM)
+  float* rf = nullptr;      // float32 running residual G x - d (bf16 GEMM, incremental mode)
   double* partial = nullptr;// endpoint partial sums [mblocks][B_pad]
   int64_t n_mblocks = 0;
   double* k0 = nullptr;     // kinetic energy at the start of the trajectory
@@ -118,6 +121,16 @@ struct Chains {
 
   // learned surrogate driving the trajectory gradient (GEMM engine), or null
   void* surrogate = nullptr;
+
+  // tempering: chain c samples exp(-beta[c] chi); null = all beta 1
+  double* beta = nullptr;
+  // replica exchange schedule for device runs: event e (after proposal e*ex_interval)
+  // pairs ex_pairs[e][2k], ex_pairs[e][2k+1] (the second decides)
+  int32_t* ex_pairs = nullptr;
+  int64_t ex_events = 0, ex_npairs = 0, ex_interval = 0;
+  int64_t* ex_count = nullptr;  // accepted exchanges per chain
+  int32_t* ex_tmp = nullptr;    // staging of one host-supplied event: pairs, flags
+  double* ex_u = nullptr;       // staging of its uniforms
 
   std::vector<void*> owned;
 };
@@ -167,9 +180,11 @@ void time_end(Chains& c, int kind, hipEvent_t a);
 void time_collect(Chains& c);
 
 // ---- GEMM engine (hmc_gemm.hip) -------------------------------------------
+// Row padding of G and G^T: every GEMM configuration's M-tile divides it.
+int64_t gemm_row_pad();
 // Trajectory gradient + fused kick/drift for every chain:
 //   p -= kick*eps*g(xt); xt += drift*eps*Minv p; corrector; [xt += drift2*eps*Minv p; corrector]
-void gemm_kick_drift(Chains& c, double kick, double drift, double drift2);
+void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);
 // Endpoint partial sums of the data misfit in FP64 into c.partial.
 void gemm_endpoint(Chains& c, const void* X, int x_dtype);
 // gradient (B x d) of model rows X into out (device, f64, row stride ldx)
@@ -190,6 +205,12 @@ void elem_finish_misfit(Chains& c, const void* X, bool have_partials, double* ou
 void elem_load_rows(Chains& c, const double* src, void* dst, uint16_t* hi, uint16_t* lo);
 void elem_store_rows(Chains& c, const void* src, int src_dtype, int64_t ld, double* dst);
 void elem_oob_flags(Chains& c, const void* X);
+// one replica-exchange event over npairs pairs (device array [2*npairs]); u: device
+// uniforms per pair, or null for the Philox exchange stream keyed by (seed, decider, prop)
+// slot >= 0 rewrites that trace slot's rows of swapped chains (the recorded
+// sample of a proposal is its post-exchange state)
+void elem_exchange(Chains& c, const int32_t* pairs, int64_t npairs, const double* u, uint64_t seed,
+                   int64_t prop, int32_t* swapped, int64_t slot);
 
 // ---- fused engine (hmc_fused.hip) ------------------------------------------
 bool fused_supported(const Problem& p);

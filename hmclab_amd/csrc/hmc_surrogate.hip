@@ -166,17 +166,18 @@ __global__ void k_norm(const T* X, int64_t ldx, float* Xn, int64_t Dp, int64_t D
 // surrogate kick + drift: g = G[b][j] * gscale_j + prior'(x); then as EpiLeap
 template <typename T>
 __global__ void k_sur_leap(T* x, T* p, int64_t ldx, const float* G, int64_t Dp, int64_t D,
-                           const float* gscale, const double* eps, double kick, double drift,
-                           double drift2, int64_t rows, DevModel md) {
+                           const float* gscale, const double* eps, const double* beta, double kick,
+                           double drift, double drift2, int64_t rows, DevModel md) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= rows * D) return;
   const int64_t b = i / D, j = i % D;
   const double e = eps[b];
+  const double ke = beta ? kick * e * beta[b] : kick * e;
   const int64_t o = b * ldx + j;
   T xv = x[o], pv = p[o];
   T g = (T)(G[b * Dp + j] * gscale[j]) + prior_grad<T>(md, j, xv);
   if (out_of_bounds(md, j, (double)xv)) g = (T)INFINITY;
-  pv -= (T)(kick * e) * g;
+  pv -= (T)ke * g;
   xv += (T)(drift * e) * kgrad<T>(md, j, pv);
   corrector<T>(md, j, xv, pv);
   if (drift2 != 0.0) {
@@ -439,10 +440,12 @@ void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2) {
   time_begin(c, 3, ev2);
   if (p.state_dtype == HMC_F64)
     hipLaunchKernelGGL(k_sur_leap<double>, g1(c.B * p.d), dim3(256), 0, s.stream, (double*)c.xt, (double*)c.pt,
-                       c.ldx, s.G, s.Dp, p.d, s.gscale, c.eps, kick, drift, drift2, c.B, p.model());
+                       c.ldx, s.G, s.Dp, p.d, s.gscale, c.eps, (const double*)c.beta, kick, drift, drift2, c.B,
+                       p.model());
   else
     hipLaunchKernelGGL(k_sur_leap<float>, g1(c.B * p.d), dim3(256), 0, s.stream, (float*)c.xt, (float*)c.pt,
-                       c.ldx, s.G, s.Dp, p.d, s.gscale, c.eps, kick, drift, drift2, c.B, p.model());
+                       c.ldx, s.G, s.Dp, p.d, s.gscale, c.eps, (const double*)c.beta, kick, drift, drift2, c.B,
+                       p.model());
   HMC_HIP(hipGetLastError());
   time_end(c, 3, ev2);
 }

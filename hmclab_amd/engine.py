@@ -213,6 +213,37 @@ class Chains:
     def sync(self):
         call("hmc_chains_sync", self._h)
 
+    def set_temperatures(self, temperatures=None):
+        """Chain c samples exp(-chi / T_c); None resets every chain to T = 1."""
+        if temperatures is None:
+            call("hmc_chains_set_temperatures", self._h, None)
+            return
+        T = np.ascontiguousarray(np.broadcast_to(np.asarray(temperatures, dtype=np.float64), (self.B,)))
+        call("hmc_chains_set_temperatures", self._h, dptr(T))
+
+    def exchange(self, pairs, u, proposal: int = 0):
+        """One replica-exchange event: pairs (k, 2) with the second chain deciding, u (k,)."""
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(pairs.shape[0]))
+        swapped = np.zeros(pairs.shape[0], dtype=np.int32)
+        call("hmc_chains_exchange", self._h, iptr(pairs), pairs.shape[0], dptr(u), int(proposal), iptr(swapped))
+        return swapped.astype(bool)
+
+    def set_exchange(self, interval=None, schedule=None):
+        """Schedule (events, 2k) applied by run() after proposals t % interval == 0."""
+        if schedule is None:
+            call("hmc_chains_set_exchange", self._h, 0, None, 0, 0)
+            return
+        sched = np.ascontiguousarray(np.asarray(schedule, dtype=np.int32))
+        if sched.ndim != 2 or sched.shape[1] % 2:
+            raise ValueError("schedule must have shape (events, 2 * pairs)")
+        call("hmc_chains_set_exchange", self._h, int(interval), iptr(sched), sched.shape[0], sched.shape[1] // 2)
+
+    def exchange_counts(self):
+        n = np.zeros(self.B, dtype=np.int64)
+        call("hmc_chains_exchange_counts", self._h, lptr(n))
+        return n
+
     def set_surrogate(self, surrogate=None):
         """Drive the trajectory with a learned surrogate (None: exact gradient)."""
         call("hmc_chains_set_surrogate", self._h, None if surrogate is None else surrogate._h)

@@ -120,8 +120,15 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 GEMMs (2 = hi/lo split, ~16-bit operand precision; default 2)
  *   "bf16_split_grad" 1 or 2: the same for the gradient GEMM only (its operand
  *                 is the scaled residual)
- *   "gemm_config" 0 or 1: bf16 GEMM tile (0 = 128x128, 4 waves of 64x64;
- *                 1 = 128x128, 8 waves of 64x32; default 1)
+ *   "incremental_residual" 0 or 1: bf16 trajectories keep a float32
+ *                 residual G x - d; the first stage of a trajectory computes it
+ *                 from hi/lo planes of x, later stages add G dx with one bf16
+ *                 plane of the position change dx (half the MFMA work of the
+ *                 residual GEMM; default 0)
+ *   "gemm_config" 0, 1 or 2: bf16 GEMM tile of both trajectory GEMMs
+ *                 (0 = 128x128, 4 waves of 64x64; 1 = 128x128, 8 waves of
+ *                 64x32; 2 = 256x128, 4 waves of 128x64; default 1)
+ *   "gemm_config_grad" 0, 1 or 2: the same for the gradient GEMM only
  * Must be set before hmc_chains_create. */
 int hmc_problem_set_option(hmc_problem* p, const char* name, double value);
 
@@ -199,6 +206,28 @@ int hmc_chains_timing(hmc_chains* c, int kind, double* total_ms, int64_t* launch
 
 /* Synchronise the engine's stream. */
 int hmc_chains_sync(hmc_chains* c);
+
+/* ---- tempering and replica exchange ----------------------------------------
+ * Replaces ParallelSampleSMP with exchange (Samplers.py:1816-1977, exchange
+ * step :589-669) for chains whose posteriors differ by a temperature T_c
+ * (misfit / T_c, as the reference's tempered distributions do, e.g.
+ * base.py:392-414): chain c samples exp(-chi(m) / T_c).  NULL resets all
+ * chains to T = 1.  Misfits returned by hmc_chains_get_state and recorded in
+ * the trace are the tempered ones, chi / T_c. */
+int hmc_chains_set_temperatures(hmc_chains* c, const double* temperatures);
+/* One exchange event with host uniforms: pairs[2k], pairs[2k+1] (the second
+ * decides, as the reference's odd schedule position does), u[k] its uniform;
+ * the pair swaps positions and misfits when
+ * exp(chi_b/T_b - chi_a/T_b + chi_a/T_a - chi_b/T_a) > u[k].  swapped[k] out (may be NULL). */
+int hmc_chains_exchange(hmc_chains* c, const int32_t* pairs, int64_t npairs, const double* u,
+                        int64_t proposal, int32_t* swapped);
+/* Exchange schedule for hmc_chains_run: after proposal t with t % interval
+ * == 0 and t / interval < events, row t / interval of schedule (events x
+ * 2*npairs chain indices) is applied with device uniforms.  NULL clears it. */
+int hmc_chains_set_exchange(hmc_chains* c, int64_t interval, const int32_t* schedule, int64_t events,
+                            int64_t npairs);
+/* Accepted exchanges per chain since creation. */
+int hmc_chains_exchange_counts(hmc_chains* c, int64_t* counts);
 
 /* ---- learned surrogate potential ------------------------------------------
  * No counterpart in the reference: hmclab's HMC always calls the target's

@@ -507,3 +507,88 @@ def analytic_linear_gaussian(G, d, data_variance, prior_means, prior_variance):
     Sigma = np.linalg.inv(H)
     mu = Sigma @ rhs
     return mu, Sigma
+
+
+# --------------------------------------------------------------------------- #
+# Tempering and replica exchange (ParallelSampleSMP, Samplers.py:1804-1977)
+# --------------------------------------------------------------------------- #
+class Tempered:
+    """exp(-chi / T): misfit and gradient divided by the temperature, as the
+    reference's tempered targets do (e.g. StandardNormal1D, base.py:392-414)."""
+
+    def __init__(self, dist, temperature):
+        self.dist = dist
+        self.temperature = float(temperature)
+        self.dimensions = dist.dimensions
+
+    def misfit(self, x):
+        return self.dist.misfit(x) / self.temperature
+
+    def gradient(self, x):
+        return self.dist.gradient(x) / self.temperature
+
+    def corrector(self, x, p):
+        self.dist.corrector(x, p)
+
+
+def exchange_schedule(rng, chains, proposals, exchange_interval):
+    """Samplers.py:1880-1893: one row of 2*floor(N/2) distinct chains per event."""
+    per = int(np.floor(chains / 2))
+    return np.vstack([rng.choice(chains, per * 2, replace=False)
+                      for _ in range(int(proposals / exchange_interval))])
+
+
+def exchange_event(dists, xs, chis, row, uniforms):
+    """One exchange event (Samplers.py:596-669) on lists of positions and
+    current misfits (each chain's own, tempered).  Pair k = (row[2k], row[2k+1]);
+    the second ("master", odd schedule position) accepts when
+    exp(improvement_master + improvement_other) > u_k.  Here the misfits move
+    with the models (the reference leaves current_x in place)."""
+    swapped = []
+    for k in range(len(row) // 2):
+        a, b = int(row[2 * k]), int(row[2 * k + 1])
+        imp_b = chis[b] - dists[b].misfit(xs[a])
+        imp_a = chis[a] - dists[a].misfit(xs[b])
+        with np.errstate(over="ignore"):
+            ok = bool(np.exp(imp_b + imp_a) > uniforms[k])
+        if ok:
+            xs[a], xs[b] = xs[b].copy(), xs[a].copy()
+            chis[a], chis[b] = dists[a].misfit(xs[a]), dists[b].misfit(xs[b])
+        swapped.append(ok)
+    return swapped
+
+
+def sample_parallel(dists, masses, rngs, schedule=None, exchange_interval=1, stepsize=0.1,
+                    randomize_stepsize=True, amount_of_steps=10, initial_models=None, proposals=100):
+    """N seeded chains advanced in lock step with replica exchange: per chain the
+    reference's draw order (momentum, step jitter, accept uniform), then the
+    exchange event's uniform from the deciding chain's generator.  Returns the
+    list of per-chain sample arrays (d+1, proposals) and the swap decisions."""
+    n = len(dists)
+    d = dists[0].dimensions
+    for m, r in zip(masses, rngs):
+        m.rng = r
+    xs = [np.zeros((d, 1)) if initial_models is None else np.array(initial_models[c], dtype=np.float64).reshape(d, 1)
+          for c in range(n)]
+    chis = [dists[c].misfit(xs[c]) for c in range(n)]
+    cols = [[] for _ in range(n)]
+    decisions = []
+    for i in range(proposals):
+        for c in range(n):
+            p0 = masses[c].generate_momentum()
+            eps = rngs[c].uniform(0.5, 1.5) * stepsize if randomize_stepsize else stepsize
+            u = None
+            res = propose(dists[c], masses[c], xs[c], None, p0, eps, amount_of_steps, 0.0)
+            u = rngs[c].uniform(0, 1)
+            if bool(res["a"] > u):
+                xs[c] = np.copy(res["x1"])
+                chis[c] = res["chi1"]
+            else:
+                chis[c] = res["chi0"]
+        if schedule is not None and i % exchange_interval == 0 and i // exchange_interval < len(schedule):
+            row = schedule[i // exchange_interval]
+            us = [rngs[int(row[2 * k + 1])].uniform(0, 1) for k in range(len(row) // 2)]
+            decisions.append(exchange_event(dists, xs, chis, row, us))
+        for c in range(n):
+            cols[c].append(np.vstack([xs[c], chis[c]]))
+    return [np.hstack(c) for c in cols], decisions
