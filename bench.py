@@ -69,9 +69,22 @@ def analytic_posterior(torch, Gf, dd, sigma2, prior_var=1.0):
     mu = torch.cholesky_solve(rhs[:, None], L)[:, 0]
     Linv = torch.linalg.solve_triangular(L, torch.eye(H.shape[0], device="cuda", dtype=torch.float64), upper=False)
     var = (Linv * Linv).sum(dim=0)        # diag(H^-1) = column norms of L^-1
-    lam_max = torch.linalg.eigvalsh(H)[-1].item()
+    lam_max = _power_lam_max(torch, H)
     del G, H
     return mu, var, L, lam_max
+
+
+def _power_lam_max(torch, H, iters=60):
+    """Largest eigenvalue of a symmetric positive definite H by power iteration
+    (only the initial step size depends on it; the warm-up tunes the step)."""
+    g = torch.Generator(device=H.device).manual_seed(0)
+    v = torch.randn(H.shape[0], device=H.device, dtype=H.dtype, generator=g)
+    lam = 0.0
+    for _ in range(iters):
+        w = H @ v
+        lam = float(torch.dot(v, w) / torch.dot(v, v))
+        v = w / torch.linalg.vector_norm(w)
+    return lam
 
 
 def posterior_draws(torch, mu, L, B, seed):
@@ -132,7 +145,7 @@ def _posterior_errors(torch, ch, K, mu_np, sd_np, d):
     from hmclab_amd import diagnostics
 
     draws = trace_to_torch(torch, ch, K)
-    ess = diagnostics.ess_bulk(draws)
+    ess = diagnostics.ess_bulk(draws) if K >= 4 else np.full(d, np.nan)   # short profiling runs
     pooled = draws.to(torch.float64).reshape(-1, d)
     mean = pooled.mean(dim=0).cpu().numpy()
     std = pooled.std(dim=0).cpu().numpy()
@@ -251,7 +264,10 @@ def run_dense(args):
             opts["bf16_split_grad"] = args.bf16_split_grad
         if args.gemm_config_grad is not None:
             opts["gemm_config_grad"] = args.gemm_config_grad
-        opts["incremental_residual"] = args.incremental_residual
+        if args.incremental_residual:
+            opts["incremental_residual"] = args.incremental_residual
+        if args.tile_order:
+            opts["tile_order"] = args.tile_order
     prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)
@@ -299,9 +315,11 @@ def run_dense(args):
         f"kick-drift gemm {leap_ms:.1f} ms/{leap_n}, endpoint {end_ms:.1f} ms/{end_n}, "
         f"elementwise {ew_ms:.1f} ms/{ew_n}")
     if leap_ms >= res_ms:
-        gemm_ms, gemm_n, gemm_name = leap_ms, leap_n, "k_gemm_bf16<...,EpiLeap> (gradient GEMM + kick/drift)"
+        gemm_ms, gemm_n, gemm_key = leap_ms, leap_n, "gradient"
+        gemm_name = "k_gemm_bf16<...,EpiLeap> (gradient GEMM + kick/drift)"
     else:
-        gemm_ms, gemm_n, gemm_name = res_ms, res_n, "k_gemm_bf16<...,EpiResid> (residual GEMM)"
+        gemm_ms, gemm_n, gemm_key = res_ms, res_n, "residual"
+        gemm_name = "k_gemm_bf16<...,EpiResid> (residual GEMM)"
     min_ess, err_mean, err_std = _posterior_errors(torch, ch, args.steps, mu_np, sd_np, d)
     acc_rate = float(acc.mean() / args.steps)
     value = min_ess / elapsed
@@ -310,11 +328,14 @@ def run_dense(args):
     avg_s = gemm_ms / 1e3 / max(gemm_n, 1)
     achieved = flop_launch / avg_s
     peak = PEAK["bf16"] if args.precision == "bf16" else PEAK[args.precision]
+    # HBM bytes per launch of the dominant kernel from the committed counter
+    # profile (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel); only for
+    # the configuration it was measured on
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_dense_gemm.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and args.precision == "bf16" and n == 20000 and d == 8192 and B == 512:
         try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
+            traffic = json.load(open(tfile))["bytes_per_launch"].get(gemm_key)
         except Exception:
             traffic = None
 
@@ -329,7 +350,7 @@ def run_dense(args):
         "trajectory_gradient_rel_err_vs_fp64": grad_err,
         "gemm": {"config": args.gemm_config, "config_grad": args.gemm_config_grad,
                  "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
-                 "incremental_residual": args.incremental_residual},
+                 "incremental_residual": args.incremental_residual, "tile_order": args.tile_order},
     }
     if sur_info is not None:
         sur_info.update(chunk_acceptance=sur_acc, fallback_after_proposals=fallback_at,
@@ -432,7 +453,7 @@ def run_tomography(args):
     sd = torch.sqrt((Linv * Linv).sum(dim=0)).cpu().numpy()
     mass = H.diagonal().cpu().numpy().copy()          # Diagonal mass = diag of the posterior precision
     Ms = torch.from_numpy(1.0 / np.sqrt(mass)).to("cuda")
-    lam_max = torch.linalg.eigvalsh(Ms[:, None] * H * Ms[None, :])[-1].item()
+    lam_max = _power_lam_max(torch, Ms[:, None] * H * Ms[None, :])
     X0 = posterior_draws(torch, mu, Lc, B, args.seed + 1)
     mu_np = mu.cpu().numpy()
     del Gt, H, Linv, Lc
@@ -448,7 +469,7 @@ def run_tomography(args):
     elapsed, acc = _timed(ch, B, L, eps, args.steps, args.seed, done, torch)
     k_ms, k_n = ch.timing(0)
     draws = trace_to_torch(torch, ch, args.steps)
-    ess = diagnostics.ess_bulk(draws)
+    ess = diagnostics.ess_bulk(draws) if args.steps >= 4 else np.full(d, np.nan)
     pooled = draws.to(torch.float64).reshape(-1, d)
     mean, std = pooled.mean(dim=0).cpu().numpy(), pooled.std(dim=0).cpu().numpy()
     del draws, pooled
@@ -527,7 +548,7 @@ def run_uneven(args):
     for p in range(args.problems):
         a, b = ps.offsets[p], ps.offsets[p + 1]
         draws = X[:, a:b].reshape(args.steps, CHAINS, int(d[p])).transpose(1, 0, 2)
-        ess_min.append(float(np.min(diagnostics.ess_bulk(draws))))
+        ess_min.append(float(np.min(diagnostics.ess_bulk(draws))) if args.steps >= 4 else float("nan"))
     ess_min = np.array(ess_min)
     rates = acc.reshape(-1, CHAINS).sum(axis=1) / (CHAINS * args.steps)
     # algorithmic bytes per round (one pass of G per step, x and p read+write)
@@ -658,7 +679,8 @@ def main():
                     help="bf16 planes of the scaled residual in the gradient GEMM (1: one plane, "
                          "measured to leave acceptance unchanged; 2: hi/lo)")
     ap.add_argument("--gemm-config", type=int, default=1, choices=[0, 1, 2])
-    ap.add_argument("--incremental-residual", type=int, default=0, choices=[0, 1],
+    ap.add_argument("--tile-order", type=int, default=0, choices=[0, 1])
+    ap.add_argument("--incremental-residual", type=int, default=1, choices=[0, 1],
                     help="bf16: advance a float32 residual by G dx after the first stage")
     ap.add_argument("--gemm-config-grad", type=int, default=None, choices=[0, 1, 2],
                     help="tile configuration of the gradient GEMM (default: --gemm-config)")

@@ -456,6 +456,9 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
     } else if (key == "bf16_split_grad") {
       HMC_REQUIRE(value == 1.0 || value == 2.0, HMC_ERR_ARG, "bf16_split_grad must be 1 or 2");
       P.bf16_split_grad = (int)value;
+    } else if (key == "tile_order") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "tile_order must be 0 or 1");
+      P.tile_order = (int)value;
     } else if (key == "incremental_residual") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "incremental_residual must be 0 or 1");
       P.resid_update = (int)value;

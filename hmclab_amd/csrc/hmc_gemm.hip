@@ -50,14 +50,21 @@ __device__ inline void ld8(const uint16_t* p, double o[8]) {
 // XCD-aware tile order: consecutive workgroup ids are dealt round-robin to
 // the 8 XCDs; give each XCD a contiguous range of tiles (m-major) so tiles
 // sharing rows of A run on the same L2.
-__device__ inline bool map_tile(int gm, int gn, int& tm, int& tn) {
+// order 1 deals n-major ranges instead (tiles sharing columns of B, i.e. the
+// same chains, run on the same L2).
+__device__ inline bool map_tile(int gm, int gn, int& tm, int& tn, int order = 0) {
   const int T = gm * gn;
   const int chunk = (T + 7) / 8;
   const int L = blockIdx.x;
   const int t = (L % 8) * chunk + (L / 8);
   if (t >= T) return false;
-  tm = t / gn;
-  tn = t % gn;
+  if (order == 0) {
+    tm = t / gn;
+    tn = t % gn;
+  } else {
+    tm = t % gm;
+    tn = t / gm;
+  }
   return true;
 }
 static inline int grid_for(int gm, int gn) { return ((gm * gn + 7) / 8) * 8; }
@@ -231,7 +238,7 @@ __device__ inline void lds_barrier() {
 template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
     const uint16_t* __restrict__ A, int64_t lda, const uint16_t* __restrict__ B0,
-    const uint16_t* __restrict__ B1, int64_t ldb, int M, int N, int K, Epi epi) {
+    const uint16_t* __restrict__ B1, int64_t ldb, int M, int N, int K, int order, Epi epi) {
   constexpr int BK = 64;                 // elements per row per k-tile (128 bytes)
   constexpr int NW = (BM / WM) * (BN / WN);
   constexpr int TM = WM / 32, TN = WN / 32;
@@ -242,10 +249,14 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
   constexpr int GPW = GROUPS / NW;       // load instructions per wave per k-tile
   constexpr int STAGE = (BM + NSPLIT * BN) * BK;   // elements per stage
   static_assert(GPW * (S - 2) < 64, "vmcnt range");
-  __shared__ __attribute__((aligned(1024))) uint16_t smem[S * STAGE];
+  // the k ring, reused by the epilogue for the block's FP32 tile C[BN][BM + 4]
+  constexpr int CLD = BM + 4;
+  constexpr int SMEM = (S * STAGE > BN * CLD * 2) ? S * STAGE : BN * CLD * 2;
+  static_assert(SMEM * 2 <= 160 * 1024, "LDS budget");
+  __shared__ __attribute__((aligned(1024))) uint16_t smem[SMEM];
 
   int tm, tn;
-  if (!map_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn)) return;
+  if (!map_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn, order)) return;
   const int64_t m_blk = (int64_t)tm * BM, n_blk = (int64_t)tn * BN;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -332,18 +343,32 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
     }
   }
 
-  // D layout (32x32): lane (q, r) holds rows 8g + 4q + v (g, v = 0..3) of column r
+  // Epilogue through LDS.  D layout (32x32): lane (q, r) holds rows 8g + 4q + v
+  // (g, v = 0..3) of column r, so direct stores would scatter 16-byte pieces
+  // over 32 chains.  The block's tile goes to LDS as C[n][m] (the ring is free
+  // once every wave has left the k loop), then each thread applies the
+  // epilogue to 4 consecutive rows of one chain: a wavefront covers 256
+  // consecutive rows of one chain row in the chain-major outputs.
+  float* Cs = reinterpret_cast<float*>(smem);
+  __syncthreads();
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        const int64_t m0 = m_blk + wm * WM + i * 32 + 8 * g + 4 * q;
-        const int64_t n = n_blk + wn * WN + j * 32 + r;
-        float v[4] = {acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]};
-        epi.apply(m0, 1, n, v);
+        const int ml = wm * WM + i * 32 + 8 * g + 4 * q;
+        const int nl = wn * WN + j * 32 + r;
+        *reinterpret_cast<floatx4*>(&Cs[nl * CLD + ml]) =
+            floatx4{acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]};
       }
+  __syncthreads();
+  for (int idx = tid; idx < BN * (BM / 4); idx += NW * 64) {
+    const int nl = idx / (BM / 4), ml = 4 * (idx % (BM / 4));
+    const floatx4 c = *reinterpret_cast<const floatx4*>(&Cs[nl * CLD + ml]);
+    const float v[4] = {c[0], c[1], c[2], c[3]};
+    epi.apply(m_blk + ml, 1, n_blk + nl, v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -523,11 +548,11 @@ static void launch_f64(hipStream_t s, const TA* A, int64_t lda, const TB* Bt, in
 
 template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
 static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const uint16_t* B0,
-                            const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+                            const uint16_t* B1, int64_t ldb, int M, int N, int K, int order, const Epi& epi) {
   const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN;
   constexpr int NT = (BM / WM) * (BN / WN) * 64;
   hipLaunchKernelGGL((k_gemm_bf16<BM, BN, WM, WN, NSPLIT, S, Epi>), dim3(grid_for(gm, gn)), dim3(NT), 0, s, A,
-                     lda, B0, B1, ldb, M, N, K, epi);
+                     lda, B0, B1, ldb, M, N, K, order, epi);
   HMC_HIP(hipGetLastError());
 }
 
@@ -537,20 +562,22 @@ static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const
 //     MFMA of config 1, a quarter of the tiles).
 template <int NSPLIT, class Epi>
 static void launch_bf16_split(int cfg, hipStream_t st, const uint16_t* A, int64_t lda, const uint16_t* B0,
-                              const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+                              const uint16_t* B1, int64_t ldb, int M, int N, int K, int order, const Epi& epi) {
+  // one-plane operands need 32 KB per stage: a 4-stage ring keeps three k-tiles in flight
+  constexpr int S = NSPLIT == 1 ? 4 : BF_STAGES;
   if (cfg == 0)
-    launch_bf16_cfg<BF_BM, BF_BN, 64, 64, NSPLIT, BF_STAGES>(st, A, lda, B0, B1, ldb, M, N, K, epi);
+    launch_bf16_cfg<BF_BM, BF_BN, 64, 64, NSPLIT, S>(st, A, lda, B0, B1, ldb, M, N, K, order, epi);
   else if (cfg == 1)
-    launch_bf16_cfg<BF_BM, BF_BN, 64, 32, NSPLIT, BF_STAGES>(st, A, lda, B0, B1, ldb, M, N, K, epi);
+    launch_bf16_cfg<BF_BM, BF_BN, 64, 32, NSPLIT, S>(st, A, lda, B0, B1, ldb, M, N, K, order, epi);
   else
-    launch_bf16_cfg<256, 128, 128, 64, NSPLIT, 2>(st, A, lda, B0, B1, ldb, M, N, K, epi);
+    launch_bf16_cfg<256, 128, 128, 64, NSPLIT, 2>(st, A, lda, B0, B1, ldb, M, N, K, order, epi);
 }
 
 template <class Epi>
 static void launch_bf16(const Problem& p, int cfg, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
                         const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
-  if (split == 2) launch_bf16_split<2>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
-  else launch_bf16_split<1>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, epi);
+  if (split == 2) launch_bf16_split<2>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, epi);
+  else launch_bf16_split<1>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, epi);
 }
 
 int64_t gemm_row_pad() { return 256; }   // largest M-tile of the GEMM configs

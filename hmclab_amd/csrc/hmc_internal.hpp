@@ -33,6 +33,7 @@ struct Problem {
   int bf16_split = 2;       // bf16 planes of the positions in the residual GEMM (1 or 2)
   int bf16_split_grad = 2;  // bf16 planes of the residuals in the gradient GEMM (1 or 2)
   int resid_update = 0;     // bf16: advance a float32 residual by G dx after the first stage
+  int tile_order = 0;       // bf16 GEMM tiles per XCD: 0 m-major, 1 n-major
   int gemm_cfg = 1;         // bf16 GEMM tile configuration of the residual GEMM
   int gemm_cfg_grad = 1;    // ... of the gradient GEMM
   int tgt_dtype = HMC_F64;  // element type defining the target's G

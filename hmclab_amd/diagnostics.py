@@ -89,7 +89,10 @@ def _split(draws):
 
 
 def ess_bulk(draws) -> np.ndarray:
-    """Bulk ESS per parameter of draws (M chains, N draws, P params)."""
+    """Bulk ESS per parameter of draws (M chains, N draws, P params); N >= 4
+    (split halves need at least two draws each)."""
+    if draws.shape[1] < 4:
+        raise ValueError("bulk ESS needs at least 4 draws per chain")
     if hasattr(draws, "narrow"):
         return _ess_bulk_torch(draws)
     x = np.asarray(draws, dtype=np.float64)

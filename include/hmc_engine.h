@@ -129,6 +129,9 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 (0 = 128x128, 4 waves of 64x64; 1 = 128x128, 8 waves of
  *                 64x32; 2 = 256x128, 4 waves of 128x64; default 1)
  *   "gemm_config_grad" 0, 1 or 2: the same for the gradient GEMM only
+ *   "tile_order"  0 or 1: bf16 GEMM tiles dealt to the 8 XCDs in m-major
+ *                 (tiles sharing G rows share an L2; default) or n-major
+ *                 (tiles sharing chains share an L2) ranges
  * Must be set before hmc_chains_create. */
 int hmc_problem_set_option(hmc_problem* p, const char* name, double value);
 

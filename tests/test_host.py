@@ -130,3 +130,12 @@ def test_sampler_argument_checks(kw, msg):
 def test_sampler_unknown_integrator():
     with pytest.raises(ValueError, match="Unknown integrator"):
         Samplers.HMC().sample(None, D.Normal(np.zeros((2, 1)), 1.0), integrator="rk4")
+
+
+def test_ess_rejects_too_short_chains():
+    import pytest
+
+    from hmclab_amd import diagnostics
+
+    with pytest.raises(ValueError, match="at least 4 draws"):
+        diagnostics.ess_bulk(np.zeros((3, 3, 2)))
