@@ -142,6 +142,9 @@ def trace_to_torch(torch, ch, K):
 
 
 def _posterior_errors(torch, ch, K, mu_np, sd_np, d):
+    """min-ESS and the RMS errors of the pooled mean (in units of the reference
+    sd) and std (relative) against the reference moments (mu_np, sd_np; None:
+    no reference).  Also returns the pooled mean and std."""
     from hmclab_amd import diagnostics
 
     draws = trace_to_torch(torch, ch, K)
@@ -150,9 +153,11 @@ def _posterior_errors(torch, ch, K, mu_np, sd_np, d):
     mean = pooled.mean(dim=0).cpu().numpy()
     std = pooled.std(dim=0).cpu().numpy()
     del draws, pooled
-    err_mean = float(np.sqrt(np.mean(((mean - mu_np) / sd_np) ** 2)))
-    err_std = float(np.sqrt(np.mean((std / sd_np - 1.0) ** 2)))
-    return float(np.min(ess)), err_mean, err_std
+    err_mean = err_std = None
+    if mu_np is not None:
+        err_mean = float(np.sqrt(np.mean(((mean - mu_np) / sd_np) ** 2)))
+        err_std = float(np.sqrt(np.mean((std / sd_np - 1.0) ** 2)))
+    return float(np.min(ess)), err_mean, err_std, mean, std
 
 
 def _kernel_times(ch):
@@ -188,7 +193,7 @@ def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np):
     sub.warmup = min(args.warmup, 5)
     eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), sub, args.seed + 7, "fp64 ")
     elapsed, acc = _timed(ch, B, L, eps, args.plain_steps, args.seed + 7, done, torch)
-    min_ess, em, es = _posterior_errors(torch, ch, args.plain_steps, mu_np, sd_np, d)
+    min_ess, em, es, mean, std = _posterior_errors(torch, ch, args.plain_steps, mu_np, sd_np, d)
     kt = _kernel_times(ch)
     ch.close()
     prob.close()
@@ -197,7 +202,7 @@ def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np):
     return dict(value=min_ess / elapsed, min_ess=min_ess, seconds=elapsed, proposals=args.plain_steps,
                 acceptance_rate=float(acc.mean() / args.plain_steps), stepsize=eps,
                 posterior_rms_mean_err_over_sd=em, posterior_rms_std_rel_err=es,
-                ms_per_proposal=elapsed / args.plain_steps * 1e3,
+                ms_per_proposal=elapsed / args.plain_steps * 1e3, _mean=mean, _std=std,
                 gemm_fp64_avg_us=(kt[0][0] + kt[3][0]) / max(kt[0][1] + kt[3][1], 1) * 1e3)
 
 
@@ -253,9 +258,18 @@ def run_dense(args):
     torch.cuda.empty_cache()
     log(f"analytic posterior: lam_max={lam_max:.4g} ({time.perf_counter() - t_setup:.1f}s)")
 
-    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0),
-                        D.LinearMatrix(Gf.astype(np.float64), dd, sigma * sigma, dtype="bfloat16")])
+    if args.prior == "laplace":
+        # Laplace prior (dispersion 0.3): no closed-form posterior; both legs are
+        # compared with each other (the plain FP64 leg is the reference run) and
+        # start from the Gaussian-prior posterior, which the warm-up moves off
+        prior = D.Laplace(np.zeros((d, 1)), np.full((d, 1), 0.3))
+        mu_np = sd_np = None
+    else:
+        prior = D.Normal(np.zeros((d, 1)), 1.0)
+    post = D.BayesRule([prior, D.LinearMatrix(Gf.astype(np.float64), dd, sigma * sigma, dtype="bfloat16")])
     plain = _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np) if args.plain_steps > 0 else None
+    if plain is not None and mu_np is None:
+        mu_np, sd_np = plain["_mean"], plain["_std"]
 
     opts = {}
     if args.precision == "bf16":
@@ -320,7 +334,7 @@ def run_dense(args):
     else:
         gemm_ms, gemm_n, gemm_key = res_ms, res_n, "residual"
         gemm_name = "k_gemm_bf16<...,EpiResid> (residual GEMM)"
-    min_ess, err_mean, err_std = _posterior_errors(torch, ch, args.steps, mu_np, sd_np, d)
+    min_ess, err_mean, err_std, _, _ = _posterior_errors(torch, ch, args.steps, mu_np, sd_np, d)
     acc_rate = float(acc.mean() / args.steps)
     value = min_ess / elapsed
 
@@ -330,17 +344,25 @@ def run_dense(args):
     peak = PEAK["bf16"] if args.precision == "bf16" else PEAK[args.precision]
     # HBM bytes per launch of the dominant kernel from the committed counter
     # profile (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel); only for
-    # the configuration it was measured on
+    # the configuration and GEMM options it was measured with
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_dense_gemm.json")
     if os.path.exists(tfile) and args.precision == "bf16" and n == 20000 and d == 8192 and B == 512:
         try:
-            traffic = json.load(open(tfile))["bytes_per_launch"].get(gemm_key)
+            tj = json.load(open(tfile))
+            ran = {"gemm_config": args.gemm_config, "gemm_config_grad": args.gemm_config_grad,
+                   "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
+                   "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
+                   "prior": args.prior}
+            if all(tj.get("options", {}).get(k, object()) == v for k, v in ran.items()):
+                traffic = tj["bytes_per_launch"].get(gemm_key)
         except Exception:
             traffic = None
 
     cfg = {
-        "workload": f"dense linear-Gaussian {n}x{d}, G stored bf16, {B} chains, L={L}: surrogate+batched "
+        "prior": "Laplace(0, 0.3); errors are against the plain fp64 leg" if args.prior == "laplace"
+                 else "Normal(0, 1); errors are against the analytic fp64 posterior",
+        "workload": f"dense linear {n}x{d} ({args.prior} prior), G stored bf16, {B} chains, L={L}: surrogate+batched "
                     f"({args.precision} trajectory GEMMs, fp64 endpoint Metropolis, MLP surrogate with "
                     f"fallback) vs plain batched fp64 GPU HMC",
         "chains": B, "n_data": n, "n_params": d, "leapfrog_steps": L, "stepsize": eps,
@@ -357,6 +379,8 @@ def run_dense(args):
                         fallback_threshold=args.sur_fallback)
         cfg["surrogate"] = sur_info
     if plain is not None:
+        plain.pop("_mean", None)
+        plain.pop("_std", None)
         cfg["plain_fp64"] = plain
         cfg["speedup_vs_plain_fp64"] = value / plain["value"]
     result = {
@@ -662,6 +686,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small", "uneven", "copy"])
     ap.add_argument("--problems", type=int, default=256)
+    ap.add_argument("--prior", default="normal", choices=["normal", "laplace"],
+                    help="dense workload prior: Normal(0,1) (analytic posterior) or Laplace(0,0.3)")
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=20000)

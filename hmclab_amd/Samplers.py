@@ -355,27 +355,31 @@ class HMC:
                 chunk = self._chunk or max(self.online_thinning, min(self.proposals, 64))
                 chunk = max(self.online_thinning, (chunk // self.online_thinning) * self.online_thinning)
                 self._chains.set_trace(chunk // self.online_thinning + 1)
-                while done < self.proposals:
-                    n = min(chunk, self.proposals - done)
-                    self._chains.reset_trace()
-                    self._step, acc = self._chains.run(
-                        n, self.amount_of_steps, self._step, self.integrator, self.randomize_stepsize,
-                        self.autotuning, self.target_acceptance_rate, self.learning_rate,
-                        self._device_seed, done, self.online_thinning)
-                    self.accepted_proposals = self.accepted_proposals + (acc if B > 1 else int(acc[0]))
-                    self._surrogate_check(float(np.sum(acc)) / (n * B), done + n)
-                    cnt = self._chains.trace_count()
-                    if cnt:
-                        X, chi = self._chains.read_trace(0, cnt)
-                        for k in range(cnt):
-                            self._write(X[k], chi[k])
-                    done += n
-                    self.current_proposal = done - 1
-                    if bar is not None:
-                        bar.update(n)
-                        bar.set_description(f"Tot. acc rate: {self._rate():.2f}. Progress", refresh=False)
-                    if deadline is not None and time.time() > deadline:
-                        raise TimeoutError
+                # samples of chunk k are written to the files by a writer thread
+                # while the device runs chunk k+1 (ctypes calls release the GIL)
+                writer = _ChunkWriter(self._write)
+                try:
+                    while done < self.proposals:
+                        n = min(chunk, self.proposals - done)
+                        self._chains.reset_trace()
+                        self._step, acc = self._chains.run(
+                            n, self.amount_of_steps, self._step, self.integrator, self.randomize_stepsize,
+                            self.autotuning, self.target_acceptance_rate, self.learning_rate,
+                            self._device_seed, done, self.online_thinning)
+                        self.accepted_proposals = self.accepted_proposals + (acc if B > 1 else int(acc[0]))
+                        self._surrogate_check(float(np.sum(acc)) / (n * B), done + n)
+                        cnt = self._chains.trace_count()
+                        if cnt:
+                            writer.put(self._chains.read_trace(0, cnt))
+                        done += n
+                        self.current_proposal = done - 1
+                        if bar is not None:
+                            bar.update(n)
+                            bar.set_description(f"Tot. acc rate: {self._rate():.2f}. Progress", refresh=False)
+                        if deadline is not None and time.time() > deadline:
+                            raise TimeoutError
+                finally:
+                    writer.close()
         except KeyboardInterrupt:
             # Samplers.py:688-690: the interrupted proposal is dropped
             self.current_proposal = max(done - 1, 0)
@@ -480,6 +484,47 @@ class HMC:
             return self.results[0][:, burn_in:]
         with Samples(self._filenames[0], burn_in=burn_in) as s:
             return np.array(s.numpy)
+
+
+class _ChunkWriter:
+    """Single background thread appending trace chunks (X (k, B, d), chi (k, B))
+    to the samples files in order; close() drains the queue and re-raises a
+    writer error in the caller."""
+
+    def __init__(self, write):
+        import queue
+        import threading
+
+        self._write = write
+        self._q = queue.Queue(maxsize=4)
+        self._err = None
+        self._t = threading.Thread(target=self._run, daemon=True)
+        self._t.start()
+
+    def _run(self):
+        while True:
+            item = self._q.get()
+            if item is None:
+                return
+            if self._err is not None:
+                continue
+            try:
+                X, chi = item
+                for k in range(X.shape[0]):
+                    self._write(X[k], chi[k])
+            except BaseException as e:   # reported by close()
+                self._err = e
+
+    def put(self, item):
+        if self._err is not None:
+            raise self._err
+        self._q.put(item)
+
+    def close(self):
+        self._q.put(None)
+        self._t.join()
+        if self._err is not None:
+            raise self._err
 
 
 class ParallelSampleSMP:

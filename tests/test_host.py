@@ -139,3 +139,24 @@ def test_ess_rejects_too_short_chains():
 
     with pytest.raises(ValueError, match="at least 4 draws"):
         diagnostics.ess_bulk(np.zeros((3, 3, 2)))
+
+
+def test_chunk_writer_keeps_order_and_reports_errors():
+    import pytest
+
+    from hmclab_amd.Samplers import _ChunkWriter
+
+    seen = []
+    w = _ChunkWriter(lambda X, chi: seen.append((X.copy(), chi.copy())))
+    for c in range(5):
+        w.put((np.full((3, 2, 4), c, dtype=float), np.full((3, 2), c, dtype=float)))
+    w.close()
+    assert [int(x[0, 0]) for x, _ in seen] == [0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4]
+
+    def boom(X, chi):
+        raise OSError("disk full")
+
+    w = _ChunkWriter(boom)
+    w.put((np.zeros((1, 1, 1)), np.zeros((1, 1))))
+    with pytest.raises(OSError, match="disk full"):
+        w.close()
