@@ -282,6 +282,8 @@ def run_dense(args):
             opts["incremental_residual"] = args.incremental_residual
         if args.tile_order:
             opts["tile_order"] = args.tile_order
+        if not args.split_tail:
+            opts["gemm_split_tail"] = 0
     prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)
@@ -353,7 +355,7 @@ def run_dense(args):
             ran = {"gemm_config": args.gemm_config, "gemm_config_grad": args.gemm_config_grad,
                    "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
                    "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
-                   "prior": args.prior}
+                   "split_tail": args.split_tail, "prior": args.prior}
             if all(tj.get("options", {}).get(k, object()) == v for k, v in ran.items()):
                 traffic = tj["bytes_per_launch"].get(gemm_key)
         except Exception:
@@ -372,7 +374,8 @@ def run_dense(args):
         "trajectory_gradient_rel_err_vs_fp64": grad_err,
         "gemm": {"config": args.gemm_config, "config_grad": args.gemm_config_grad,
                  "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
-                 "incremental_residual": args.incremental_residual, "tile_order": args.tile_order},
+                 "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
+                 "split_tail": args.split_tail},
     }
     if sur_info is not None:
         sur_info.update(chunk_acceptance=sur_acc, fallback_after_proposals=fallback_at,
@@ -704,11 +707,13 @@ def main():
     ap.add_argument("--bf16-split-grad", type=int, default=1, choices=[1, 2],
                     help="bf16 planes of the scaled residual in the gradient GEMM (1: one plane, "
                          "measured to leave acceptance unchanged; 2: hi/lo)")
-    ap.add_argument("--gemm-config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--gemm-config", type=int, default=3, choices=[0, 1, 2, 3])
+    ap.add_argument("--split-tail", type=int, default=1, choices=[0, 1],
+                    help="bf16 GEMMs: split the last partial round of tiles along K")
     ap.add_argument("--tile-order", type=int, default=0, choices=[0, 1])
     ap.add_argument("--incremental-residual", type=int, default=1, choices=[0, 1],
                     help="bf16: advance a float32 residual by G dx after the first stage")
-    ap.add_argument("--gemm-config-grad", type=int, default=None, choices=[0, 1, 2],
+    ap.add_argument("--gemm-config-grad", type=int, default=None, choices=[0, 1, 2, 3],
                     help="tile configuration of the gradient GEMM (default: --gemm-config)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-proposals", type=int, default=2)

@@ -279,6 +279,10 @@ int hmc_problem_create(int64_t dimensions, int precision, hmc_problem** out) {
     P.tgt_dtype = P.state_dtype;
     try {
       HMC_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+      int dev = 0, cus = 0;
+      HMC_HIP(hipGetDevice(&dev));
+      HMC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      P.n_cu = cus;
     } catch (...) {
       delete h;
       throw;
@@ -463,13 +467,28 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "incremental_residual must be 0 or 1");
       P.resid_update = (int)value;
     } else if (key == "gemm_config") {
-      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0, HMC_ERR_ARG, "gemm_config must be 0, 1 or 2");
+      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0, HMC_ERR_ARG,
+                  "gemm_config must be 0, 1, 2 or 3");
       P.gemm_cfg = (int)value;
       P.gemm_cfg_grad = (int)value;
     } else if (key == "gemm_config_grad") {
-      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0, HMC_ERR_ARG,
-                  "gemm_config_grad must be 0, 1 or 2");
+      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0, HMC_ERR_ARG,
+                  "gemm_config_grad must be 0, 1, 2 or 3");
       P.gemm_cfg_grad = (int)value;
+    } else if (key == "gemm_split_tail") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_split_tail must be 0 or 1");
+      P.split_tail = (int)value;
+    } else if (key == "gemm_split_cus") {
+      HMC_REQUIRE(value >= 0.0 && value <= 65536.0 && value == (double)(int)value, HMC_ERR_ARG,
+                  "gemm_split_cus must be a non-negative integer");
+      if (value > 0.0) {
+        P.n_cu = (int)value;
+      } else {
+        int dev = 0, cus = 0;
+        HMC_HIP(hipGetDevice(&dev));
+        HMC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        P.n_cu = cus;
+      }
     } else {
       throw Error{HMC_ERR_ARG, "unknown option " + key};
     }
@@ -527,6 +546,8 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
           c.xl = (uint16_t*)dev_alloc(c.owned, rows * 2);
           c.rh = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
           if (P.resid_update) c.rf = (float*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 4);
+          c.split_ws_elems = gemm_split_workspace(P, c.B);
+          if (c.split_ws_elems) c.split_ws = (float*)dev_alloc(c.owned, c.split_ws_elems * 4);
           c.rl = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
         } else if (P.resolved_engine() == HMC_ENGINE_GEMM) {
           c.r = dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * es);

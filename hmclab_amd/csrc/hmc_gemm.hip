@@ -69,6 +69,67 @@ __device__ inline bool map_tile(int gm, int gn, int& tm, int& tn, int order = 0)
 }
 static inline int grid_for(int gm, int gn) { return ((gm * gn + 7) / 8) * 8; }
 
+// Split tail.  T tiles on n_cu compute units leave a last round of T mod n_cu
+// tiles that runs on a fraction of the chip.  The first F tiles (whole rounds,
+// F a multiple of 8 and of gn) run over all of K; each of the T - F tail tiles
+// is cut into SPL contiguous k-ranges, one workgroup each, that store float32
+// partial tiles; k_split_epilogue sums the SPL partials of every tail element
+// in fixed order and applies the epilogue.  Workgroup u < F takes full tile
+// (u % 8) * (F / 8) + u / 8 (XCD-contiguous, as map_tile); the tail units are
+// dealt the same way, the gn chain tiles of one (row block, k-range) adjacent.
+struct SplitPlan {
+  int F = 0, SPL = 1, grid = 0;
+  int64_t TR = 0;   // rows of a partial plane (tail row blocks x BM)
+  int64_t mt0 = 0;  // first tail row
+  bool split() const { return SPL > 1; }
+};
+
+static SplitPlan plan_split(int BM, int BN, int M, int N, int K, int n_cu, bool allow) {
+  SplitPlan s;
+  const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN, T = gm * gn;
+  const int nk = (K + 63) / 64;
+  s.F = T;
+  s.grid = grid_for(gm, gn);
+  if (!allow || n_cu <= 0) return s;
+  int L = 8;
+  while (L % gn) L += 8;                       // lcm(8, gn)
+  const int F = ((T / n_cu) * n_cu / L) * L;
+  const int tail = T - F;
+  if (tail <= 0) return s;
+  int spl = n_cu / tail;
+  spl = spl > 8 ? 8 : spl;
+  spl = spl > nk / 8 ? nk / 8 : spl;           // at least 8 k-tiles per piece
+  if (spl < 2) return s;
+  s.F = F;
+  s.SPL = spl;
+  s.mt0 = (int64_t)(F / gn) * BM;
+  s.TR = (int64_t)(gm - F / gn) * BM;
+  s.grid = F + ((tail * spl + 7) / 8) * 8;
+  return s;
+}
+
+__device__ inline bool split_job(int gm, int gn, int nk, int F, int SPL, int& tm, int& tn, int& kt0, int& kt1,
+                                 int& sp) {
+  const int T = gm * gn;
+  const int u = blockIdx.x;
+  if (u < F) {
+    const int t = (u % 8) * (F / 8) + u / 8;
+    tm = t / gn;
+    tn = t % gn;
+    return true;
+  }
+  const int units = (T - F) * SPL, chunk = (units + 7) / 8, v = u - F;
+  const int t2 = (v % 8) * chunk + v / 8;
+  if (t2 >= units) return false;
+  const int g = t2 / gn;
+  tn = t2 % gn;
+  tm = F / gn + g / SPL;
+  sp = g % SPL;
+  kt0 = sp * nk / SPL;
+  kt1 = (sp + 1) * nk / SPL;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 MFMA GEMM (v_mfma_f64_16x16x4f64): C = A * Bt^T, block tile BM x BN,
 // k-tile 16, wave tile WM x WN.  K is permuted inside a k-tile: lane group
@@ -238,7 +299,8 @@ __device__ inline void lds_barrier() {
 template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
     const uint16_t* __restrict__ A, int64_t lda, const uint16_t* __restrict__ B0,
-    const uint16_t* __restrict__ B1, int64_t ldb, int M, int N, int K, int order, Epi epi) {
+    const uint16_t* __restrict__ B1, int64_t ldb, int M, int N, int K, int order, int F, int SPL,
+    float* __restrict__ part, int64_t TR, Epi epi) {
   constexpr int BK = 64;                 // elements per row per k-tile (128 bytes)
   constexpr int NW = (BM / WM) * (BN / WN);
   constexpr int TM = WM / 32, TN = WN / 32;
@@ -255,9 +317,17 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
   static_assert(SMEM * 2 <= 160 * 1024, "LDS budget");
   __shared__ __attribute__((aligned(1024))) uint16_t smem[SMEM];
 
-  int tm, tn;
-  if (!map_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn, order)) return;
+  const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN;
+  int tm, tn, kt0 = 0, kt1 = (K + BK - 1) / BK, sp = -1;
+  if (F >= gm * gn) {
+    if (!map_tile(gm, gn, tm, tn, order)) return;
+  } else if (!split_job(gm, gn, kt1, F, SPL, tm, tn, kt0, kt1, sp)) {
+    return;
+  }
   const int64_t m_blk = (int64_t)tm * BM, n_blk = (int64_t)tn * BN;
+  A += (int64_t)kt0 * BK;
+  B0 += (int64_t)kt0 * BK;
+  if (NSPLIT == 2) B1 += (int64_t)kt0 * BK;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w / WPR, wn = w % WPR;
@@ -298,7 +368,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-  const int nk = (K + BK - 1) / BK;
+  const int nk = kt1 - kt0;
 #pragma unroll
   for (int s0 = 0; s0 < S - 1; ++s0)
     if (s0 < nk) issue(s0, s0);
@@ -363,11 +433,45 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
             floatx4{acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]};
       }
   __syncthreads();
-  for (int idx = tid; idx < BN * (BM / 4); idx += NW * 64) {
-    const int nl = idx / (BM / 4), ml = 4 * (idx % (BM / 4));
-    const floatx4 c = *reinterpret_cast<const floatx4*>(&Cs[nl * CLD + ml]);
-    const float v[4] = {c[0], c[1], c[2], c[3]};
-    epi.apply(m_blk + ml, 1, n_blk + nl, v);
+  if (sp < 0) {
+    for (int idx = tid; idx < BN * (BM / 4); idx += NW * 64) {
+      const int nl = idx / (BM / 4), ml = 4 * (idx % (BM / 4));
+      const floatx4 c = *reinterpret_cast<const floatx4*>(&Cs[nl * CLD + ml]);
+      const float v[4] = {c[0], c[1], c[2], c[3]};
+      epi.apply(m_blk + ml, 1, n_blk + nl, v);
+    }
+  } else {
+    // partial tile of k-range sp: part[sp][chain][row - first tail row]
+    const int64_t mt0 = (int64_t)(F / gn) * BM;
+    for (int idx = tid; idx < BN * (BM / 4); idx += NW * 64) {
+      const int nl = idx / (BM / 4), ml = 4 * (idx % (BM / 4));
+      const int64_t n = n_blk + nl;
+      if (n >= N) continue;
+      *reinterpret_cast<floatx4*>(&part[((int64_t)sp * N + n) * TR + (m_blk - mt0) + ml]) =
+          *reinterpret_cast<const floatx4*>(&Cs[nl * CLD + ml]);
+    }
+  }
+}
+
+// sum of the SPL partial planes of the tail rows (fixed order), then the
+// epilogue on 4 consecutive rows of one chain
+template <class Epi>
+__global__ __launch_bounds__(256) void k_split_epilogue(const float* __restrict__ part, int SPL, int64_t TR, int N,
+                                                        int64_t mt0, Epi epi) {
+  const int64_t per_chain = TR / 4, total = (int64_t)N * per_chain;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int n = (int)(i / per_chain);
+    const int64_t ml = 4 * (i % per_chain);
+    floatx4 acc = *reinterpret_cast<const floatx4*>(&part[(int64_t)n * TR + ml]);
+    for (int k = 1; k < SPL; ++k) {
+      const floatx4 c = *reinterpret_cast<const floatx4*>(&part[((int64_t)k * N + n) * TR + ml]);
+      acc[0] += c[0];
+      acc[1] += c[1];
+      acc[2] += c[2];
+      acc[3] += c[3];
+    }
+    const float v[4] = {acc[0], acc[1], acc[2], acc[3]};
+    epi.apply(mt0 + ml, 1, n, v);
   }
 }
 
@@ -546,38 +650,87 @@ static void launch_f64(hipStream_t s, const TA* A, int64_t lda, const TB* Bt, in
   HMC_HIP(hipGetLastError());
 }
 
+// Workspace for the split tail: float32 partial planes, or null (no split).
+struct SplitWs {
+  float* ptr = nullptr;
+  size_t elems = 0;
+  int n_cu = 0;
+};
+
 template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
 static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const uint16_t* B0,
-                            const uint16_t* B1, int64_t ldb, int M, int N, int K, int order, const Epi& epi) {
-  const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN;
+                            const uint16_t* B1, int64_t ldb, int M, int N, int K, int order, const SplitWs& ws,
+                            const Epi& epi) {
   constexpr int NT = (BM / WM) * (BN / WN) * 64;
-  hipLaunchKernelGGL((k_gemm_bf16<BM, BN, WM, WN, NSPLIT, S, Epi>), dim3(grid_for(gm, gn)), dim3(NT), 0, s, A,
-                     lda, B0, B1, ldb, M, N, K, order, epi);
+  SplitPlan pl = plan_split(BM, BN, M, N, K, ws.n_cu, ws.ptr != nullptr);
+  if (pl.split() && (size_t)pl.SPL * N * pl.TR > ws.elems) pl = plan_split(BM, BN, M, N, K, 0, false);
+  hipLaunchKernelGGL((k_gemm_bf16<BM, BN, WM, WN, NSPLIT, S, Epi>), dim3(pl.grid), dim3(NT), 0, s, A, lda, B0, B1,
+                     ldb, M, N, K, order, pl.F, pl.SPL, ws.ptr, pl.TR, epi);
   HMC_HIP(hipGetLastError());
+  if (pl.split()) {
+    const int64_t work = (int64_t)N * (pl.TR / 4);
+    const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_split_epilogue<Epi>), dim3(blocks), dim3(256), 0, s, ws.ptr, pl.SPL, pl.TR, N, pl.mt0,
+                       epi);
+    HMC_HIP(hipGetLastError());
+  }
 }
 
-// Tile configurations: 0 = 128x128 block, 4 waves of 64x64, 3-stage ring;
-// 1 = 128x128, 8 waves of 64x32 (two per SIMD), 3 stages;
-// 2 = 256x128, 4 waves of 128x64, 2 stages (half the LDS fragment traffic per
-//     MFMA of config 1, a quarter of the tiles).
+template <int BM, int BN>
+static size_t split_elems(int M, int N, int K, int n_cu) {
+  const SplitPlan pl = plan_split(BM, BN, M, N, K, n_cu, true);
+  return pl.split() ? (size_t)pl.SPL * N * pl.TR : 0;
+}
+
+// Tile configurations: 0 = 128x128 block, 4 waves of 64x64;
+// 1 = 128x128, 8 waves of 64x32 (two per SIMD);
+// 2 = 256x128, 4 waves of 128x64, 2 stages;
+// 3 = 256x128, 8 waves of 64x64, 3 stages (one-plane operands; two-plane
+//     operands use the config 1 tile): a third less G and chain traffic from
+//     L2 per MFMA than 128x128 tiles.
+// One-plane operands need 32 KB per stage of a 128x128 tile: a 4-stage ring
+// keeps three k-tiles in flight.
 template <int NSPLIT, class Epi>
 static void launch_bf16_split(int cfg, hipStream_t st, const uint16_t* A, int64_t lda, const uint16_t* B0,
-                              const uint16_t* B1, int64_t ldb, int M, int N, int K, int order, const Epi& epi) {
-  // one-plane operands need 32 KB per stage: a 4-stage ring keeps three k-tiles in flight
+                              const uint16_t* B1, int64_t ldb, int M, int N, int K, int order, const SplitWs& ws,
+                              const Epi& epi) {
   constexpr int S = NSPLIT == 1 ? 4 : BF_STAGES;
   if (cfg == 0)
-    launch_bf16_cfg<BF_BM, BF_BN, 64, 64, NSPLIT, S>(st, A, lda, B0, B1, ldb, M, N, K, order, epi);
-  else if (cfg == 1)
-    launch_bf16_cfg<BF_BM, BF_BN, 64, 32, NSPLIT, S>(st, A, lda, B0, B1, ldb, M, N, K, order, epi);
+    launch_bf16_cfg<BF_BM, BF_BN, 64, 64, NSPLIT, S>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
+  else if (cfg == 2)
+    launch_bf16_cfg<256, 128, 128, 64, NSPLIT, 2>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
+  else if (cfg == 3 && NSPLIT == 1)
+    launch_bf16_cfg<256, 128, 64, 64, 1, 3>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
   else
-    launch_bf16_cfg<256, 128, 128, 64, NSPLIT, 2>(st, A, lda, B0, B1, ldb, M, N, K, order, epi);
+    launch_bf16_cfg<BF_BM, BF_BN, 64, 32, NSPLIT, S>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
+}
+
+// partial-plane elements a launch of (cfg, planes) may need
+static size_t split_elems_cfg(int cfg, int nsplit, int M, int N, int K, int n_cu) {
+  if (cfg == 2) return split_elems<256, 128>(M, N, K, n_cu);
+  if (cfg == 3 && nsplit == 1) return split_elems<256, 128>(M, N, K, n_cu);
+  return split_elems<BF_BM, BF_BN>(M, N, K, n_cu);
 }
 
 template <class Epi>
-static void launch_bf16(const Problem& p, int cfg, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
+static void launch_bf16(const Chains& c, int cfg, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
                         const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
-  if (split == 2) launch_bf16_split<2>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, epi);
-  else launch_bf16_split<1>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, epi);
+  const Problem& p = *c.p;
+  SplitWs ws;
+  if (p.split_tail && c.split_ws) ws = SplitWs{c.split_ws, c.split_ws_elems, p.n_cu};
+  if (split == 2) launch_bf16_split<2>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, ws, epi);
+  else launch_bf16_split<1>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, ws, epi);
+}
+
+size_t gemm_split_workspace(const Problem& p, int64_t B) {
+  if (p.precision != HMC_BF16 || p.op != 1 || !p.split_tail) return 0;
+  const int n = (int)p.n, d = (int)p.d;
+  size_t e = 0;
+  for (int planes = 1; planes <= 2; ++planes) {
+    e = std::max(e, split_elems_cfg(p.gemm_cfg, planes, n, (int)B, (int)p.ldg, p.n_cu));
+    e = std::max(e, split_elems_cfg(p.gemm_cfg_grad, planes, d, (int)B, (int)p.ldgt, p.n_cu));
+  }
+  return e;
 }
 
 int64_t gemm_row_pad() { return 256; }   // largest M-tile of the GEMM configs
@@ -597,18 +750,18 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
       // first stage: full product G x (hi/lo planes of x); later stages: the
       // float32 residual advances by G dx with one plane of dx
       EpiResidRaw e1{c.rf, c.rh, c.rl, c.ldr, p.dobs, p.var, first ? 0 : 1, n, B};
-      launch_bf16(p, p.gemm_cfg, first ? p.bf16_split : 1, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B,
+      launch_bf16(c, p.gemm_cfg, first ? p.bf16_split : 1, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B,
                   (int)p.ldg, e1);
     } else {
       EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
-      launch_bf16(p, p.gemm_cfg, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg,
+      launch_bf16(c, p.gemm_cfg, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg,
                   e1);
     }
     time_end(c, 0, ev1);
     EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, c.xl, c.ldx, c.eps, c.beta, kick, drift, drift2, md, d, B,
                       inc ? 1 : 0};
     time_begin(c, 3, ev2);
-    launch_bf16(p, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_bf16(c, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
@@ -660,9 +813,9 @@ void gemm_gradient(Chains& c, const void* X, double* out) {
   if (p.precision == HMC_BF16) {
     // hi/lo planes of X were prepared by the caller into c.xh / c.xl
     EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
-    launch_bf16(p, p.gemm_cfg, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
+    launch_bf16(c, p.gemm_cfg, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg, e1);
     EpiGrad<float> e2{(const float*)X, out, c.ldx, c.oob, md, d, B};
-    launch_bf16(p, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_bf16(c, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
     launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e1);

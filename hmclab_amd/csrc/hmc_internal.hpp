@@ -36,6 +36,8 @@ struct Problem {
   int tile_order = 0;       // bf16 GEMM tiles per XCD: 0 m-major, 1 n-major
   int gemm_cfg = 1;         // bf16 GEMM tile configuration of the residual GEMM
   int gemm_cfg_grad = 1;    // ... of the gradient GEMM
+  int split_tail = 1;       // bf16 GEMMs: split the last partial round of tiles along K
+  int n_cu = 0;             // compute units of the device (split-tail planning)
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 
   // dense (row-major, zero padded): G [n_pad][ldg], Gt [d_pad][ldgt]
@@ -96,6 +98,8 @@ struct Chains {
   void* r = nullptr;        // residual [B_pad][ldr] (state dtype, F64/F32 GEMM)
   float* rf = nullptr;      // float32 running residual G x - d (bf16 GEMM, incremental mode)
   double* partial = nullptr;// endpoint partial sums [mblocks][B_pad]
+  float* split_ws = nullptr;// bf16 GEMM split-tail partial planes
+  size_t split_ws_elems = 0;
   int64_t n_mblocks = 0;
   double* k0 = nullptr;     // kinetic energy at the start of the trajectory
   double* eps = nullptr;    // local step per chain
@@ -183,6 +187,7 @@ void time_collect(Chains& c);
 // ---- GEMM engine (hmc_gemm.hip) -------------------------------------------
 // Row padding of G and G^T: every GEMM configuration's M-tile divides it.
 int64_t gemm_row_pad();
+size_t gemm_split_workspace(const Problem& p, int64_t B);  // float elements
 // Trajectory gradient + fused kick/drift for every chain:
 //   p -= kick*eps*g(xt); xt += drift*eps*Minv p; corrector; [xt += drift2*eps*Minv p; corrector]
 void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);

@@ -125,10 +125,17 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 from hi/lo planes of x, later stages add G dx with one bf16
  *                 plane of the position change dx (half the MFMA work of the
  *                 residual GEMM; default 0)
- *   "gemm_config" 0, 1 or 2: bf16 GEMM tile of both trajectory GEMMs
+ *   "gemm_config" 0-3: bf16 GEMM tile of both trajectory GEMMs
  *                 (0 = 128x128, 4 waves of 64x64; 1 = 128x128, 8 waves of
- *                 64x32; 2 = 256x128, 4 waves of 128x64; default 1)
- *   "gemm_config_grad" 0, 1 or 2: the same for the gradient GEMM only
+ *                 64x32; 2 = 256x128, 4 waves of 128x64; 3 = 256x128, 8 waves
+ *                 of 64x64 for one-plane operands, config 1 for two; default 1)
+ *   "gemm_config_grad" 0-3: the same for the gradient GEMM only
+ *   "gemm_split_tail" 0 or 1: when the tiles of a bf16 GEMM leave a last
+ *                 partial round on the compute units, split those tiles along
+ *                 K into float32 partials summed in fixed order (default 1)
+ *   "gemm_split_cus" n: compute units the split-tail plan assumes (0 = the
+ *                 device's count, the default; a smaller n plans for a share
+ *                 of the device)
  *   "tile_order"  0 or 1: bf16 GEMM tiles dealt to the 8 XCDs in m-major
  *                 (tiles sharing G rows share an L2; default) or n-major
  *                 (tiles sharing chains share an L2) ranges

@@ -90,7 +90,7 @@ def test_sparse_misfit_gradient(precision):
         assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
 @pytest.mark.parametrize("n,d,B", [(520, 384, 3), (300, 700, 130)])
 def test_bf16_gemm_gradient_and_fp64_misfit(cfg, n, d, B):
     rng = np.random.default_rng(3)
@@ -140,6 +140,47 @@ def test_bf16_trajectory_close_to_fp64(incremental, split_grad):
         chi1 = orc.misfit(res["X1"][c].reshape(d, 1))
         k1 = 0.5 * float(np.sum(res["P1"][c] ** 2))
         assert abs(res["H1"][c] - (chi1 + k1)) <= 1e-6 * abs(chi1 + k1), c
+
+
+@pytest.mark.parametrize("cfg", [1, 3])
+def test_bf16_split_tail_matches_unsplit(cfg):
+    """Planning for 16 compute units makes both GEMMs of a 2500 x 2200 problem
+    with 256 chains leave a partial last round (residual 40 tiles of 128 rows
+    or 20 of 256, gradient 36 or 18), so the tail tiles run as float32
+    k-range partials summed by the split epilogue.  The trajectory equals the
+    unsplit one up to float32 summation order (1e-5 of the largest
+    coordinate after 5 leapfrog steps), its end-point Hamiltonians are FP64
+    of those end points, and gradients stay within the bf16 tolerance of the
+    oracle."""
+    rng = np.random.default_rng(21)
+    n, d, B, L = 2500, 2200, 256, 5
+    G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
+    dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, dtype="bfloat16")])
+    orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
+                       O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
+    X0 = (rng.normal(size=(B, d)) * 0.3).astype(np.float32).astype(np.float64)
+    P0 = rng.normal(size=(B, d)).astype(np.float32).astype(np.float64)
+    e = 0.002 * rng.uniform(0.5, 1.5, size=B)
+    u = rng.uniform(size=B)
+    res, grad = {}, {}
+    for name, opts in (("split", {"gemm_split_cus": 16}), ("plain", {"gemm_split_tail": 0})):
+        prob = build_problem(post, precision="bf16",
+                             options={"gemm_config": cfg, "incremental_residual": 1, **opts})
+        ch = Chains(prob, B)
+        ch.set_state(X0)
+        res[name] = ch.propose(P0, e, u, L, "lf", trajectory=True)
+        grad[name] = prob.gradient_batch(X0[:4])
+    assert _rel(res["split"]["X1"], res["plain"]["X1"]) < 1e-5
+    assert _rel(res["split"]["P1"], res["plain"]["P1"]) < 1e-5
+    for c in range(0, B, 37):
+        chi1 = orc.misfit(res["split"]["X1"][c].reshape(d, 1))
+        k1 = 0.5 * float(np.sum(res["split"]["P1"][c] ** 2))
+        assert abs(res["split"]["H1"][c] - (chi1 + k1)) <= 1e-6 * abs(chi1 + k1), c
+    for b in range(4):
+        ref = orc.gradient(X0[b].reshape(d, 1))[:, 0]
+        assert _rel(grad["split"][b], ref) < 2e-4
+        assert _rel(grad["split"][b], grad["plain"][b]) < 1e-5
 
 
 def test_bf16_single_plane_gradient_gemm_tolerance():
