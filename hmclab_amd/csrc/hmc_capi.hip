@@ -541,6 +541,8 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
       if (P.op == 1) {
         c.n_mblocks = P.n_pad / 128;
         c.partial = (double*)dev_alloc(c.owned, (size_t)c.n_mblocks * c.B_pad * 8);
+        c.split_ws64_elems = gemm_split_workspace_f64(P, c.B);
+        if (c.split_ws64_elems) c.split_ws64 = (double*)dev_alloc(c.owned, c.split_ws64_elems * 8);
         if (P.precision == HMC_BF16) {
           c.xh = (uint16_t*)dev_alloc(c.owned, rows * 2);
           c.xl = (uint16_t*)dev_alloc(c.owned, rows * 2);

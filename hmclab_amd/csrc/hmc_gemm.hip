@@ -84,10 +84,10 @@ struct SplitPlan {
   bool split() const { return SPL > 1; }
 };
 
-static SplitPlan plan_split(int BM, int BN, int M, int N, int K, int n_cu, bool allow) {
+static SplitPlan plan_split(int BM, int BN, int M, int N, int K, int n_cu, bool allow, int BK = 64) {
   SplitPlan s;
   const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN, T = gm * gn;
-  const int nk = (K + 63) / 64;
+  const int nk = (K + BK - 1) / BK * BK / 64;   // pieces of at least 8 x 64 k
   s.F = T;
   s.grid = grid_for(gm, gn);
   if (!allow || n_cu <= 0) return s;
@@ -139,7 +139,7 @@ __device__ inline bool split_job(int gm, int gn, int nk, int F, int SPL, int& tm
 template <int BM, int BN, int WM, int WN, typename TA, typename TB, class Epi>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
     const TA* __restrict__ A, int64_t lda, const TB* __restrict__ Bt, int64_t ldb, int M, int N,
-    int K, Epi epi) {
+    int K, int F, int SPL, double* __restrict__ part, int64_t TR, Epi epi) {
   constexpr int BK = 16, LDS = BK + 2;
   constexpr int NT = (BM / WM) * (BN / WN) * 64;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -147,9 +147,16 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
   __shared__ double As[BM * LDS];
   __shared__ double Bs[BN * LDS];
 
-  int tm, tn;
-  if (!map_tile((M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn)) return;
+  const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN;
+  int tm, tn, kt0 = 0, kt1 = (K + BK - 1) / BK, sp = -1;
+  if (F >= gm * gn) {
+    if (!map_tile(gm, gn, tm, tn)) return;
+  } else if (!split_job(gm, gn, kt1, F, SPL, tm, tn, kt0, kt1, sp)) {
+    return;
+  }
   const int64_t m_blk = (int64_t)tm * BM, n_blk = (int64_t)tn * BN;
+  A += (int64_t)kt0 * BK;
+  Bt += (int64_t)kt0 * BK;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w / WPR, wn = w % WPR;
@@ -194,7 +201,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
 
-  const int nk = (K + BK - 1) / BK;
+  const int nk = kt1 - kt0;
   gload(0);
   for (int kt = 0; kt < nk; ++kt) {
     __syncthreads();
@@ -225,6 +232,22 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
 
   // D layout (measured on gfx950): lane (q, r) holds rows q, q+4, q+8, q+12 of
   // column r of each 16x16 tile
+  if (sp >= 0) {
+    // partial tile of k-range sp: part[sp][chain][row - first tail row]
+    const int64_t mt0 = (int64_t)(F / gn) * BM;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int64_t m0 = m_blk + wm * WM + i * 16 + q;
+        const int64_t n = n_blk + wn * WN + j * 16 + r;
+        if (n >= N) continue;
+        double* dst = part + ((int64_t)sp * N + n) * TR + (m0 - mt0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dst[4 * v] = acc[i][j][v];
+      }
+    return;
+  }
   if constexpr (Epi::kReduce) {
     // per-column sums over the block's rows, deterministic order
     __shared__ double red[(BM / WM)][4][BN];
@@ -261,6 +284,48 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f64(
         epi.apply(m0, 4, n, v);
       }
   }
+}
+
+// Split-tail epilogues of the FP64 GEMM.  Element-wise epilogues: sum the
+// SPL partials of 4 consecutive rows of one chain in fixed order, then apply.
+template <class Epi>
+__global__ __launch_bounds__(256) void k_split_epilogue_f64(const double* __restrict__ part, int SPL, int64_t TR,
+                                                            int N, int64_t mt0, Epi epi) {
+  const int64_t per_chain = TR / 4, total = (int64_t)N * per_chain;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int n = (int)(i / per_chain);
+    const int64_t ml = 4 * (i % per_chain);
+    double v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = part[(int64_t)n * TR + ml + e];
+    for (int k = 1; k < SPL; ++k)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] += part[((int64_t)k * N + n) * TR + ml + e];
+    epi.apply(mt0 + ml, 1, n, v);
+  }
+}
+
+// Reducing epilogue (endpoint misfit): one wavefront per (128-row block,
+// chain) sums the SPL partials of each row, applies term() and reduces the
+// block's rows with a fixed butterfly.
+template <class Epi>
+__global__ __launch_bounds__(256) void k_split_reduce_f64(const double* __restrict__ part, int SPL, int64_t TR,
+                                                          int N, int64_t mt0, Epi epi) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int rb = blockIdx.y;
+  if (n >= N) return;
+  double s = 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t row = (int64_t)rb * 128 + lane + 64 * h;
+    double v = part[(int64_t)n * TR + row];
+    for (int k = 1; k < SPL; ++k) v += part[((int64_t)k * N + n) * TR + row];
+    s += epi.term(mt0 + row, n, v);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) epi.store_partial((int)(mt0 / 128) + rb, n, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -641,13 +706,34 @@ constexpr int F64_BM = 128, F64_BN = 128, F64_WM = 64, F64_WN = 64;
 constexpr int BF_BM = 128, BF_BN = 128, BF_STAGES = 3;
 
 template <typename TA, typename TB, class Epi>
-static void launch_f64(hipStream_t s, const TA* A, int64_t lda, const TB* Bt, int64_t ldb, int M,
+static void launch_f64(const Chains& c, const TA* A, int64_t lda, const TB* Bt, int64_t ldb, int M,
                        int N, int K, const Epi& epi) {
-  const int gm = (M + F64_BM - 1) / F64_BM, gn = (N + F64_BN - 1) / F64_BN;
+  const Problem& p = *c.p;
+  const hipStream_t s = p.stream;
   constexpr int NT = (F64_BM / F64_WM) * (F64_BN / F64_WN) * 64;
-  hipLaunchKernelGGL((k_gemm_f64<F64_BM, F64_BN, F64_WM, F64_WN, TA, TB, Epi>), dim3(grid_for(gm, gn)),
-                     dim3(NT), 0, s, A, lda, Bt, ldb, M, N, K, epi);
+  const bool ws = p.split_tail && c.split_ws64;
+  SplitPlan pl = plan_split(F64_BM, F64_BN, M, N, K, p.n_cu, ws, 16);
+  if (pl.split() && (size_t)pl.SPL * N * pl.TR > c.split_ws64_elems) pl = plan_split(F64_BM, F64_BN, M, N, K, 0, false);
+  hipLaunchKernelGGL((k_gemm_f64<F64_BM, F64_BN, F64_WM, F64_WN, TA, TB, Epi>), dim3(pl.grid), dim3(NT), 0, s, A,
+                     lda, Bt, ldb, M, N, K, pl.F, pl.SPL, c.split_ws64, pl.TR, epi);
   HMC_HIP(hipGetLastError());
+  if (pl.split()) {
+    if constexpr (Epi::kReduce) {
+      hipLaunchKernelGGL((k_split_reduce_f64<Epi>), dim3((N + 3) / 4, (unsigned)(pl.TR / 128)), dim3(256), 0, s,
+                         c.split_ws64, pl.SPL, pl.TR, N, pl.mt0, epi);
+    } else {
+      const int64_t work = (int64_t)N * (pl.TR / 4);
+      const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+      hipLaunchKernelGGL((k_split_epilogue_f64<Epi>), dim3(blocks), dim3(256), 0, s, c.split_ws64, pl.SPL, pl.TR, N,
+                         pl.mt0, epi);
+    }
+    HMC_HIP(hipGetLastError());
+  }
+}
+
+static size_t split_elems_f64(int M, int N, int K, int n_cu) {
+  const SplitPlan pl = plan_split(F64_BM, F64_BN, M, N, K, n_cu, true, 16);
+  return pl.split() ? (size_t)pl.SPL * N * pl.TR : 0;
 }
 
 // Workspace for the split tail: float32 partial planes, or null (no split).
@@ -722,6 +808,14 @@ static void launch_bf16(const Chains& c, int cfg, int split, const uint16_t* A, 
   else launch_bf16_split<1>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, ws, epi);
 }
 
+size_t gemm_split_workspace_f64(const Problem& p, int64_t B) {
+  if (p.op != 1 || !p.split_tail) return 0;
+  const int n = (int)p.n, d = (int)p.d;
+  size_t e = split_elems_f64(n, (int)B, (int)p.ldg, p.n_cu);        // endpoint, FP64 residual
+  if (p.precision == HMC_F64) e = std::max(e, split_elems_f64(d, (int)B, (int)p.ldgt, p.n_cu));
+  return e;
+}
+
 size_t gemm_split_workspace(const Problem& p, int64_t B) {
   if (p.precision != HMC_BF16 || p.op != 1 || !p.split_tail) return 0;
   const int n = (int)p.n, d = (int)p.d;
@@ -766,12 +860,12 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
     time_begin(c, 0, ev1);
-    launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)c.xt, c.ldx, n, B, (int)p.ldg, e1);
+    launch_f64(c, (const double*)p.G, p.ldg, (const double*)c.xt, c.ldx, n, B, (int)p.ldg, e1);
     time_end(c, 0, ev1);
     EpiLeap<double> e2{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
                        md, d, B, 0};
     time_begin(c, 3, ev2);
-    launch_f64(p.stream, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_f64(c, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
   } else {
     throw Error{HMC_ERR_UNSUPPORTED, "GEMM engine supports float64 and bfloat16 trajectories"};
@@ -787,19 +881,19 @@ void gemm_endpoint(Chains& c, const void* X, int x_dtype) {
   time_begin(c, 1, ev);
   if (p.tgt_dtype == HMC_BF16) {
     if (x_dtype == HMC_F32)
-      launch_f64(p.stream, (const uint16_t*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
+      launch_f64(c, (const uint16_t*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
     else
-      launch_f64(p.stream, (const uint16_t*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
+      launch_f64(c, (const uint16_t*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
   } else if (p.tgt_dtype == HMC_F32) {
     if (x_dtype == HMC_F32)
-      launch_f64(p.stream, (const float*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
+      launch_f64(c, (const float*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
     else
-      launch_f64(p.stream, (const float*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
+      launch_f64(c, (const float*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
   } else {
     if (x_dtype == HMC_F32)
-      launch_f64(p.stream, (const double*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
+      launch_f64(c, (const double*)p.G_tgt, p.ldg, (const float*)X, c.ldx, n, B, (int)p.ldg, e);
     else
-      launch_f64(p.stream, (const double*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
+      launch_f64(c, (const double*)p.G_tgt, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e);
   }
   time_end(c, 1, ev);
 }
@@ -818,9 +912,9 @@ void gemm_gradient(Chains& c, const void* X, double* out) {
     launch_bf16(c, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
   } else if (p.precision == HMC_F64) {
     EpiResid<double> e1{(double*)c.r, nullptr, nullptr, c.ldr, p.dobs, p.var, n, B};
-    launch_f64(p.stream, (const double*)p.G, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e1);
+    launch_f64(c, (const double*)p.G, p.ldg, (const double*)X, c.ldx, n, B, (int)p.ldg, e1);
     EpiGrad<double> e2{(const double*)X, out, c.ldx, c.oob, md, d, B};
-    launch_f64(p.stream, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
+    launch_f64(c, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
   } else {
     throw Error{HMC_ERR_UNSUPPORTED, "GEMM engine supports float64 and bfloat16 trajectories"};
   }

@@ -100,6 +100,8 @@ struct Chains {
   double* partial = nullptr;// endpoint partial sums [mblocks][B_pad]
   float* split_ws = nullptr;// bf16 GEMM split-tail partial planes
   size_t split_ws_elems = 0;
+  double* split_ws64 = nullptr;  // FP64 GEMM split-tail partial planes
+  size_t split_ws64_elems = 0;
   int64_t n_mblocks = 0;
   double* k0 = nullptr;     // kinetic energy at the start of the trajectory
   double* eps = nullptr;    // local step per chain
@@ -187,7 +189,8 @@ void time_collect(Chains& c);
 // ---- GEMM engine (hmc_gemm.hip) -------------------------------------------
 // Row padding of G and G^T: every GEMM configuration's M-tile divides it.
 int64_t gemm_row_pad();
-size_t gemm_split_workspace(const Problem& p, int64_t B);  // float elements
+size_t gemm_split_workspace(const Problem& p, int64_t B);      // float elements
+size_t gemm_split_workspace_f64(const Problem& p, int64_t B);  // double elements
 // Trajectory gradient + fused kick/drift for every chain:
 //   p -= kick*eps*g(xt); xt += drift*eps*Minv p; corrector; [xt += drift2*eps*Minv p; corrector]
 void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);

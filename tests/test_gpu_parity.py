@@ -142,8 +142,8 @@ def test_bf16_trajectory_close_to_fp64(incremental, split_grad):
         assert abs(res["H1"][c] - (chi1 + k1)) <= 1e-6 * abs(chi1 + k1), c
 
 
-@pytest.mark.parametrize("cfg", [1, 3])
-def test_bf16_split_tail_matches_unsplit(cfg):
+@pytest.mark.parametrize("precision,cfg", [("bf16", 1), ("bf16", 3), ("f64", 1)])
+def test_split_tail_matches_unsplit(precision, cfg):
     """Planning for 16 compute units makes both GEMMs of a 2500 x 2200 problem
     with 256 chains leave a partial last round (residual 40 tiles of 128 rows
     or 20 of 256, gradient 36 or 18), so the tail tiles run as float32
@@ -151,7 +151,8 @@ def test_bf16_split_tail_matches_unsplit(cfg):
     unsplit one up to float32 summation order (1e-5 of the largest
     coordinate after 5 leapfrog steps), its end-point Hamiltonians are FP64
     of those end points, and gradients stay within the bf16 tolerance of the
-    oracle."""
+    oracle.  The FP64 endpoint GEMM (128-row tiles: 80 of them) splits its
+    tail the same way, and so do both GEMMs of FP64 trajectories (to 1e-12)."""
     rng = np.random.default_rng(21)
     n, d, B, L = 2500, 2200, 256, 5
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
@@ -165,22 +166,24 @@ def test_bf16_split_tail_matches_unsplit(cfg):
     u = rng.uniform(size=B)
     res, grad = {}, {}
     for name, opts in (("split", {"gemm_split_cus": 16}), ("plain", {"gemm_split_tail": 0})):
-        prob = build_problem(post, precision="bf16",
-                             options={"gemm_config": cfg, "incremental_residual": 1, **opts})
+        base = {"gemm_config": cfg, "incremental_residual": 1} if precision == "bf16" else {}
+        prob = build_problem(post, precision=precision, engine="gemm", options={**base, **opts})
         ch = Chains(prob, B)
         ch.set_state(X0)
         res[name] = ch.propose(P0, e, u, L, "lf", trajectory=True)
         grad[name] = prob.gradient_batch(X0[:4])
-    assert _rel(res["split"]["X1"], res["plain"]["X1"]) < 1e-5
-    assert _rel(res["split"]["P1"], res["plain"]["P1"]) < 1e-5
+    same = 1e-5 if precision == "bf16" else 1e-12
+    assert _rel(res["split"]["X1"], res["plain"]["X1"]) < same
+    assert _rel(res["split"]["P1"], res["plain"]["P1"]) < same
+    assert _rel(res["split"]["H0"], res["plain"]["H0"]) < 1e-12
     for c in range(0, B, 37):
         chi1 = orc.misfit(res["split"]["X1"][c].reshape(d, 1))
         k1 = 0.5 * float(np.sum(res["split"]["P1"][c] ** 2))
-        assert abs(res["split"]["H1"][c] - (chi1 + k1)) <= 1e-6 * abs(chi1 + k1), c
+        assert abs(res["split"]["H1"][c] - (chi1 + k1)) <= (1e-6 if precision == "bf16" else 1e-10) * abs(chi1 + k1), c
     for b in range(4):
         ref = orc.gradient(X0[b].reshape(d, 1))[:, 0]
-        assert _rel(grad["split"][b], ref) < 2e-4
-        assert _rel(grad["split"][b], grad["plain"][b]) < 1e-5
+        assert _rel(grad["split"][b], ref) < (2e-4 if precision == "bf16" else 1e-10)
+        assert _rel(grad["split"][b], grad["plain"][b]) < same
 
 
 def test_bf16_single_plane_gradient_gemm_tolerance():
