@@ -589,6 +589,25 @@ struct EpiResidRaw {
   template <typename V>
   __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
+    if (ms == 1 && m0 + 3 < M) {
+      // 4 consecutive rows of one chain: one 16-byte access of the residual
+      // and one 8-byte store per bf16 plane (o is a multiple of 4)
+      const int64_t o = n * ld + m0;
+      floatx4 r4 = accumulate ? *reinterpret_cast<const floatx4*>(&raw[o]) : floatx4{0.f, 0.f, 0.f, 0.f};
+      uint32_t h[4], l[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float r = accumulate ? r4[e] + (float)v[e] : (float)v[e] - (float)dobs[m0 + e];
+        r4[e] = r;
+        const float f = r / (float)var[m0 + e];
+        h[e] = f32_to_bf16_bits(f);
+        l[e] = f32_to_bf16_bits(f - bf16_bits_to_f32((uint16_t)h[e]));
+      }
+      *reinterpret_cast<floatx4*>(&raw[o]) = r4;
+      *reinterpret_cast<uint2*>(&oh[o]) = uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+      *reinterpret_cast<uint2*>(&ol[o]) = uint2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int64_t m = m0 + (int64_t)e * ms;
@@ -620,11 +639,50 @@ struct EpiLeap {
   DevModel md;
   int M, N;
   int delta_planes;     // planes hold x_new - x_old (residual increments) instead of x
+  __device__ __forceinline__ void step(int64_t j, T& xv, T& pv, T g, T kk, T dd, T dd2) const {
+    g += prior_grad<T>(md, j, xv);
+    if (out_of_bounds(md, j, (double)xv)) g = (T)INFINITY;
+    pv -= kk * g;
+    xv += dd * kgrad<T>(md, j, pv);
+    corrector<T>(md, j, xv, pv);
+    if (drift2 != 0.0) {
+      xv += dd2 * kgrad<T>(md, j, pv);
+      corrector<T>(md, j, xv, pv);
+    }
+  }
   template <typename V>
   __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
     const double e = eps[n];
     const T kk = (T)(beta ? kick * e * beta[n] : kick * e), dd = (T)(drift * e), dd2 = (T)(drift2 * e);
+    if constexpr (sizeof(T) == 4) {
+      if (ms == 1 && m0 + 3 < M) {
+        // 4 consecutive parameters of one chain: 16-byte accesses of x and p,
+        // one 8-byte store per bf16 plane (o is a multiple of 4)
+        const int64_t o = n * ld + m0;
+        floatx4 x4 = *reinterpret_cast<const floatx4*>(&x[o]);
+        floatx4 p4 = *reinterpret_cast<const floatx4*>(&p[o]);
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          T xv = x4[k], pv = p4[k];
+          const T xold = xv;
+          step(m0 + k, xv, pv, (T)v[k], kk, dd, dd2);
+          x4[k] = xv;
+          p4[k] = pv;
+          const float f = delta_planes ? (float)(xv - xold) : (float)xv;
+          h[k] = f32_to_bf16_bits(f);
+          l[k] = f32_to_bf16_bits(f - bf16_bits_to_f32((uint16_t)h[k]));
+        }
+        *reinterpret_cast<floatx4*>(&x[o]) = x4;
+        *reinterpret_cast<floatx4*>(&p[o]) = p4;
+        if (xh) {
+          *reinterpret_cast<uint2*>(&xh[o]) = uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+          *reinterpret_cast<uint2*>(&xl[o]) = uint2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int64_t j = m0 + (int64_t)k * ms;
@@ -632,15 +690,7 @@ struct EpiLeap {
       const int64_t o = n * ld + j;
       T xv = x[o], pv = p[o];
       const T xold = xv;
-      T g = (T)v[k] + prior_grad<T>(md, j, xv);
-      if (out_of_bounds(md, j, (double)xv)) g = (T)INFINITY;
-      pv -= kk * g;
-      xv += dd * kgrad<T>(md, j, pv);
-      corrector<T>(md, j, xv, pv);
-      if (drift2 != 0.0) {
-        xv += dd2 * kgrad<T>(md, j, pv);
-        corrector<T>(md, j, xv, pv);
-      }
+      step(j, xv, pv, (T)v[k], kk, dd, dd2);
       x[o] = xv;
       p[o] = pv;
       if (xh) {
