@@ -501,7 +501,8 @@ def run_tomography(args):
     post = D.BayesRule([D.Normal(np.full((d, 1), prob_d["prior_mean"]), sm2),
                         D.LinearMatrix(G.astype(dtype), dd.astype(dtype) if dtype == np.float32 else dd,
                                        sd2, dtype=dtype if dtype == np.float32 else None)])
-    prob = build_problem(post, precision=args.precision, mass_diagonal=mass)
+    prob = build_problem(post, precision=args.precision, mass_diagonal=mass,
+                         options={"fused_tile_chains": args.fused_tile_chains} if args.fused_tile_chains else None)
     ch = Chains(prob, B)
     ch.set_state(X0)
     eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), args, args.seed)
@@ -707,8 +708,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=20000)
     ap.add_argument("--d", type=int, default=8192)
-    ap.add_argument("--chains", type=int, default=512)
-    ap.add_argument("--steps-per-traj", type=int, default=25)
+    ap.add_argument("--chains", type=int, default=None,
+                    help="default: 512 (dense), 1024 (tomography)")
+    ap.add_argument("--steps-per-traj", type=int, default=None,
+                    help="default: 25 (dense), 20 (tomography)")
     ap.add_argument("--eps-scale", type=float, default=0.3)
     ap.add_argument("--tune-rounds", type=int, default=6)
     ap.add_argument("--tune-props", type=int, default=3)
@@ -720,6 +723,8 @@ def main():
                     help="bf16 planes of the scaled residual in the gradient GEMM (1: one plane, "
                          "measured to leave acceptance unchanged; 2: hi/lo)")
     ap.add_argument("--gemm-config", type=int, default=3, choices=[0, 1, 2, 3])
+    ap.add_argument("--fused-tile-chains", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="tomography: largest chains per fused-engine workgroup (0: as LDS allows)")
     ap.add_argument("--split-tail", type=int, default=1, choices=[0, 1],
                     help="bf16 GEMMs: split the last partial round of tiles along K")
     ap.add_argument("--tile-order", type=int, default=0, choices=[0, 1])
@@ -748,6 +753,11 @@ def main():
     args = ap.parse_args()
     if args.plain_steps is None:
         args.plain_steps = args.steps
+    tomo = args.workload == "tomography"
+    if args.chains is None:
+        args.chains = 1024 if tomo else 512
+    if args.steps_per_traj is None:
+        args.steps_per_traj = 20 if tomo else 25
     if args.cpu_baseline_child:
         cpu_baseline_child(args)
         return
@@ -755,7 +765,7 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); chains shard across GPUs "
               "without communication, so the one-GPU figure is the per-GPU rate.", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "tomography":
+    if tomo:
         if args.precision == "bf16":
             args.precision = "f32"
         res = run_tomography(args)

@@ -478,6 +478,10 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
     } else if (key == "gemm_split_tail") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_split_tail must be 0 or 1");
       P.split_tail = (int)value;
+    } else if (key == "fused_tile_chains") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0, HMC_ERR_ARG,
+                  "fused_tile_chains must be 0, 1, 2 or 4");
+      P.fused_ct = (int)value;
     } else if (key == "gemm_split_cus") {
       HMC_REQUIRE(value >= 0.0 && value <= 65536.0 && value == (double)(int)value, HMC_ERR_ARG,
                   "gemm_split_cus must be a non-negative integer");

@@ -38,6 +38,7 @@ struct Problem {
   int gemm_cfg_grad = 1;    // ... of the gradient GEMM
   int split_tail = 1;       // bf16 GEMMs: split the last partial round of tiles along K
   int n_cu = 0;             // compute units of the device (split-tail planning)
+  int fused_ct = 0;         // fused engine: largest chains per workgroup (0: as LDS allows)
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 
   // dense (row-major, zero padded): G [n_pad][ldg], Gt [d_pad][ldgt]

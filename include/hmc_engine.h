@@ -133,6 +133,10 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *   "gemm_split_tail" 0 or 1: when the tiles of a bf16 GEMM leave a last
  *                 partial round on the compute units, split those tiles along
  *                 K into float32 partials summed in fixed order (default 1)
+ *   "fused_tile_chains" 0, 1, 2 or 4: chains one workgroup of the fused
+ *                 engine carries (0 = automatic: 2 if the LDS tile fits,
+ *                 else 4 (float32) or 1; a value that does not fit the LDS
+ *                 tile or exceeds the chains is an error at the first call)
  *   "gemm_split_cus" n: compute units the split-tail plan assumes (0 = the
  *                 device's count, the default; a smaller n plans for a share
  *                 of the device)
