@@ -88,6 +88,29 @@ __device__ inline void sell_pass(const DevSell& S, const V* __restrict__ val,
   }
 }
 
+// One dense row of a lane over the k-major layout (element (row, k) at
+// mp[k * ld]): acc[c] += sum_k mp[k * ld] * in[k][c], k ascending, with the
+// matrix loads of SELL_U consecutive k issued before their use.
+template <typename Acc, typename V, typename Tin, int CT>
+__device__ inline void dense_row(const V* __restrict__ mp, int64_t ld, int64_t ncols,
+                                 const Tin* __restrict__ in, Acc acc[CT]) {
+  int64_t k = 0;
+  for (; k + SELL_U <= ncols; k += SELL_U) {
+    Acc vl[SELL_U];
+#pragma unroll
+    for (int u = 0; u < SELL_U; ++u) vl[u] = (Acc)mp[(k + u) * ld];
+#pragma unroll
+    for (int u = 0; u < SELL_U; ++u)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[c] += vl[u] * (Acc)in[(k + u) * CT + c];
+  }
+  for (; k < ncols; ++k) {
+    const Acc v = (Acc)mp[k * ld];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] += v * (Acc)in[k * CT + c];
+  }
+}
+
 template <typename Acc, typename V, typename Tin, int CT, class F>
 __device__ inline void dense_pass(const DevDense& D, const V* __restrict__ M,
                                   const Tin* __restrict__ in, int w, int lane, F&& f) {
@@ -97,14 +120,7 @@ __device__ inline void dense_pass(const DevDense& D, const V* __restrict__ M,
     Acc acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = Acc(0);
-    if (row < D.nrows) {
-      const V* mp = M + row;
-      for (int64_t k = 0; k < D.ncols; ++k) {
-        const Acc v = (Acc)mp[k * D.ld];
-#pragma unroll
-        for (int c = 0; c < CT; ++c) acc[c] += v * (Acc)in[k * CT + c];
-      }
-    }
+    if (row < D.nrows) dense_row<Acc, V, Tin, CT>(M + row, D.ld, D.ncols, in, acc);
     f(s, row < D.nrows ? row : (int64_t)-1, acc);
   }
 }
@@ -280,14 +296,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
           sell_row<T, T, T, CT>(op.bwd_s.col, op.bwd_val, op.bwd_s.base[s], op.bwd_s.width[s], lane, Rs, g);
         } else {
           const int64_t row = s * 64 + lane;
-          if (row < op.d) {
-            const T* mp = op.bwd_val + row;
-            for (int64_t k = 0; k < op.n; ++k) {
-              const T v = mp[k * op.bwd_d.ld];
-#pragma unroll
-              for (int c = 0; c < CT; ++c) g[c] += v * Rs[k * CT + c];
-            }
-          }
+          if (row < op.d) dense_row<T, T, T, CT>(op.bwd_val + row, op.bwd_d.ld, op.n, Rs, g);
         }
         const int64_t j = op.param_of(s, lane);
         if (j >= 0) {
@@ -490,14 +499,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
           sell_row<T, T, T, CT>(op.bwd_s.col, op.bwd_val, op.bwd_s.base[s], op.bwd_s.width[s], lane, Rs, g);
         } else {
           const int64_t row = s * 64 + lane;
-          if (row < op.d) {
-            const T* mp = op.bwd_val + row;
-            for (int64_t k = 0; k < op.n; ++k) {
-              const T v = mp[k * op.bwd_d.ld];
-#pragma unroll
-              for (int c = 0; c < CT; ++c) g[c] += v * Rs[k * CT + c];
-            }
-          }
+          if (row < op.d) dense_row<T, T, T, CT>(op.bwd_val + row, op.bwd_d.ld, op.n, Rs, g);
         }
       }
       const int64_t j = op.param_of(s, lane);
