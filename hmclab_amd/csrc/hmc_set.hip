@@ -36,6 +36,9 @@
 
 namespace hmc {
 
+// k blocks (residual) or row groups (gradient) whose loads are issued together
+constexpr int SET_U = 4;
+
 constexpr int SC = 16;   // chains per problem: one MFMA tile width
 
 struct SetDesc {
@@ -103,7 +106,29 @@ __global__ __launch_bounds__(256) void k_set_resid(const SetDesc* __restrict__ d
     act[r] = rbase + 16 * r < D.n;
     grow[r] = G + D.g_off + (int64_t)(rbase + 16 * r + i) * D.ldg + 4 * q;
   }
-  for (int k0 = 0; k0 < D.ldg; k0 += 16) {
+  // the loads of SET_U consecutive 16-wide k blocks are issued before their
+  // MFMAs (same accumulation order)
+  int k0 = 0;
+  for (; k0 + 16 * SET_U <= D.ldg; k0 += 16 * SET_U) {
+    floatx4 bb[SET_U], aa[SET_U][RW];
+#pragma unroll
+    for (int u = 0; u < SET_U; ++u) {
+      bb[u] = *(const floatx4*)(xrow + k0 + 16 * u);
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+        aa[u][r] = act[r] ? *(const floatx4*)(grow[r] + k0 + 16 * u) : floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < SET_U; ++u)
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (!act[r]) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[u][r][e], bb[u][e], acc[r], 0, 0, 0);
+      }
+  }
+  for (; k0 < D.ldg; k0 += 16) {
     const floatx4 b = *(const floatx4*)(xrow + k0);
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
@@ -162,7 +187,27 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
   const float* wrow = W + D.w_off + (int64_t)i * D.ldr + 4 * q;
   const float* gcol = G + D.g_off + col0 + 4 * i;
   const int ngroups = (D.n + 15) >> 4;
-  for (int g = wv; g < ngroups; g += NW) {
+  int g = wv;
+  // SET_U row groups of this wave at a time: loads first, then the MFMAs
+  for (; g + NW * (SET_U - 1) < ngroups; g += NW * SET_U) {
+    floatx4 bb[SET_U], aa[SET_U][4];
+#pragma unroll
+    for (int v = 0; v < SET_U; ++v) {
+      const int r0 = (g + NW * v) * 16;
+      bb[v] = *(const floatx4*)(wrow + r0);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        aa[v][e] = cin ? *(const floatx4*)(gcol + (int64_t)(r0 + 4 * q + e) * D.ldg) : floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int v = 0; v < SET_U; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[v][e][u], bb[v][e], acc[u], 0, 0, 0);
+  }
+  for (; g < ngroups; g += NW) {
     const int r0 = g * 16;
     const floatx4 b = *(const floatx4*)(wrow + r0);
 #pragma unroll
