@@ -342,10 +342,12 @@ def run_dense(args):
         f"elementwise {ew_ms:.1f} ms/{ew_n}")
     if leap_ms >= res_ms:
         gemm_ms, gemm_n, gemm_key = leap_ms, leap_n, "gradient"
-        gemm_name = "k_gemm_bf16<...,EpiLeap> (gradient GEMM + kick/drift)"
+        gemm_name = ("k_gemm_bf16<...,EpiLeap> + k_split_epilogue (gradient GEMM + kick/drift; "
+                     "the timed region holds both kernels)")
     else:
         gemm_ms, gemm_n, gemm_key = res_ms, res_n, "residual"
-        gemm_name = "k_gemm_bf16<...,EpiResid> (residual GEMM)"
+        gemm_name = ("k_gemm_bf16<...,EpiResidRaw> + k_split_epilogue (residual GEMM; the timed region "
+                     "holds both kernels)")
     min_ess, err_mean, err_std, _, _ = _posterior_errors(torch, ch, args.steps, mu_np, sd_np, d)
     acc_rate = float(acc.mean() / args.steps)
     value = min_ess / elapsed
