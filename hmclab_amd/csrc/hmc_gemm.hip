@@ -575,7 +575,8 @@ struct EpiResid {
 };
 
 // residual with a float32 running copy: raw = acc - d (full product G x) or
-// raw += acc (increment G dx); writes raw and the bf16 hi/lo planes of raw/var.
+// raw += acc (increment G dx); writes raw and the bf16 hi/lo planes of raw/var
+// (ol null: the gradient GEMM reads the hi plane only).
 struct EpiResidRaw {
   static constexpr bool kReduce = false;
   float* raw;       // [N_pad][ld]
@@ -605,7 +606,7 @@ struct EpiResidRaw {
       }
       *reinterpret_cast<floatx4*>(&raw[o]) = r4;
       *reinterpret_cast<uint2*>(&oh[o]) = uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
-      *reinterpret_cast<uint2*>(&ol[o]) = uint2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
+      if (ol) *reinterpret_cast<uint2*>(&ol[o]) = uint2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
       return;
     }
 #pragma unroll
@@ -618,7 +619,7 @@ struct EpiResidRaw {
       const float f = r / (float)var[m];
       const uint16_t h = f32_to_bf16_bits(f);
       oh[o] = h;
-      ol[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
+      if (ol) ol[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
     }
   }
 };
@@ -678,7 +679,7 @@ struct EpiLeap {
         *reinterpret_cast<floatx4*>(&p[o]) = p4;
         if (xh) {
           *reinterpret_cast<uint2*>(&xh[o]) = uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
-          *reinterpret_cast<uint2*>(&xl[o]) = uint2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
+          if (xl) *reinterpret_cast<uint2*>(&xl[o]) = uint2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
         }
         return;
       }
@@ -697,7 +698,7 @@ struct EpiLeap {
         const float f = delta_planes ? (float)(xv - xold) : (float)xv;
         const uint16_t h = f32_to_bf16_bits(f);
         xh[o] = h;
-        xl[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
+        if (xl) xl[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
       }
     }
   }
@@ -893,7 +894,7 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
     if (inc) {
       // first stage: full product G x (hi/lo planes of x); later stages: the
       // float32 residual advances by G dx with one plane of dx
-      EpiResidRaw e1{c.rf, c.rh, c.rl, c.ldr, p.dobs, p.var, first ? 0 : 1, n, B};
+      EpiResidRaw e1{c.rf, c.rh, p.bf16_split_grad == 2 ? c.rl : nullptr, c.ldr, p.dobs, p.var, first ? 0 : 1, n, B};
       launch_bf16(c, p.gemm_cfg, first ? p.bf16_split : 1, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B,
                   (int)p.ldg, e1);
     } else {
@@ -902,8 +903,9 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
                   e1);
     }
     time_end(c, 0, ev1);
-    EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, c.xl, c.ldx, c.eps, c.beta, kick, drift, drift2, md, d, B,
-                      inc ? 1 : 0};
+    // incremental: the planes carry dx for the next one-plane increment (no lo plane)
+    EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, inc ? nullptr : c.xl, c.ldx, c.eps, c.beta, kick, drift,
+                      drift2, md, d, B, inc ? 1 : 0};
     time_begin(c, 3, ev2);
     launch_bf16(c, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
