@@ -98,6 +98,10 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
                        const std::vector<double>& val, int traj_dt, int tgt_dt) {
   S.rows = rows;
   S.cols = cols;
+  // 16-bit column indices: a sparse target runs on the fused engine, whose LDS
+  // tile holds (rows + cols) values per chain, far below 65536 of either
+  HMC_REQUIRE(cols <= 65535 && rows <= 65535, HMC_ERR_UNSUPPORTED,
+              "sparse operator with more than 65535 rows or columns exceeds the fused engine's LDS tile");
   S.nslices = (rows + 63) / 64;
   std::vector<int32_t> order(rows);
   std::iota(order.begin(), order.end(), 0);
@@ -125,7 +129,7 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
   HMC_REQUIRE(total <= INT32_MAX, HMC_ERR_UNSUPPORTED, "sparse matrix too large for int32 SELL offsets");
   S.total = total;
   S.max_width = maxw;
-  std::vector<int32_t> col(std::max<int64_t>(total, 1), 0);
+  std::vector<uint16_t> col(std::max<int64_t>(total, 1), 0);
   std::vector<unsigned char> vtraj(std::max<int64_t>(total, 1) * dtype_size(traj_dt), 0);
   std::vector<unsigned char> vtgt;
   const bool alias = (traj_dt == tgt_dt);
@@ -136,7 +140,7 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
       if (row < 0) continue;
       for (int64_t k = 0; k < ptr[row + 1] - ptr[row]; ++k) {
         const int64_t o = base[s] + k * 64 + r;
-        col[o] = idx[ptr[row] + k];
+        col[o] = (uint16_t)idx[ptr[row] + k];
         store_elem(vtraj.data(), traj_dt, o, val[ptr[row] + k]);
         if (!alias) store_elem(vtgt.data(), tgt_dt, o, val[ptr[row] + k]);
       }
@@ -144,7 +148,7 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
   S.base = (int32_t*)upload(P.owned, base.data(), base.size() * 4, P.stream);
   S.width = (int32_t*)upload(P.owned, width.data(), width.size() * 4, P.stream);
   S.perm = (int32_t*)upload(P.owned, perm.data(), perm.size() * 4, P.stream);
-  S.col = (int32_t*)upload(P.owned, col.data(), col.size() * 4, P.stream);
+  S.col = (uint16_t*)upload(P.owned, col.data(), col.size() * 2, P.stream);
   S.val = upload(P.owned, vtraj.data(), vtraj.size(), P.stream);
   S.val_tgt = alias ? S.val : upload(P.owned, vtgt.data(), vtgt.size(), P.stream);
 }

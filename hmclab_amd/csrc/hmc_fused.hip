@@ -25,7 +25,7 @@ struct DevSell {
   const int32_t* base;
   const int32_t* width;
   const int32_t* perm;
-  const int32_t* col;
+  const uint16_t* col;
   int64_t nslices;
 };
 
@@ -47,7 +47,7 @@ struct TypeOf<HMC_F32> { using T = float; };
 // unchanged).
 constexpr int SELL_U = 8;
 template <typename Acc, typename V, typename Tin, int CT>
-__device__ inline void sell_row(const int32_t* __restrict__ colp, const V* __restrict__ valp, int64_t base,
+__device__ inline void sell_row(const uint16_t* __restrict__ colp, const V* __restrict__ valp, int64_t base,
                                 int32_t width, int lane, const Tin* __restrict__ in, Acc acc[CT]) {
   int32_t k = 0;
   for (; k + SELL_U <= width; k += SELL_U) {
