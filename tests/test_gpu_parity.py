@@ -266,6 +266,30 @@ def test_teacher_forced_sparse(precision):
              "lf", precision, "auto", eps=0.02)
 
 
+@pytest.mark.parametrize("precision,tiles", [("f32", (1, 2, 4)), ("f64", (1, 2))])
+def test_fused_tile_chains_do_not_change_results(precision, tiles):
+    """Each chain's sums run in the same order whatever the number of chains
+    a fused-engine workgroup carries, so 1, 2 or 4 chains per workgroup give
+    bitwise identical proposals (sparse operator, host randoms)."""
+    dtype = np.float64 if precision == "f64" else np.float32
+    post, _, rng = _dense_case(8, 300, 200, prior="normal", sparse=True, g_dtype=dtype)
+    d = post.dimensions
+    B = 9
+    X0 = rng.normal(size=(B, d)) * 0.3
+    P0 = rng.normal(size=(B, d))
+    e = 0.02 * rng.uniform(0.5, 1.5, size=B)
+    u = rng.uniform(size=B)
+    out = []
+    for ct in tiles:
+        prob = build_problem(post, precision=precision, engine="fused", options={"fused_tile_chains": ct})
+        ch = Chains(prob, B)
+        ch.set_state(X0)
+        out.append(ch.propose(P0, e, u, 8, "lf", trajectory=True))
+    for r in out[1:]:
+        for k in ("X1", "P1", "H0", "H1"):
+            assert np.array_equal(r[k], out[0][k]), k
+
+
 def test_bounds_reflection_in_trajectory():
     # tight bounds force reflections inside the fused kick/drift epilogue
     rng = np.random.default_rng(10)
