@@ -565,11 +565,12 @@ def run_uneven(args):
     log(f"{args.problems} problems (sum n*d = {int(np.sum(n * d)):,}) ready in {time.perf_counter() - t0:.1f}s")
     # tune each problem's step towards acceptance 0.65 (multiplicative, per problem)
     done = 0
-    for _ in range(args.tune_rounds):
+    for r in range(args.tune_rounds):
         acc = ps.run(args.tune_props, L, step, seed=args.seed, first_proposal=done)
         done += args.tune_props
         rate = acc.reshape(-1, CHAINS).sum(axis=1) / (CHAINS * args.tune_props)
-        step *= np.repeat(np.exp(1.5 * (rate - 0.65)), CHAINS)
+        gain = 1.5 / (1.0 + r / 5.0)           # shrinking steps average out the acceptance noise
+        step *= np.repeat(np.exp(gain * (rate - 0.65)), CHAINS)
     log(f"tuned: acceptance per problem {rate.min():.2f}..{rate.max():.2f}; tiles {ps.tile_config()}")
     ps.run(args.warmup, L, step, seed=args.seed, first_proposal=done)
     done += args.warmup
@@ -715,8 +716,8 @@ def main():
     ap.add_argument("--steps-per-traj", type=int, default=None,
                     help="default: 25 (dense), 20 (tomography)")
     ap.add_argument("--eps-scale", type=float, default=0.3)
-    ap.add_argument("--tune-rounds", type=int, default=6)
-    ap.add_argument("--tune-props", type=int, default=3)
+    ap.add_argument("--tune-rounds", type=int, default=None, help="default: 6 (20 for the uneven batch)")
+    ap.add_argument("--tune-props", type=int, default=None, help="default: 3 (5 for the uneven batch)")
     ap.add_argument("--cpu-baseline-child", action="store_true")
     ap.add_argument("--cpu-eps", type=float, default=1e-3)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "f64", "f32"])
@@ -756,6 +757,11 @@ def main():
     if args.plain_steps is None:
         args.plain_steps = args.steps
     tomo = args.workload == "tomography"
+    uneven = args.workload == "uneven"
+    if args.tune_rounds is None:
+        args.tune_rounds = 20 if uneven else 6
+    if args.tune_props is None:
+        args.tune_props = 5 if uneven else 3
     if args.chains is None:
         args.chains = 1024 if tomo else 512
     if args.steps_per_traj is None:
