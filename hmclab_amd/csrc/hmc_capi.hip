@@ -163,6 +163,8 @@ static void set_data(Problem& P, int64_t n, const double* d_obs, const double* v
   P.dobs = (double*)upload(P.owned, d_obs, n * 8, P.stream);
   P.var = (double*)upload(P.owned, v.data(), n * 8, P.stream);
   P.n = n;
+  P.dtd = 0.0;
+  for (int64_t i = 0; i < n; ++i) P.dtd += d_obs[i] * (d_obs[i] / v[i]);
 }
 
 // ---- timing ------------------------------------------------------------------
@@ -482,6 +484,9 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
     } else if (key == "gemm_split_tail") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_split_tail must be 0 or 1");
       P.split_tail = (int)value;
+    } else if (key == "endpoint_premult") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "endpoint_premult must be 0 or 1");
+      P.premult = (int)value;
     } else if (key == "fused_tile_chains") {
       HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0, HMC_ERR_ARG,
                   "fused_tile_chains must be 0, 1, 2 or 4");
@@ -547,7 +552,7 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
       c.ex_tmp = (int32_t*)dev_alloc(c.owned, (size_t)c.B_pad * 2 * 4);
       c.ex_u = (double*)dev_alloc(c.owned, c.B_pad * 8);
       if (P.op == 1) {
-        c.n_mblocks = P.n_pad / 128;
+        c.n_mblocks = std::max(P.n_pad, P.d_pad) / 128;   // residual (n) or premultiplied (d) endpoint
         c.partial = (double*)dev_alloc(c.owned, (size_t)c.n_mblocks * c.B_pad * 8);
         c.split_ws64_elems = gemm_split_workspace_f64(P, c.B);
         if (c.split_ws64_elems) c.split_ws64 = (double*)dev_alloc(c.owned, c.split_ws64_elems * 8);

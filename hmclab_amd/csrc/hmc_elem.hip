@@ -288,20 +288,21 @@ void elem_drift(Chains& c, double drift) {
 void elem_metropolis(Chains& c, bool have_partials, int autotune, double target,
                      double learning_rate, int64_t prop_index, int64_t trace_slot) {
   Problem& p = *c.p;
-  const int64_t nmb = have_partials ? c.n_mblocks : 0;
+  const int64_t nmb = have_partials ? endpoint_blocks(c) : 0;
+  const double mconst = have_partials ? endpoint_const(p) : p.misfit_const;
   hipEvent_t ev;
   time_begin(c, 2, ev);
   if (trace_slot >= 0) HMC_REQUIRE(trace_slot < c.tr_cap, HMC_ERR_STATE, "trace capacity exceeded");
   if (p.state_dtype == HMC_F64)
     hipLaunchKernelGGL(k_metropolis<double>, dim3(c.B), dim3(EW_NT), 0, p.stream, (double*)c.x0,
                        c.chi0, (const double*)c.xt, (const double*)c.pt, c.ldx, p.d, c.partial, nmb,
-                       c.B_pad, p.misfit_const, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
+                       c.B_pad, mconst, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
                        autotune, target, learning_rate, prop_index, (double*)c.tr_x, c.tr_chi,
                        trace_slot, c.B, (const double*)c.beta, p.model());
   else
     hipLaunchKernelGGL(k_metropolis<float>, dim3(c.B), dim3(EW_NT), 0, p.stream, (float*)c.x0,
                        c.chi0, (const float*)c.xt, (const float*)c.pt, c.ldx, p.d, c.partial, nmb,
-                       c.B_pad, p.misfit_const, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
+                       c.B_pad, mconst, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
                        autotune, target, learning_rate, prop_index, (float*)c.tr_x, c.tr_chi,
                        trace_slot, c.B, (const double*)c.beta, p.model());
   HMC_HIP(hipGetLastError());
@@ -325,14 +326,15 @@ void elem_exchange(Chains& c, const int32_t* pairs, int64_t npairs, const double
 
 void elem_finish_misfit(Chains& c, const void* X, bool have_partials, double* out) {
   Problem& p = *c.p;
-  const int64_t nmb = have_partials ? c.n_mblocks : 0;
+  const int64_t nmb = have_partials ? endpoint_blocks(c) : 0;
+  const double mconst = have_partials ? endpoint_const(p) : p.misfit_const;
   if (p.state_dtype == HMC_F64)
     hipLaunchKernelGGL(k_finish_misfit<double>, dim3(c.B), dim3(EW_NT), 0, p.stream,
-                       (const double*)X, c.ldx, p.d, c.partial, nmb, c.B_pad, p.misfit_const, out,
+                       (const double*)X, c.ldx, p.d, c.partial, nmb, c.B_pad, mconst, out,
                        p.model());
   else
     hipLaunchKernelGGL(k_finish_misfit<float>, dim3(c.B), dim3(EW_NT), 0, p.stream,
-                       (const float*)X, c.ldx, p.d, c.partial, nmb, c.B_pad, p.misfit_const, out,
+                       (const float*)X, c.ldx, p.d, c.partial, nmb, c.B_pad, mconst, out,
                        p.model());
   HMC_HIP(hipGetLastError());
 }

@@ -747,6 +747,44 @@ struct EpiMisfit {
   }
 };
 
+// plain FP64 store out[n][m] = acc (m < M, n < N)
+struct EpiStoreF64 {
+  static constexpr bool kReduce = false;
+  double* out;
+  int64_t ld;
+  int M, N;
+  template <typename V>
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t m = m0 + (int64_t)e * ms;
+      if (m >= M) return;
+      out[n * ld + m] = (double)v[e];
+    }
+  }
+};
+
+// premultiplied endpoint: with y = H x (acc), partial sums over the block's
+// rows m of x_m (y_m - 2 b_m); chi_data = 0.5 (sum + d^T S^-1 d)
+template <typename TX>
+struct EpiQuad {
+  static constexpr bool kReduce = true;
+  const TX* x;       // chain rows [N][ldx]
+  int64_t ldx;
+  const double* hb;  // G^T S^-1 d
+  double* partial;   // [blocks][ldp]
+  int64_t ldp;
+  int M, N;
+  __device__ double term(int64_t m, int64_t n, double v) const {
+    if (m >= M || n >= N) return 0.0;
+    return (double)x[n * ldx + m] * (v - 2.0 * hb[m]);
+  }
+  __device__ void store_partial(int tm, int64_t n, double s) const {
+    if (n < N) partial[(int64_t)tm * ldp + n] = s;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
@@ -863,6 +901,7 @@ size_t gemm_split_workspace_f64(const Problem& p, int64_t B) {
   if (p.op != 1 || !p.split_tail) return 0;
   const int n = (int)p.n, d = (int)p.d;
   size_t e = split_elems_f64(n, (int)B, (int)p.ldg, p.n_cu);        // endpoint, FP64 residual
+  e = std::max(e, split_elems_f64(d, (int)B, (int)round_up(d, 64), p.n_cu));   // premultiplied endpoint
   if (p.precision == HMC_F64) e = std::max(e, split_elems_f64(d, (int)B, (int)p.ldgt, p.n_cu));
   return e;
 }
@@ -924,10 +963,99 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
   }
 }
 
-// Endpoint data misfit in FP64: partial sums of (G x - d)^2 / var per row block.
+// ---------------------------------------------------------------------------
+// Premultiplied endpoint (LinearMatrix.py:164-191, the reference's algebra
+// when n > d): chi_data = 0.5 (x^T H x - 2 b^T x + d^T S^-1 d) with
+// H = G^T S^-1 G and b = G^T S^-1 d built once in FP64 from the target G.
+// The endpoint GEMM then has K = d instead of n.
+// ---------------------------------------------------------------------------
+bool endpoint_premult(const Problem& p) {
+  if (p.op != 1 || p.n <= 0 || p.resolved_engine() != HMC_ENGINE_GEMM) return false;
+  return p.premult > 0 || (p.premult < 0 && p.n > p.d);
+}
+
+int64_t endpoint_blocks(const Chains& c) {
+  const Problem& p = *c.p;
+  return endpoint_premult(p) ? (p.d + F64_BM - 1) / F64_BM : (p.n + F64_BM - 1) / F64_BM;
+}
+
+double endpoint_const(const Problem& p) { return p.misfit_const + (endpoint_premult(p) ? 0.5 * p.dtd : 0.0); }
+
+// W[j][k] = Gt[j][k] / var[k] (k < n), 0 in the padding
+template <typename TG>
+__global__ void k_scale_cols(const TG* __restrict__ Gt, int64_t ldgt, int64_t rows, int64_t n,
+                             const double* __restrict__ var, double* __restrict__ W) {
+  for (int64_t j = blockIdx.y; j < rows; j += gridDim.y)
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ldgt; k += (int64_t)gridDim.x * blockDim.x) {
+      double g;
+      if constexpr (sizeof(TG) == 2) g = (double)bf16_bits_to_f32((uint16_t)Gt[j * ldgt + k]);
+      else g = (double)Gt[j * ldgt + k];
+      W[j * ldgt + k] = k < n ? g / var[k] : 0.0;
+    }
+}
+
+// v[k] = dobs[k] / var[k] (k < n); rows 1..127 of the 128-row operand stay 0
+__global__ void k_scaled_data(const double* __restrict__ dobs, const double* __restrict__ var, int64_t n,
+                              double* __restrict__ v) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    v[k] = dobs[k] / var[k];
+}
+
+template <typename TG>
+static void build_premult_t(Chains& c) {
+  Problem& p = *c.p;
+  const int d = (int)p.d;
+  const TG* Gt = (const TG*)p.Gt_tgt;
+  std::vector<void*> tmp;
+  try {
+    double* W = (double*)dev_alloc(tmp, (size_t)p.d_pad * p.ldgt * 8);
+    double* v = (double*)dev_alloc(tmp, (size_t)F64_BN * p.ldgt * 8);
+    const dim3 grid((unsigned)std::min<int64_t>((p.ldgt + 255) / 256, 64), (unsigned)std::min<int64_t>(p.d_pad, 65535));
+    hipLaunchKernelGGL(k_scale_cols<TG>, grid, dim3(256), 0, p.stream, Gt, p.ldgt, p.d_pad, p.n, p.var, W);
+    HMC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scaled_data, dim3((unsigned)std::min<int64_t>((p.n + 255) / 256, 1024)), dim3(256), 0,
+                       p.stream, p.dobs, p.var, p.n, v);
+    HMC_HIP(hipGetLastError());
+    // H[j2][j1] = sum_k Gt[j1][k] W[j2][k]; b[j] = sum_k Gt[j][k] v[k]
+    launch_f64(c, Gt, p.ldgt, (const double*)W, p.ldgt, d, d, (int)p.ldgt, EpiStoreF64{p.H, p.ldh, d, d});
+    launch_f64(c, Gt, p.ldgt, (const double*)v, p.ldgt, d, 1, (int)p.ldgt, EpiStoreF64{p.hb, p.ldh, d, 1});
+    HMC_HIP(hipStreamSynchronize(p.stream));
+  } catch (...) {
+    for (void* q : tmp) (void)hipFree(q);
+    throw;
+  }
+  for (void* q : tmp) (void)hipFree(q);
+}
+
+static void build_premult(Chains& c) {
+  Problem& p = *c.p;
+  p.ldh = round_up(p.d, 64);
+  p.H = (double*)dev_alloc(p.owned, (size_t)p.d_pad * p.ldh * 8);
+  p.hb = (double*)dev_alloc(p.owned, (size_t)p.d_pad * 8);
+  if (p.tgt_dtype == HMC_BF16) build_premult_t<uint16_t>(c);
+  else if (p.tgt_dtype == HMC_F32) build_premult_t<float>(c);
+  else build_premult_t<double>(c);
+}
+
+// Endpoint data misfit in FP64: partial sums per row block of (G x - d)^2 / var
+// (residual form), or of x (H x - 2 b) (premultiplied form).
 void gemm_endpoint(Chains& c, const void* X, int x_dtype) {
   Problem& p = *c.p;
   const int n = (int)p.n, B = (int)c.B;
+  if (endpoint_premult(p)) {
+    if (!p.H) build_premult(c);
+    const int d = (int)p.d;
+    hipEvent_t ev;
+    time_begin(c, 1, ev);
+    if (x_dtype == HMC_F32)
+      launch_f64(c, (const double*)p.H, p.ldh, (const float*)X, c.ldx, d, B, (int)p.ldh,
+                 EpiQuad<float>{(const float*)X, c.ldx, p.hb, c.partial, c.B_pad, d, B});
+    else
+      launch_f64(c, (const double*)p.H, p.ldh, (const double*)X, c.ldx, d, B, (int)p.ldh,
+                 EpiQuad<double>{(const double*)X, c.ldx, p.hb, c.partial, c.B_pad, d, B});
+    time_end(c, 1, ev);
+    return;
+  }
   EpiMisfit e{p.dobs, p.var, c.partial, c.B_pad, n, B};
   hipEvent_t ev;
   time_begin(c, 1, ev);

@@ -39,6 +39,13 @@ struct Problem {
   int split_tail = 1;       // bf16 GEMMs: split the last partial round of tiles along K
   int n_cu = 0;             // compute units of the device (split-tail planning)
   int fused_ct = 0;         // fused engine: largest chains per workgroup (0: as LDS allows)
+  // FP64 endpoint misfit through H = G^T S^-1 G (dense GEMM engine): -1 when
+  // n > d (the reference's premultiplication rule), 0 never, 1 always
+  int premult = -1;
+  double* H = nullptr;      // [d_pad][ldh] FP64, built at the first endpoint
+  double* hb = nullptr;     // [d_pad] G^T S^-1 d
+  int64_t ldh = 0;
+  double dtd = 0.0;         // d^T S^-1 d
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 
   // dense (row-major, zero padded): G [n_pad][ldg], Gt [d_pad][ldgt]
@@ -192,6 +199,9 @@ void time_collect(Chains& c);
 int64_t gemm_row_pad();
 size_t gemm_split_workspace(const Problem& p, int64_t B);      // float elements
 size_t gemm_split_workspace_f64(const Problem& p, int64_t B);  // double elements
+bool endpoint_premult(const Problem& p);      // endpoint misfit uses H (dense GEMM engine)
+int64_t endpoint_blocks(const Chains& c);     // rows of the endpoint partial sums
+double endpoint_const(const Problem& p);      // constant added to every misfit
 // Trajectory gradient + fused kick/drift for every chain:
 //   p -= kick*eps*g(xt); xt += drift*eps*Minv p; corrector; [xt += drift2*eps*Minv p; corrector]
 void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);

@@ -186,6 +186,40 @@ def test_split_tail_matches_unsplit(precision, cfg):
         assert _rel(grad["split"][b], grad["plain"][b]) < same
 
 
+@pytest.mark.parametrize("precision,g_dtype", [("bf16", "bfloat16"), ("f64", None)])
+def test_premultiplied_endpoint_matches_residual_form(precision, g_dtype):
+    """With n > d the GEMM engine evaluates the FP64 endpoint misfit as
+    0.5 (x^T H x - 2 b^T x + d^T S^-1 d), H = G^T S^-1 G built once in FP64
+    (the reference's premultiplied algebra, LinearMatrix.py:164-191).  It
+    equals the residual form and the oracle to 1e-11 relative, for misfits
+    and for the H0/H1 of a proposal; vector variances enter H and b."""
+    rng = np.random.default_rng(31)
+    n, d, B = 700, 300, 6
+    G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
+    dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+    var = rng.uniform(0.005, 0.02, size=(n, 1))
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, var, dtype=g_dtype)])
+    orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
+                       O.LinearMatrixSimple(G, dd, var, dtype=np.float64, premultiplication=False)])
+    X = (rng.normal(size=(B, d)) * 0.5).astype(np.float32).astype(np.float64)
+    P0 = rng.normal(size=(B, d)).astype(np.float32).astype(np.float64)
+    e = 0.002 * rng.uniform(0.5, 1.5, size=B)
+    u = rng.uniform(size=B)
+    chi, res = {}, {}
+    for name, opt in (("premult", {}), ("residual", {"endpoint_premult": 0})):
+        prob = build_problem(post, precision=precision, engine="gemm", options=opt)
+        chi[name] = prob.misfit_batch(X)
+        ch = Chains(prob, B)
+        ch.set_state(X)
+        res[name] = ch.propose(P0, e, u, 4, "lf", trajectory=True)
+    for b in range(B):
+        assert _rel(chi["premult"][b], orc.misfit(X[b].reshape(d, 1))) < 1e-11
+    assert _rel(chi["premult"], chi["residual"]) < 1e-11
+    for k in ("H0", "H1"):
+        assert _rel(res["premult"][k], res["residual"][k]) < 1e-11, k
+    assert np.array_equal(res["premult"]["X1"], res["residual"]["X1"])
+
+
 def test_bf16_single_plane_gradient_gemm_tolerance():
     """bf16_split_grad = 1 rounds the scaled residual to one bf16 plane in the
     gradient GEMM: gradients within 1e-2 of the oracle (stated tolerance of
