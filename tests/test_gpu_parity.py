@@ -142,11 +142,12 @@ def test_bf16_trajectory_close_to_fp64(incremental, split_grad):
         assert abs(res["H1"][c] - (chi1 + k1)) <= 1e-6 * abs(chi1 + k1), c
 
 
-@pytest.mark.parametrize("precision,cfg", [("bf16", 1), ("bf16", 3), ("f64", 1)])
-def test_split_tail_matches_unsplit(precision, cfg):
+@pytest.mark.parametrize("precision,cfg,B", [("bf16", 1, 256), ("bf16", 3, 256), ("bf16", 3, 200), ("f64", 1, 256)])
+def test_split_tail_matches_unsplit(precision, cfg, B):
     """Planning for 16 compute units makes both GEMMs of a 2500 x 2200 problem
     with 256 chains leave a partial last round (residual 40 tiles of 128 rows
-    or 20 of 256, gradient 36 or 18), so the tail tiles run as float32
+    or 20 of 256, gradient 36 or 18; 200 chains leave a partly empty chain
+    tile), so the tail tiles run as float32
     k-range partials summed by the split epilogue.  The trajectory equals the
     unsplit one up to float32 summation order (1e-5 of the largest
     coordinate after 5 leapfrog steps), its end-point Hamiltonians are FP64
@@ -154,7 +155,7 @@ def test_split_tail_matches_unsplit(precision, cfg):
     oracle.  The FP64 endpoint GEMM (128-row tiles: 80 of them) splits its
     tail the same way, and so do both GEMMs of FP64 trajectories (to 1e-12)."""
     rng = np.random.default_rng(21)
-    n, d, B, L = 2500, 2200, 256, 5
+    n, d, L = 2500, 2200, 5
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
     post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, dtype="bfloat16")])
