@@ -971,7 +971,10 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
 // ---------------------------------------------------------------------------
 bool endpoint_premult(const Problem& p) {
   if (p.op != 1 || p.n <= 0 || p.resolved_engine() != HMC_ENGINE_GEMM) return false;
-  return p.premult > 0 || (p.premult < 0 && p.n > p.d);
+  if (p.premult >= 0) return p.premult > 0;
+  // automatic: the reference's rule n > d, while H (d^2 doubles) stays modest
+  const double h_bytes = 8.0 * (double)round_up(p.d, 256) * (double)round_up(p.d, 64);
+  return p.n > p.d && h_bytes <= 16.0 * 1024 * 1024 * 1024;
 }
 
 int64_t endpoint_blocks(const Chains& c) {

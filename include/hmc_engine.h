@@ -137,7 +137,8 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 through H = G^T S^-1 G and b = G^T S^-1 d, built once in FP64
  *                 (chi = 0.5 (x^T H x - 2 b^T x + d^T S^-1 d), a GEMM with K = d
  *                 instead of n); default: when n > d, the reference's
- *                 premultiplication rule (LinearMatrix.py:164-167)
+ *                 premultiplication rule (LinearMatrix.py:164-167), and H
+ *                 takes at most 16 GiB
  *   "fused_tile_chains" 0, 1, 2 or 4: chains one workgroup of the fused
  *                 engine carries (0 = automatic: 2 if the LDS tile fits,
  *                 else 4 (float32) or 1; a value that does not fit the LDS
