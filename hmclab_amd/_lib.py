@@ -127,6 +127,15 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build the HIP engine with `python -m hmclab_amd.build`"
             )
+        # PyTorch ships its own HIP and HSA runtimes under the same sonames as
+        # the ROCm ones this library links.  Whichever set is mapped first
+        # serves the whole process, and torch's device initialisation fails
+        # ("No HIP GPUs are available") on the set it was not built with, so
+        # when torch is installed it is imported before this library loads.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)

@@ -45,8 +45,11 @@ def test_null_arguments_are_rejected():
         _lib.call("hmc_problem_create", 0, 0, C.byref(C.c_void_p()))
 
 
-@pytest.mark.skipif(_lib.device_count() > 0, reason="checks the no-device error path")
 def test_no_device_is_a_clear_runtime_error():
+    # decided at run time, not at collection, so that collecting this module
+    # never loads the library
+    if _lib.device_count() > 0:
+        pytest.skip("checks the no-device error path")
     from hmclab_amd.engine import Problem
 
     with pytest.raises(RuntimeError, match="no HIP device"):
