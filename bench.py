@@ -739,7 +739,7 @@ def main():
     ap.add_argument("--cpu-proposals", type=int, default=6,
                     help="oracle proposals per CPU-baseline process (about 1.4 s each on the dense workload)")
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1),
-                    help="concurrent single-core CPU-baseline processes (the GPU box's CPU share is 16)")
+                    help="concurrent single-core CPU-baseline processes (default: 16, or fewer cores)")
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--plain-steps", type=int, default=None,
                     help="timed proposals of the plain fp64 comparator (default: --steps, so both "
