@@ -21,11 +21,21 @@
 // waves per gradient tile) are chosen per set by timing every combination
 // on the set itself.
 //
+// Premultiplied trajectories (LinearMatrix.py:163-177, 203-207: the
+// reference's default when n > d): H = G^T C^-1 G and b = G^T C^-1 d are
+// built once in FP64 (stored fp32), and each step is one pass over H,
+//   grad_leap    tiles (problem, 64 columns): g = H x - b + prior'(x); kick/drift
+// with no residual launch.  Every workgroup of a problem then reads the whole
+// x, so x alternates between two buffers: step s reads buffer (L - s) mod 2
+// and writes the other, and the last step always writes xt.  The endpoint
+// stays in the residual form above (exact for the fp32 state).
+//
 // Layouts in HBM (fp32 unless noted), per problem p with d, n:
 //   G   [round_up(n,16)][ldg = round_up(d,16)]  row-major, zero padded
 //   x0, xt, pt  [C][ldg]   chain-major states (fp32), zero padded
 //   w   [C][ldr = round_up(n,16)]   scaled residuals
 //   dobs, ivar  [n] fp64;   prior a, b  [d] fp64
+//   premultiplied: H [ldg][ldg], xt2 [C][ldg] (fp32);  b [d] fp64
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -44,6 +54,7 @@ constexpr int SC = 16;   // chains per problem: one MFMA tile width
 struct SetDesc {
   int32_t d, n, ldg, ldr;
   int64_t g_off, x_off, w_off, v_off, a_off, h_off;   // h_off: offset in host-layout rows
+  int64_t hm_off;                                     // premultiplied G^T C^-1 G [ldg][ldg]
   int32_t prior_kind, L;
   int32_t ptile0, n_ptiles;                           // endpoint tiles of this problem
 };
@@ -156,13 +167,107 @@ __global__ __launch_bounds__(256) void k_set_resid(const SetDesc* __restrict__ d
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// residual, LDS-staged variant: the tile's 64*RW rows of G are read 64 columns
+// at a time, 16 lanes per row (256 contiguous bytes per row, as the gradient
+// tiles read G), into LDS through registers (the next chunk's loads are in
+// flight while the current chunk's MFMAs run); the MFMA sequence per
+// accumulator is the one of k_set_resid (k = 16 s + 4 q + e), so both give
+// identical results.
+// ---------------------------------------------------------------------------
+template <int RW>
+__global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict__ desc,
+                                                       const int2* __restrict__ tiles,
+                                                       const float* __restrict__ G,
+                                                       const float* __restrict__ X,
+                                                       const double* __restrict__ dobs,
+                                                       const double* __restrict__ ivar,
+                                                       float* __restrict__ W) {
+  constexpr int ROWS = 64 * RW, KC = 64, LD = KC + 4;
+  constexpr int GPT = ROWS * (KC / 4) / 256;   // float4 loads of G per thread per chunk
+  __shared__ __attribute__((aligned(16))) float Gs[ROWS * LD];
+  __shared__ __attribute__((aligned(16))) float Xs[SC * LD];
+  const int2 t = tiles[blockIdx.x];
+  const SetDesc D = desc[t.x];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i = lane & 15, q = lane >> 4;
+  const int r0 = t.y * ROWS;
+  const int nrow = (D.n + 15) & ~15;            // rows present in memory (zero padded)
+  const float* Gp = G + D.g_off;
+  const float* Xp = X + D.x_off;
+  // loader mapping: thread tid covers column 4 * (tid % 16) of rows tid / 16 + 16 j
+  const int lc = 4 * (tid & 15), lr = tid >> 4;
+  floatx4 gr[GPT], xr;
+  auto gload = [&](int k0) {
+    const int col = k0 + lc;
+#pragma unroll
+    for (int j = 0; j < GPT; ++j) {
+      const int row = r0 + lr + 16 * j;
+      gr[j] = (row < nrow && col < D.ldg) ? *(const floatx4*)(Gp + (int64_t)row * D.ldg + col)
+                                           : floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+    xr = (col < D.ldg) ? *(const floatx4*)(Xp + (int64_t)lr * D.ldg + col) : floatx4{0.f, 0.f, 0.f, 0.f};
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int j = 0; j < GPT; ++j) *(floatx4*)(&Gs[(lr + 16 * j) * LD + lc]) = gr[j];
+    *(floatx4*)(&Xs[lr * LD + lc]) = xr;
+  };
+  floatx4 acc[RW];
+  bool act[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    acc[r] = floatx4{0.f, 0.f, 0.f, 0.f};
+    act[r] = r0 + wv * 16 * RW + 16 * r < D.n;
+  }
+  gload(0);
+  for (int k0 = 0; k0 < D.ldg; k0 += KC) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + KC < D.ldg) gload(k0 + KC);
+    const int kmax = D.ldg - k0 < KC ? D.ldg - k0 : KC;   // a multiple of 16
+    for (int s16 = 0; s16 < kmax; s16 += 16) {
+      const floatx4 b = *(const floatx4*)(&Xs[i * LD + s16 + 4 * q]);
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (!act[r]) continue;
+        const floatx4 a = *(const floatx4*)(&Gs[(wv * 16 * RW + 16 * r + i) * LD + s16 + 4 * q]);
+        acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], b[1], acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], b[2], acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], acc[r], 0, 0, 0);
+      }
+    }
+  }
+  const double* dp = dobs + D.v_off;
+  const double* vp = ivar + D.v_off;
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    if (!act[r]) continue;
+    const int row0 = r0 + wv * 16 * RW + 16 * r + 4 * q;
+    floatx4 o;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = row0 + v;
+      o[v] = (row < D.n) ? (acc[r][v] - (float)dp[row]) * (float)vp[row] : 0.f;
+    }
+    *(floatx4*)(W + D.w_off + (int64_t)i * D.ldr + row0) = o;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // gradient + kick/drift on 64 columns of one problem.  NW waves split the row
 // groups of 16; lane (i, q) loads G[row][col0 + 4i .. 4i+3] (one float4 per
 // row, 256 contiguous bytes per wave) and feeds component t to MFMA tile t,
 // so tile t's A row i is column col0 + 4i + t; B = w[chain i][row].  The
 // accumulator of lane (q, i) then holds columns col0 + 16q + 4v + t of chain i.
-// MODE 0: kick/drift; MODE 1: store the gradient (f64) for gradient queries.
+// MODE 0: kick/drift; MODE 1: store the gradient (f64) for gradient queries;
+// MODE 2: kick/drift from the premultiplied form: G is H, the rows summed are
+// H's ldg rows and the weights are the chain states x (H symmetric, so
+// sum_k H[k][j] x[k] = (H x)[j]), read from buffer (L - step) mod 2 of
+// X / X2 and written to the other; b is subtracted.
 // ---------------------------------------------------------------------------
 template <int NW, int MODE>
 __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict__ desc,
@@ -173,7 +278,9 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
                                                       const double* __restrict__ pa,
                                                       const double* __restrict__ pb,
                                                       const double* __restrict__ eps, int step,
-                                                      double* __restrict__ gout) {
+                                                      double* __restrict__ gout,
+                                                      float* __restrict__ X2,
+                                                      const double* __restrict__ hb) {
   __shared__ float red[NW][SC][64];
   const int2 t = tiles[blockIdx.x];
   const SetDesc D = desc[t.x];
@@ -184,9 +291,13 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
   floatx4 acc[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const float* wrow = W + D.w_off + (int64_t)i * D.ldr + 4 * q;
-  const float* gcol = G + D.g_off + col0 + 4 * i;
-  const int ngroups = (D.n + 15) >> 4;
+  const bool flip = MODE == 2 && ((D.L - step) & 1);
+  const float* Xin = flip ? X2 : X;
+  float* Xout = MODE == 2 ? (flip ? X : X2) : X;
+  const float* wrow = MODE == 2 ? Xin + D.x_off + (int64_t)i * D.ldg + 4 * q
+                                : W + D.w_off + (int64_t)i * D.ldr + 4 * q;
+  const float* gcol = G + (MODE == 2 ? D.hm_off : D.g_off) + col0 + 4 * i;
+  const int ngroups = MODE == 2 ? D.ldg >> 4 : (D.n + 15) >> 4;
   int g = wv;
   // SET_U row groups of this wave at a time: loads first, then the MFMAs
   for (; g + NW * (SET_U - 1) < ngroups; g += NW * SET_U) {
@@ -232,7 +343,8 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
 #pragma unroll
     for (int w = 0; w < NW; ++w) gs += red[w][c][col];
     const int64_t o = D.x_off + (int64_t)c * D.ldg + j;
-    const float xv0 = X[o];
+    const float xv0 = Xin[o];
+    if (MODE == 2) gs -= (float)hb[D.a_off + j];
     const float gv = gs + set_prior_grad(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], xv0);
     if (MODE == 1) {
       gout[D.h_off + (int64_t)c * D.d + j] = (double)gv;
@@ -242,10 +354,84 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
       const float drift = (step == D.L - 1) ? 0.5f : 1.f;
       float pv = P[o];
       pv -= ec * gv;
-      X[o] = xv0 + (drift * ec) * pv;
+      Xout[o] = xv0 + (drift * ec) * pv;
       P[o] = pv;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// premultiplied operator of one problem, 64 x 64 tile t.y of H (row-major
+// tiles of ceil(ldg/64) per side): H[j][k] = sum_r G[r][j] (1/var_r) G[r][k]
+// in FP64 (MFMA 16x16x4 f64, rows r in order), stored fp32.  Wave wv covers
+// rows j0 + 16 wv .. + 15; accumulator u holds columns k0 + 16 u + i of rows
+// j0 + 16 wv + q + 4 v in lane (i, q).  G's zero padding gives zero padding.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_set_build_h(const SetDesc* __restrict__ desc,
+                                                     const int2* __restrict__ tiles,
+                                                     const float* __restrict__ G,
+                                                     const double* __restrict__ ivar,
+                                                     float* __restrict__ H) {
+  const int2 t = tiles[blockIdx.x];
+  const SetDesc D = desc[t.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = lane & 15, q = lane >> 4;
+  const int nt = (D.ldg + 63) / 64;
+  const int j0 = (t.y / nt) * 64 + 16 * wv, k0 = (t.y % nt) * 64;
+  const float* Gp = G + D.g_off;
+  const double* wp = ivar + D.v_off;
+  const bool ja = j0 + i < D.ldg;
+  bool kb[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) kb[u] = k0 + 16 * u + i < D.ldg;
+  doublex4 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = doublex4{0.0, 0.0, 0.0, 0.0};
+  for (int r0 = 0; r0 < D.n; r0 += 4) {
+    const int r = r0 + q;
+    const bool rv = r < D.n;
+    const float* grow = Gp + (int64_t)r * D.ldg;
+    const double a = (rv && ja) ? (double)grow[j0 + i] : 0.0;
+    const double w = rv ? wp[r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double b = (rv && kb[u]) ? w * (double)grow[k0 + 16 * u + i] : 0.0;
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+    }
+  }
+  float* Hp = H + D.hm_off;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (!kb[u]) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int j = j0 + q + 4 * v;
+      if (j < D.ldg) Hp[(int64_t)j * D.ldg + k0 + 16 * u + i] = (float)acc[u][v];
+    }
+  }
+}
+
+// b[j] = sum_r G[r][j] d_r / var_r in FP64 for the 64 columns of a gradient
+// tile: four row phases of 64 threads, summed in fixed order.
+__global__ __launch_bounds__(256) void k_set_build_b(const SetDesc* __restrict__ desc,
+                                                     const int2* __restrict__ tiles,
+                                                     const float* __restrict__ G,
+                                                     const double* __restrict__ dobs,
+                                                     const double* __restrict__ ivar,
+                                                     double* __restrict__ hb) {
+  __shared__ double part[4][64];
+  const int2 t = tiles[blockIdx.x];
+  const SetDesc D = desc[t.x];
+  const int col = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int j = t.y * 64 + col;
+  double acc = 0.0;
+  if (j < D.d)
+    for (int r = ph; r < D.n; r += 4)
+      acc += (double)G[D.g_off + (int64_t)r * D.ldg + j] * (dobs[D.v_off + r] * ivar[D.v_off + r]);
+  part[ph][col] = acc;
+  __syncthreads();
+  if (ph == 0 && j < D.d)
+    hb[D.a_off + j] = ((part[0][col] + part[1][col]) + part[2][col]) + part[3][col];
 }
 
 // ---------------------------------------------------------------------------
@@ -305,7 +491,7 @@ constexpr int SW = 256;
 
 __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ desc,
                                                     const float* __restrict__ X0, float* __restrict__ XT,
-                                                    float* __restrict__ PT,
+                                                    float* __restrict__ XT2, float* __restrict__ PT,
                                                     const double* __restrict__ p_host,
                                                     double* __restrict__ k0, double* __restrict__ eps,
                                                     double* __restrict__ u,
@@ -331,6 +517,8 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
   const float half = (float)(0.5 * e);
   double ks = 0.0;
   const int64_t base = D.x_off + (int64_t)c * D.ldg;
+  // premultiplied trajectories start in buffer L mod 2 (their last step writes XT)
+  float* xs = (XT2 && (D.L & 1)) ? XT2 : XT;
   const int npairs = (D.d + 1) / 2;
   for (int qp = threadIdx.x; qp < npairs; qp += SW) {
     double z[2] = {0.0, 0.0};
@@ -341,7 +529,7 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
       if (j >= D.d) break;
       const float pv = R.host_p ? (float)p_host[D.h_off + (int64_t)c * D.d + j] : (float)z[h];
       ks += (double)pv * (double)pv;
-      XT[base + j] = X0[base + j] + half * pv;
+      xs[base + j] = X0[base + j] + half * pv;
       PT[base + j] = pv;
     }
   }
@@ -474,6 +662,13 @@ struct ProblemSet {
   // tiles
   std::vector<int32_t> order;     // problems by descending L (ties: index)
   int rw = 1, nw = 4;             // resid rows per wave (x16), grad waves per tile
+  int rlds = 0;                   // residual tiles staged through LDS (k_set_resid_lds)
+  // premultiplied trajectories: -1 automatic, 0 off, 1 on; pm = resolved, built
+  int premult = -1;
+  bool pm = false, pm_built = false;
+  int64_t hm_total = 0;
+  float *H = nullptr, *xt2 = nullptr;
+  double* hb = nullptr;
   bool tuned = false;
   int autotune = 1;
   int2 *rtiles = nullptr, *gtiles = nullptr, *ptiles = nullptr;
@@ -582,7 +777,16 @@ static void launch_resid(ProblemSet& s, int64_t ntiles, const float* X) {
   if (ntiles <= 0) return;
   hipEvent_t e;
   set_time(s, 0, true, e);
-  if (s.rw == 1)
+  if (s.rlds && s.rw == 1)
+    hipLaunchKernelGGL(k_set_resid_lds<1>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+                       s.G, X, s.dobs, s.ivar, s.w);
+  else if (s.rlds && s.rw == 2)
+    hipLaunchKernelGGL(k_set_resid_lds<2>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+                       s.G, X, s.dobs, s.ivar, s.w);
+  else if (s.rlds)
+    hipLaunchKernelGGL(k_set_resid_lds<4>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+                       s.G, X, s.dobs, s.ivar, s.w);
+  else if (s.rw == 1)
     hipLaunchKernelGGL(k_set_resid<1>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
                        s.G, X, s.dobs, s.ivar, s.w);
   else if (s.rw == 2)
@@ -602,13 +806,21 @@ static void launch_grad(ProblemSet& s, int64_t ntiles, float* X, int step, int m
   if (mode == 0) {
     if (s.nw == 4)
       hipLaunchKernelGGL((k_set_grad<4, 0>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
-                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr);
+                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr);
     else
       hipLaunchKernelGGL((k_set_grad<8, 0>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, s.gtiles,
-                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr);
+                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr);
+  } else if (mode == 2) {
+    // X is xt; the premultiplied step alternates between xt and xt2
+    if (s.nw == 4)
+      hipLaunchKernelGGL((k_set_grad<4, 2>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
+                         s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb);
+    else
+      hipLaunchKernelGGL((k_set_grad<8, 2>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, s.gtiles,
+                         s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb);
   } else {
     hipLaunchKernelGGL((k_set_grad<4, 1>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
-                       s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, s.gout);
+                       s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, s.gout, nullptr, nullptr);
   }
   HMC_HIP(hipGetLastError());
   set_time(s, 1, false, e);
@@ -628,13 +840,17 @@ static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
   const int64_t B = s.chains();
   hipEvent_t e;
   set_time(s, 3, true, e);
-  hipLaunchKernelGGL(k_set_refresh, dim3((unsigned)B), dim3(SW), 0, s.stream, s.desc, s.x0, s.xt, s.pt,
-                     s.p_host, s.k0, s.eps, s.u, s.step, s.run);
+  hipLaunchKernelGGL(k_set_refresh, dim3((unsigned)B), dim3(SW), 0, s.stream, s.desc, s.x0, s.xt,
+                     s.pm ? s.xt2 : nullptr, s.pt, s.p_host, s.k0, s.eps, s.u, s.step, s.run);
   HMC_HIP(hipGetLastError());
   set_time(s, 3, false, e);
   for (int st = 0; st < s.Lmax; ++st) {
-    launch_resid(s, s.rprefix[st], s.xt);
-    launch_grad(s, s.gprefix[st], s.xt, st, 0);
+    if (s.pm) {
+      launch_grad(s, s.gprefix[st], s.xt, st, 2);
+    } else {
+      launch_resid(s, s.rprefix[st], s.xt);
+      launch_grad(s, s.gprefix[st], s.xt, st, 0);
+    }
   }
   launch_endpoint(s, s.xt);
   set_time(s, 3, true, e);
@@ -687,6 +903,53 @@ static void set_L(ProblemSet& s, const int32_t* L) {
   }
 }
 
+// Resolve the trajectory form and build H, b on first use.  Automatic: on when
+// one pass over every H (sum d^2) reads fewer bytes than the residual and
+// gradient passes over every G (2 sum n d) and H fits in 16 GiB.
+static void ensure_premult(ProblemSet& s) {
+  bool want = s.premult > 0;
+  if (s.premult < 0) {
+    double hd = 0.0, gd = 0.0;
+    for (int64_t p = 0; p < s.P; ++p) {
+      hd += (double)s.hd[p].ldg * s.hd[p].ldg;
+      gd += 2.0 * (double)round_up(s.hp[p].n, 16) * s.hd[p].ldg;
+    }
+    want = hd < gd && hd * 4.0 <= 16.0 * (1ull << 30);
+  }
+  if (want != s.pm) {
+    s.pm = want;
+    s.tuned = false;
+    drop_graph(s);
+  }
+  if (!s.pm || s.pm_built) return;
+  s.H = (float*)dev_alloc(s.owned, s.hm_total * 4);
+  s.hb = (double*)dev_alloc(s.owned, s.a_total * 8);
+  s.xt2 = (float*)dev_alloc(s.owned, s.x_total * 4);   // zero filled, padding included
+  std::vector<int2> ht;
+  for (int64_t p = 0; p < s.P; ++p) {
+    const int64_t nt = (s.hd[p].ldg + 63) / 64;
+    for (int64_t k = 0; k < nt * nt; ++k) ht.push_back(make_int2((int)p, (int)k));
+  }
+  void* tl = nullptr;
+  HMC_HIP(hipMalloc(&tl, ht.size() * sizeof(int2)));
+  try {
+    h2d(tl, ht.data(), ht.size() * sizeof(int2));
+    hipLaunchKernelGGL(k_set_build_h, dim3((unsigned)ht.size()), dim3(256), 0, s.stream, s.desc,
+                       (const int2*)tl, s.G, s.ivar, s.H);
+    HMC_HIP(hipGetLastError());
+    // every problem's 64-column tiles (all L >= 1, so the step-0 prefix is the whole list)
+    hipLaunchKernelGGL(k_set_build_b, dim3((unsigned)s.gprefix[0]), dim3(256), 0, s.stream, s.desc, s.gtiles,
+                       s.G, s.dobs, s.ivar, s.hb);
+    HMC_HIP(hipGetLastError());
+    HMC_HIP(hipStreamSynchronize(s.stream));
+  } catch (...) {
+    (void)hipFree(tl);
+    throw;
+  }
+  HMC_HIP(hipFree(tl));
+  s.pm_built = true;
+}
+
 // Time every (rw, nw) combination on one trajectory step of this set.
 static void autotune_tiles(ProblemSet& s) {
   if (s.tuned || !s.autotune) return;
@@ -700,17 +963,24 @@ static void autotune_tiles(ProblemSet& s) {
   HMC_HIP(hipEventCreate(&a));
   HMC_HIP(hipEventCreate(&b));
   float best = 1e30f;
-  int brw = 1, bnw = 4;
+  int brw = s.rw, bnw = 4, blds = s.rlds;
+  for (int lds : {0, 1})
   for (int rw : {1, 2, 4})
     for (int nw : {4, 8}) {
+      if (s.pm && (lds != s.rlds || rw != s.rw)) continue;   // no residual launches
+      s.rlds = lds;
       s.rw = rw;
       s.nw = nw;
       build_tiles(s);
       for (int rep = 0; rep < 2; ++rep) {   // first pass warms caches and code
         HMC_HIP(hipEventRecord(a, s.stream));
         for (int k = 0; k < 3; ++k) {
-          launch_resid(s, s.rprefix[0], s.xt);
-          launch_grad(s, s.gprefix[0], s.xt, 0, 0);
+          if (s.pm) {
+            launch_grad(s, s.gprefix[0], s.xt, 0, 2);
+          } else {
+            launch_resid(s, s.rprefix[0], s.xt);
+            launch_grad(s, s.gprefix[0], s.xt, 0, 0);
+          }
         }
         HMC_HIP(hipEventRecord(b, s.stream));
       }
@@ -721,10 +991,12 @@ static void autotune_tiles(ProblemSet& s) {
         best = ms;
         brw = rw;
         bnw = nw;
+        blds = lds;
       }
     }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  s.rlds = blds;
   s.rw = brw;
   s.nw = bnw;
   build_tiles(s);
@@ -783,6 +1055,8 @@ int hmc_set_create(int64_t problems, const int64_t* dimensions, const int64_t* d
         s.a_total += q.d;
         D.h_off = s.h_total;
         s.h_total += (int64_t)SC * q.d;
+        D.hm_off = s.hm_total;
+        s.hm_total += (int64_t)D.ldg * D.ldg;
       }
       const int64_t B = s.chains();
       // G gets 64 floats of slack: the last gradient tile's guarded float4 loads stay inside
@@ -877,7 +1151,9 @@ int hmc_set_problem_prior(hmc_set* h, int64_t i, int kind, const double* a, cons
 }
 
 // "autotune" (1: time the tile shapes on first use, 0: keep the set ones),
-// "resid_rows" (64, 128, 256 rows per residual tile), "grad_waves" (4 or 8).
+// "resid_rows" (64, 128, 256 rows per residual tile), "grad_waves" (4 or 8),
+// "resid_lds" (1: residual tiles staged through LDS, 0: read into registers),
+// "premult" (-1 automatic, 0 residual form, 1 premultiplied trajectories).
 int hmc_set_set_option(hmc_set* h, const char* name, double value) {
   return guarded("hmc_set_set_option", [&] {
     HMC_REQUIRE(h && name, HMC_ERR_ARG, "NULL argument");
@@ -889,6 +1165,13 @@ int hmc_set_set_option(hmc_set* h, const char* name, double value) {
       HMC_REQUIRE(value == 64 || value == 128 || value == 256, HMC_ERR_ARG, "resid_rows must be 64, 128 or 256");
       s.rw = (int)value / 64;
       s.autotune = 0;
+    } else if (k == "premult") {
+      HMC_REQUIRE(value == -1 || value == 0 || value == 1, HMC_ERR_ARG, "premult must be -1, 0 or 1");
+      s.premult = (int)value;
+    } else if (k == "resid_lds") {
+      HMC_REQUIRE(value == 0 || value == 1, HMC_ERR_ARG, "resid_lds must be 0 or 1");
+      s.rlds = (int)value;
+      s.autotune = 0;
     } else if (k == "grad_waves") {
       HMC_REQUIRE(value == 4 || value == 8, HMC_ERR_ARG, "grad_waves must be 4 or 8");
       s.nw = (int)value;
@@ -897,6 +1180,22 @@ int hmc_set_set_option(hmc_set* h, const char* name, double value) {
       throw Error{HMC_ERR_ARG, "unknown option '" + k + "'"};
     }
     if (s.rtiles) build_tiles(s);
+  });
+}
+
+// Resolved values: "resid_rows", "grad_waves", "resid_lds", "autotune" and
+// "premult" (1 when trajectories use H; decided at the first run or proposal).
+int hmc_set_get_option(hmc_set* h, const char* name, double* value) {
+  return guarded("hmc_set_get_option", [&] {
+    HMC_REQUIRE(h && name && value, HMC_ERR_ARG, "NULL argument");
+    const ProblemSet& s = h->s;
+    const std::string k(name);
+    if (k == "resid_rows") *value = 64.0 * s.rw;
+    else if (k == "grad_waves") *value = s.nw;
+    else if (k == "resid_lds") *value = s.rlds;
+    else if (k == "autotune") *value = s.autotune;
+    else if (k == "premult") *value = s.pm ? 1.0 : 0.0;
+    else throw Error{HMC_ERR_ARG, "unknown option '" + k + "'"};
   });
 }
 
@@ -989,6 +1288,7 @@ int hmc_set_propose(hmc_set* h, const double* P0, const double* eps, const doubl
     ProblemSet& s = h->s;
     HMC_REQUIRE(s.state_set, HMC_ERR_STATE, "state not set");
     set_L(s, L);
+    ensure_premult(s);
     const int64_t B = s.chains();
     SetRun R{};
     R.host_p = 1;
@@ -1073,6 +1373,7 @@ int hmc_set_run(hmc_set* h, int64_t proposals, const int32_t* L, const double* s
     ProblemSet& s = h->s;
     HMC_REQUIRE(s.state_set, HMC_ERR_STATE, "state not set");
     set_L(s, L);
+    ensure_premult(s);
     autotune_tiles(s);
     const int64_t B = s.chains();
     SetRun R{};

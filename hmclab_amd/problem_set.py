@@ -74,10 +74,16 @@ class ProblemSet:
     def set_option(self, name: str, value: float):
         call("hmc_set_set_option", self._h, name.encode(), float(value))
 
+    def get_option(self, name: str) -> float:
+        v = C.c_double(0.0)
+        call("hmc_set_get_option", self._h, name.encode(), C.byref(v))
+        return v.value
+
     def tile_config(self):
         r, w = C.c_int(0), C.c_int(0)
         call("hmc_set_tile_config", self._h, C.byref(r), C.byref(w))
-        return dict(resid_rows=r.value, grad_waves=w.value)
+        return dict(resid_rows=r.value, grad_waves=w.value, resid_lds=int(self.get_option("resid_lds")),
+                    premult=int(self.get_option("premult")))
 
     # ---- layout helpers ----------------------------------------------------------
     def split(self, flat):

@@ -312,8 +312,18 @@ int hmc_set_problem_dense(hmc_set* s, int64_t problem, const float* G, const dou
  * (a = means, b = inverse dispersions), [d] each. */
 int hmc_set_problem_prior(hmc_set* s, int64_t problem, int kind, const double* a, const double* b);
 /* "autotune" 1|0 (time the tile shapes on this set at first run; default 1),
- * "resid_rows" 64|128|256, "grad_waves" 4|8 (fix a shape, no autotune). */
+ * "resid_rows" 64|128|256, "grad_waves" 4|8, "resid_lds" 0|1 (residual tiles
+ * read into registers or staged through LDS; same results) (fix a shape, no
+ * autotune); "premult" -1|0|1: trajectories from H = G^T C^-1 G and
+ * b = G^T C^-1 d (built once in FP64, stored fp32) instead of the residual
+ * form, as LinearMatrix.py:163-177/203-207 does when premultiplication is on;
+ * -1 (default) chooses it when one pass over every H reads fewer bytes than
+ * the two passes over every G.  The FP64 accept test keeps the residual form. */
 int hmc_set_set_option(hmc_set* s, const char* name, double value);
+/* Resolved "resid_rows", "grad_waves", "resid_lds", "autotune", "premult"
+ * (the last is decided at the first run or proposal).  No reference
+ * counterpart (engine configuration). */
+int hmc_set_get_option(hmc_set* s, const char* name, double* value);
 int hmc_set_tile_config(hmc_set* s, int* resid_rows, int* grad_waves);
 int hmc_set_set_state(hmc_set* s, const double* X);
 int hmc_set_get_state(hmc_set* s, double* X, double* chi);

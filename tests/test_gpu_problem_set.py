@@ -76,9 +76,15 @@ def test_misfit_and_gradient_match_oracle():
             assert _rel(g[p][c], o.gradient(x)[:, 0]) < 1e-4, (p, c)
 
 
+@pytest.mark.parametrize("premult", [0, 1])
 @pytest.mark.parametrize("Ls", [[1, 1, 1, 1, 1, 1], [3, 12, 1, 7, 5, 9]])
-def test_teacher_forced_proposal_matches_oracle(Ls):
+def test_teacher_forced_proposal_matches_oracle(Ls, premult):
+    """Residual-form and premultiplied trajectories (H = G^T C^-1 G built in
+    FP64, stored fp32; the oracle keeps the residual form) against the
+    oracle; odd and even L exercise both state buffers of the premultiplied
+    form."""
     ps, orc, rng = _build(2, DIMS, RATIOS)
+    ps.set_option("premult", premult)
     ps.set_state(ps.join([rng.normal(size=(CHAINS, d)) * 0.2 for d in DIMS]))
     X0, chi0 = ps.get_state()
     # momenta live in float32 on the device: give the oracle the same values
@@ -86,6 +92,7 @@ def test_teacher_forced_proposal_matches_oracle(Ls):
     eps = 0.01 * rng.uniform(0.5, 1.5, size=ps.chains)
     u = rng.uniform(size=ps.chains)
     res = ps.propose(P0, eps, u, Ls)
+    assert ps.get_option("premult") == premult
     xs, pz, x1 = ps.split(X0), ps.split(P0), ps.split(res["X1"])
     near = 0
     for p, o in enumerate(orc):
@@ -105,11 +112,13 @@ def test_teacher_forced_proposal_matches_oracle(Ls):
     assert near <= 4
 
 
-def test_graph_replays_match_direct_launches_and_tile_shapes():
+@pytest.mark.parametrize("premult", [0, 1])
+def test_graph_replays_match_direct_launches_and_tile_shapes(premult):
     dims = [40, 200, 75, 310]
     out = []
     for graph, rows, waves in [(True, 64, 4), (False, 64, 4), (True, 128, 8), (True, 256, 4)]:
         ps, _, rng = _build(3, dims)
+        ps.set_option("premult", premult)
         ps.set_option("resid_rows", rows)
         ps.set_option("grad_waves", waves)
         X0, step = _start_and_steps(ps)
@@ -131,9 +140,35 @@ def test_graph_replays_match_direct_launches_and_tile_shapes():
         assert np.abs(o[1] - ref[1]).sum() <= 4
 
 
-def test_device_rng_posterior_matches_analytic_for_normal_problems():
+@pytest.mark.parametrize("rows", [64, 128, 256])
+def test_lds_staged_residual_is_bitwise_identical(rows):
+    """Residual tiles staged through LDS run the same MFMA sequence per
+    accumulator as the register-fed tiles: states, misfits and decisions agree
+    bit for bit (d = 75 and 310 end in partial 64-column chunks, n = 465 in a
+    partial tile)."""
+    dims = [40, 200, 75, 310]
+    out = []
+    for lds in (0, 1):
+        ps, _, _ = _build(3, dims)
+        ps.set_option("resid_rows", rows)
+        ps.set_option("grad_waves", 4)
+        ps.set_option("resid_lds", lds)
+        ps.set_option("premult", 0)
+        X0, step = _start_and_steps(ps)
+        ps.set_state(X0)
+        acc = ps.run(4, [6, 11, 3, 8], step, seed=4)
+        X1, chi1 = ps.get_state()
+        out.append((acc, X1, chi1))
+        ps.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("premult", [0, 1])
+def test_device_rng_posterior_matches_analytic_for_normal_problems(premult):
     dims = [20, 48]
     ps, orc, rng = _build(4, dims, noise=0.3)
+    ps.set_option("premult", premult)
     # start at the analytic posterior means of the Normal-prior problem 0; problem 1 (Laplace) near 0
     ps.set_state(ps.join([np.zeros((CHAINS, d)) for d in dims]))
     o = orc[0]
@@ -176,5 +211,6 @@ def test_uneven_workload_builder_and_autotune():
     acc = ps.run(6, L, np.concatenate(steps), seed=1)
     cfg = ps.tile_config()
     assert cfg["resid_rows"] in (64, 128, 256) and cfg["grad_waves"] in (4, 8)
+    assert cfg["premult"] == 1                     # n = 1.5 d: one pass over H reads fewer bytes
     rate = acc.reshape(12, CHAINS).sum(axis=1) / (6 * CHAINS)
     assert np.all(rate > 0.3), rate
