@@ -175,15 +175,26 @@ __global__ __launch_bounds__(256) void k_set_resid(const SetDesc* __restrict__ d
 // flight while the current chunk's MFMAs run); the MFMA sequence per
 // accumulator is the one of k_set_resid (k = 16 s + 4 q + e), so both give
 // identical results.
+//
+// KICK = 1: the premultiplied step on rows of H instead of G.  H is symmetric,
+// so the tile's rows j give (H x)_j = sum_k H[j][k] x[k] for its 64*RW
+// parameters, streamed row by row; the epilogue is the kick/drift of
+// k_set_grad (g = H x - b + prior'(x), x read from buffer (L - step) mod 2 of
+// X / X2 and written to the other).
 // ---------------------------------------------------------------------------
-template <int RW>
+template <int RW, int KICK>
 __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict__ desc,
                                                        const int2* __restrict__ tiles,
                                                        const float* __restrict__ G,
-                                                       const float* __restrict__ X,
+                                                       float* __restrict__ X,
                                                        const double* __restrict__ dobs,
                                                        const double* __restrict__ ivar,
-                                                       float* __restrict__ W) {
+                                                       float* __restrict__ W,
+                                                       float* __restrict__ X2, float* __restrict__ P,
+                                                       const double* __restrict__ pa,
+                                                       const double* __restrict__ pb,
+                                                       const double* __restrict__ hb,
+                                                       const double* __restrict__ eps, int step) {
   constexpr int ROWS = 64 * RW, KC = 64, LD = KC + 4;
   constexpr int GPT = ROWS * (KC / 4) / 256;   // float4 loads of G per thread per chunk
   __shared__ __attribute__((aligned(16))) float Gs[ROWS * LD];
@@ -193,9 +204,12 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i = lane & 15, q = lane >> 4;
   const int r0 = t.y * ROWS;
-  const int nrow = (D.n + 15) & ~15;            // rows present in memory (zero padded)
-  const float* Gp = G + D.g_off;
-  const float* Xp = X + D.x_off;
+  const int nr = KICK ? D.d : D.n;               // rows of the operator
+  const int nrow = (nr + 15) & ~15;              // rows present in memory (zero padded)
+  const float* Gp = G + (KICK ? D.hm_off : D.g_off);
+  const bool flip = KICK && ((D.L - step) & 1);
+  const float* Xin = flip ? X2 : X;
+  const float* Xp = Xin + D.x_off;
   // loader mapping: thread tid covers column 4 * (tid % 16) of rows tid / 16 + 16 j
   const int lc = 4 * (tid & 15), lr = tid >> 4;
   floatx4 gr[GPT], xr;
@@ -219,7 +233,7 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     acc[r] = floatx4{0.f, 0.f, 0.f, 0.f};
-    act[r] = r0 + wv * 16 * RW + 16 * r < D.n;
+    act[r] = r0 + wv * 16 * RW + 16 * r < nr;
   }
   gload(0);
   for (int k0 = 0; k0 < D.ldg; k0 += KC) {
@@ -240,6 +254,32 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
         acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], acc[r], 0, 0, 0);
       }
     }
+  }
+  if (KICK) {
+    // lane (i, q) holds parameters row0 .. row0 + 3 of chain i
+    if (step >= D.L) return;
+    const float ec = (float)eps[(int64_t)t.x * SC + i];
+    const float drift = (step == D.L - 1) ? 0.5f : 1.f;
+    float* Xout = flip ? X : X2;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (!act[r]) continue;
+      const int row0 = r0 + wv * 16 * RW + 16 * r + 4 * q;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = row0 + v;
+        if (j >= D.d) continue;
+        const int64_t o = D.x_off + (int64_t)i * D.ldg + j;
+        const float xv0 = Xin[o];
+        const float gv = (acc[r][v] - (float)hb[D.a_off + j]) +
+                         set_prior_grad(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], xv0);
+        float pv = P[o];
+        pv -= ec * gv;
+        Xout[o] = xv0 + (drift * ec) * pv;
+        P[o] = pv;
+      }
+    }
+    return;
   }
   const double* dp = dobs + D.v_off;
   const double* vp = ivar + D.v_off;
@@ -660,9 +700,9 @@ struct ProblemSet {
   int64_t* acc_count = nullptr;
   SetRun* run = nullptr;
   // tiles
-  std::vector<int32_t> order;     // problems by descending L (ties: index)
   int rw = 1, nw = 4;             // resid rows per wave (x16), grad waves per tile
   int rlds = 0;                   // residual tiles staged through LDS (k_set_resid_lds)
+  int hrow = 0;                   // premultiplied step: 0 column strips (k_set_grad), 1 rows of H
   // premultiplied trajectories: -1 automatic, 0 off, 1 on; pm = resolved, built
   int premult = -1;
   bool pm = false, pm_built = false;
@@ -671,8 +711,9 @@ struct ProblemSet {
   double* hb = nullptr;
   bool tuned = false;
   int autotune = 1;
-  int2 *rtiles = nullptr, *gtiles = nullptr, *ptiles = nullptr;
-  std::vector<int64_t> rprefix, gprefix;   // tiles of problems with L > s, per step s
+  int2 *rtiles = nullptr, *gtiles = nullptr, *htiles = nullptr, *ptiles = nullptr;
+  // per step s: tiles of the problems with L > s at [off[s], off[s] + count[s])
+  std::vector<int64_t> roff, rprefix, goff, gprefix, hoff, hprefix;
   int64_t n_ptiles = 0;
   int Lmax = 0;
   std::vector<int32_t> Lcur;
@@ -726,7 +767,7 @@ static void set_collect(ProblemSet& s) {
 static void free_tiles(ProblemSet& s) {
   for (void* p : s.tile_mem) (void)hipFree(p);
   s.tile_mem.clear();
-  s.rtiles = s.gtiles = nullptr;
+  s.rtiles = s.gtiles = s.htiles = nullptr;
 }
 
 static void drop_graph(ProblemSet& s) {
@@ -736,90 +777,132 @@ static void drop_graph(ProblemSet& s) {
   s.graph = nullptr;
 }
 
-// Tile lists for the current L and tile shapes: problems by descending L, so
-// the tiles active at step s are a prefix.
+// Tile lists for the current L and tile shapes, one per trajectory step: the
+// tiles of the problems with L > step, problems by decreasing tile cost (the
+// length of the rows a tile streams), so that the longest tiles start first
+// and short ones fill the end of the launch.  Step 0 covers every problem.
 static void build_tiles(ProblemSet& s) {
   free_tiles(s);
-  s.order.resize(s.P);
-  std::iota(s.order.begin(), s.order.end(), 0);
-  std::stable_sort(s.order.begin(), s.order.end(),
-                   [&](int32_t a, int32_t b) { return s.Lcur[a] > s.Lcur[b]; });
-  std::vector<int2> rt, gt;
-  std::vector<int64_t> rend(s.P), gend(s.P);
+  s.Lmax = 0;
+  for (int64_t p = 0; p < s.P; ++p) s.Lmax = std::max(s.Lmax, (int)s.Lcur[p]);
+  std::vector<int2> rt, gt, ht;
   const int rows = 64 * s.rw;
-  for (int64_t k = 0; k < s.P; ++k) {
-    const int32_t p = s.order[k];
-    const int64_t nr = (s.hp[p].n + rows - 1) / rows, nc = (s.hp[p].d + 63) / 64;
-    for (int64_t b = 0; b < nr; ++b) rt.push_back(make_int2(p, (int)b));
-    for (int64_t b = 0; b < nc; ++b) gt.push_back(make_int2(p, (int)b));
-    rend[k] = (int64_t)rt.size();
-    gend[k] = (int64_t)gt.size();
-  }
-  s.Lmax = s.P ? s.Lcur[s.order[0]] : 0;
-  s.rprefix.assign(s.Lmax, 0);
-  s.gprefix.assign(s.Lmax, 0);
-  for (int st = 0; st < s.Lmax; ++st) {
-    int64_t cnt = 0;
-    while (cnt < s.P && s.Lcur[s.order[cnt]] > st) ++cnt;
-    s.rprefix[st] = cnt ? rend[cnt - 1] : 0;
-    s.gprefix[st] = cnt ? gend[cnt - 1] : 0;
-  }
+  auto build = [&](std::vector<int2>& out, std::vector<int64_t>& off, std::vector<int64_t>& cnt,
+                   auto cost, auto ntiles) {
+    std::vector<int32_t> ord(s.P);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return cost(a) > cost(b); });
+    off.assign(s.Lmax, 0);
+    cnt.assign(s.Lmax, 0);
+    for (int st = 0; st < s.Lmax; ++st) {
+      off[st] = (int64_t)out.size();
+      for (int32_t p : ord) {
+        if (s.Lcur[p] <= st) continue;
+        const int64_t nb = ntiles(p);
+        for (int64_t b = 0; b < nb; ++b) out.push_back(make_int2(p, (int)b));
+      }
+      cnt[st] = (int64_t)out.size() - off[st];
+    }
+  };
+  auto ldg = [&](int32_t p) { return (int64_t)s.hd[p].ldg; };
+  // residual tiles stream rows of G (ldg long); gradient strips stream the
+  // rows of G (n of them) or of H (ldg); row tiles of H stream rows of H
+  build(rt, s.roff, s.rprefix, ldg, [&](int32_t p) { return (s.hp[p].n + rows - 1) / rows; });
+  build(gt, s.goff, s.gprefix,
+        [&](int32_t p) { return s.pm ? ldg(p) : round_up(s.hp[p].n, 16); },
+        [&](int32_t p) { return (s.hp[p].d + 63) / 64; });
+  build(ht, s.hoff, s.hprefix, ldg, [&](int32_t p) { return (s.hp[p].d + rows - 1) / rows; });
   s.rtiles = (int2*)dev_alloc(s.tile_mem, std::max<size_t>(rt.size(), 1) * sizeof(int2));
   s.gtiles = (int2*)dev_alloc(s.tile_mem, std::max<size_t>(gt.size(), 1) * sizeof(int2));
+  s.htiles = (int2*)dev_alloc(s.tile_mem, std::max<size_t>(ht.size(), 1) * sizeof(int2));
   h2d(s.rtiles, rt.data(), rt.size() * sizeof(int2));
   h2d(s.gtiles, gt.data(), gt.size() * sizeof(int2));
+  h2d(s.htiles, ht.data(), ht.size() * sizeof(int2));
   for (int64_t p = 0; p < s.P; ++p) s.hd[p].L = s.Lcur[p];
   h2d(s.desc, s.hd.data(), s.P * sizeof(SetDesc));
   drop_graph(s);
 }
 
-static void launch_resid(ProblemSet& s, int64_t ntiles, const float* X) {
+static void launch_grad(ProblemSet& s, float* X, int step, int mode);
+
+template <int RW, int KICK>
+static void launch_rows_lds(ProblemSet& s, int64_t ntiles, const int2* tiles, const float* Gop, float* X,
+                            int step) {
+  hipLaunchKernelGGL((k_set_resid_lds<RW, KICK>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
+                     Gop, X, s.dobs, s.ivar, s.w, KICK ? s.xt2 : nullptr, KICK ? s.pt : nullptr, s.pa, s.pb,
+                     KICK ? s.hb : nullptr, s.eps, step);
+}
+
+static void launch_resid(ProblemSet& s, int step, float* X) {
+  const int64_t ntiles = s.rprefix[step];
+  const int2* tiles = s.rtiles + s.roff[step];
   if (ntiles <= 0) return;
   hipEvent_t e;
   set_time(s, 0, true, e);
   if (s.rlds && s.rw == 1)
-    hipLaunchKernelGGL(k_set_resid_lds<1>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
-                       s.G, X, s.dobs, s.ivar, s.w);
+    launch_rows_lds<1, 0>(s, ntiles, tiles, s.G, X, 0);
   else if (s.rlds && s.rw == 2)
-    hipLaunchKernelGGL(k_set_resid_lds<2>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
-                       s.G, X, s.dobs, s.ivar, s.w);
+    launch_rows_lds<2, 0>(s, ntiles, tiles, s.G, X, 0);
   else if (s.rlds)
-    hipLaunchKernelGGL(k_set_resid_lds<4>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
-                       s.G, X, s.dobs, s.ivar, s.w);
+    launch_rows_lds<4, 0>(s, ntiles, tiles, s.G, X, 0);
   else if (s.rw == 1)
-    hipLaunchKernelGGL(k_set_resid<1>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+    hipLaunchKernelGGL(k_set_resid<1>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
                        s.G, X, s.dobs, s.ivar, s.w);
   else if (s.rw == 2)
-    hipLaunchKernelGGL(k_set_resid<2>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+    hipLaunchKernelGGL(k_set_resid<2>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
                        s.G, X, s.dobs, s.ivar, s.w);
   else
-    hipLaunchKernelGGL(k_set_resid<4>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.rtiles,
+    hipLaunchKernelGGL(k_set_resid<4>, dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
                        s.G, X, s.dobs, s.ivar, s.w);
   HMC_HIP(hipGetLastError());
   set_time(s, 0, false, e);
 }
 
-static void launch_grad(ProblemSet& s, int64_t ntiles, float* X, int step, int mode) {
+// Premultiplied step of the trajectory: column strips of H (k_set_grad mode 2)
+// or rows of H (k_set_resid_lds with the kick epilogue), 64 * rw rows a tile.
+static void launch_premult_step(ProblemSet& s, int step) {
+  if (!s.hrow) {
+    launch_grad(s, s.xt, step, 2);
+    return;
+  }
+  const int64_t ntiles = s.hprefix[step];
+  const int2* tiles = s.htiles + s.hoff[step];
+  if (ntiles <= 0) return;
+  hipEvent_t e;
+  set_time(s, 1, true, e);
+  if (s.rw == 1)
+    launch_rows_lds<1, 1>(s, ntiles, tiles, s.H, s.xt, step);
+  else if (s.rw == 2)
+    launch_rows_lds<2, 1>(s, ntiles, tiles, s.H, s.xt, step);
+  else
+    launch_rows_lds<4, 1>(s, ntiles, tiles, s.H, s.xt, step);
+  HMC_HIP(hipGetLastError());
+  set_time(s, 1, false, e);
+}
+
+static void launch_grad(ProblemSet& s, float* X, int step, int mode) {
+  const int64_t ntiles = s.gprefix[step];
+  const int2* tiles = s.gtiles + s.goff[step];
   if (ntiles <= 0) return;
   hipEvent_t e;
   set_time(s, 1, true, e);
   if (mode == 0) {
     if (s.nw == 4)
-      hipLaunchKernelGGL((k_set_grad<4, 0>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
+      hipLaunchKernelGGL((k_set_grad<4, 0>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
                          s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr);
     else
-      hipLaunchKernelGGL((k_set_grad<8, 0>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, s.gtiles,
+      hipLaunchKernelGGL((k_set_grad<8, 0>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, tiles,
                          s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr);
   } else if (mode == 2) {
     // X is xt; the premultiplied step alternates between xt and xt2
     if (s.nw == 4)
-      hipLaunchKernelGGL((k_set_grad<4, 2>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
+      hipLaunchKernelGGL((k_set_grad<4, 2>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
                          s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb);
     else
-      hipLaunchKernelGGL((k_set_grad<8, 2>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, s.gtiles,
+      hipLaunchKernelGGL((k_set_grad<8, 2>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, tiles,
                          s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb);
   } else {
-    hipLaunchKernelGGL((k_set_grad<4, 1>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, s.gtiles,
+    hipLaunchKernelGGL((k_set_grad<4, 1>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
                        s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, s.gout, nullptr, nullptr);
   }
   HMC_HIP(hipGetLastError());
@@ -846,10 +929,10 @@ static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
   set_time(s, 3, false, e);
   for (int st = 0; st < s.Lmax; ++st) {
     if (s.pm) {
-      launch_grad(s, s.gprefix[st], s.xt, st, 2);
+      launch_premult_step(s, st);
     } else {
-      launch_resid(s, s.rprefix[st], s.xt);
-      launch_grad(s, s.gprefix[st], s.xt, st, 0);
+      launch_resid(s, st, s.xt);
+      launch_grad(s, s.xt, st, 0);
     }
   }
   launch_endpoint(s, s.xt);
@@ -919,7 +1002,7 @@ static void ensure_premult(ProblemSet& s) {
   if (want != s.pm) {
     s.pm = want;
     s.tuned = false;
-    drop_graph(s);
+    build_tiles(s);   // strip costs follow the form; drops the graph
   }
   if (!s.pm || s.pm_built) return;
   s.H = (float*)dev_alloc(s.owned, s.hm_total * 4);
@@ -938,7 +1021,8 @@ static void ensure_premult(ProblemSet& s) {
                        (const int2*)tl, s.G, s.ivar, s.H);
     HMC_HIP(hipGetLastError());
     // every problem's 64-column tiles (all L >= 1, so the step-0 prefix is the whole list)
-    hipLaunchKernelGGL(k_set_build_b, dim3((unsigned)s.gprefix[0]), dim3(256), 0, s.stream, s.desc, s.gtiles,
+    hipLaunchKernelGGL(k_set_build_b, dim3((unsigned)s.gprefix[0]), dim3(256), 0, s.stream, s.desc,
+                       s.gtiles + s.goff[0],
                        s.G, s.dobs, s.ivar, s.hb);
     HMC_HIP(hipGetLastError());
     HMC_HIP(hipStreamSynchronize(s.stream));
@@ -963,12 +1047,15 @@ static void autotune_tiles(ProblemSet& s) {
   HMC_HIP(hipEventCreate(&a));
   HMC_HIP(hipEventCreate(&b));
   float best = 1e30f;
-  int brw = s.rw, bnw = 4, blds = s.rlds;
+  int brw = s.rw, bnw = 4, blds = s.rlds, bh = s.hrow;
+  // residual form: (rows staged or not, rows per tile, waves per gradient
+  // tile); premultiplied: (strips or rows of H, rows per tile, waves per strip)
   for (int lds : {0, 1})
   for (int rw : {1, 2, 4})
     for (int nw : {4, 8}) {
-      if (s.pm && (lds != s.rlds || rw != s.rw)) continue;   // no residual launches
-      s.rlds = lds;
+      if (s.pm && ((lds == 0 && rw != 1) || (lds == 1 && nw != 4))) continue;   // unused knobs
+      if (s.pm) s.hrow = lds;
+      else s.rlds = lds;
       s.rw = rw;
       s.nw = nw;
       build_tiles(s);
@@ -976,10 +1063,10 @@ static void autotune_tiles(ProblemSet& s) {
         HMC_HIP(hipEventRecord(a, s.stream));
         for (int k = 0; k < 3; ++k) {
           if (s.pm) {
-            launch_grad(s, s.gprefix[0], s.xt, 0, 2);
+            launch_premult_step(s, 0);
           } else {
-            launch_resid(s, s.rprefix[0], s.xt);
-            launch_grad(s, s.gprefix[0], s.xt, 0, 0);
+            launch_resid(s, 0, s.xt);
+            launch_grad(s, s.xt, 0, 0);
           }
         }
         HMC_HIP(hipEventRecord(b, s.stream));
@@ -991,12 +1078,14 @@ static void autotune_tiles(ProblemSet& s) {
         best = ms;
         brw = rw;
         bnw = nw;
-        blds = lds;
+        if (s.pm) bh = lds;
+        else blds = lds;
       }
     }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   s.rlds = blds;
+  s.hrow = bh;
   s.rw = brw;
   s.nw = bnw;
   build_tiles(s);
@@ -1153,7 +1242,8 @@ int hmc_set_problem_prior(hmc_set* h, int64_t i, int kind, const double* a, cons
 // "autotune" (1: time the tile shapes on first use, 0: keep the set ones),
 // "resid_rows" (64, 128, 256 rows per residual tile), "grad_waves" (4 or 8),
 // "resid_lds" (1: residual tiles staged through LDS, 0: read into registers),
-// "premult" (-1 automatic, 0 residual form, 1 premultiplied trajectories).
+// "premult" (-1 automatic, 0 residual form, 1 premultiplied trajectories),
+// "premult_rows" (premultiplied step over 1: rows of H, 0: column strips).
 int hmc_set_set_option(hmc_set* h, const char* name, double value) {
   return guarded("hmc_set_set_option", [&] {
     HMC_REQUIRE(h && name, HMC_ERR_ARG, "NULL argument");
@@ -1168,6 +1258,10 @@ int hmc_set_set_option(hmc_set* h, const char* name, double value) {
     } else if (k == "premult") {
       HMC_REQUIRE(value == -1 || value == 0 || value == 1, HMC_ERR_ARG, "premult must be -1, 0 or 1");
       s.premult = (int)value;
+    } else if (k == "premult_rows") {
+      HMC_REQUIRE(value == 0 || value == 1, HMC_ERR_ARG, "premult_rows must be 0 or 1");
+      s.hrow = (int)value;
+      s.autotune = 0;
     } else if (k == "resid_lds") {
       HMC_REQUIRE(value == 0 || value == 1, HMC_ERR_ARG, "resid_lds must be 0 or 1");
       s.rlds = (int)value;
@@ -1193,6 +1287,7 @@ int hmc_set_get_option(hmc_set* h, const char* name, double* value) {
     if (k == "resid_rows") *value = 64.0 * s.rw;
     else if (k == "grad_waves") *value = s.nw;
     else if (k == "resid_lds") *value = s.rlds;
+    else if (k == "premult_rows") *value = s.hrow;
     else if (k == "autotune") *value = s.autotune;
     else if (k == "premult") *value = s.pm ? 1.0 : 0.0;
     else throw Error{HMC_ERR_ARG, "unknown option '" + k + "'"};
@@ -1271,9 +1366,8 @@ int hmc_set_gradient(hmc_set* h, const double* X, double* out) {
     HMC_HIP(hipMemcpyAsync(s.xin, X, s.h_total * 8, hipMemcpyHostToDevice, s.stream));
     hipLaunchKernelGGL(k_set_load, dim3(8, (unsigned)s.chains()), dim3(256), 0, s.stream, s.desc, s.xin, s.xt,
                        s.h_total);
-    const int64_t nr = s.rprefix.empty() ? 0 : s.rprefix[0], ng = s.gprefix.empty() ? 0 : s.gprefix[0];
-    launch_resid(s, nr, s.xt);
-    launch_grad(s, ng, s.xt, 0, 1);
+    launch_resid(s, 0, s.xt);
+    launch_grad(s, s.xt, 0, 1);
     HMC_HIP(hipMemcpyAsync(out, s.gout, s.h_total * 8, hipMemcpyDeviceToHost, s.stream));
     HMC_HIP(hipStreamSynchronize(s.stream));
   });

@@ -76,15 +76,17 @@ def test_misfit_and_gradient_match_oracle():
             assert _rel(g[p][c], o.gradient(x)[:, 0]) < 1e-4, (p, c)
 
 
-@pytest.mark.parametrize("premult", [0, 1])
+@pytest.mark.parametrize("form", ["residual", "premult_strips", "premult_rows"])
 @pytest.mark.parametrize("Ls", [[1, 1, 1, 1, 1, 1], [3, 12, 1, 7, 5, 9]])
-def test_teacher_forced_proposal_matches_oracle(Ls, premult):
+def test_teacher_forced_proposal_matches_oracle(Ls, form):
     """Residual-form and premultiplied trajectories (H = G^T C^-1 G built in
-    FP64, stored fp32; the oracle keeps the residual form) against the
-    oracle; odd and even L exercise both state buffers of the premultiplied
-    form."""
+    FP64, stored fp32, read as column strips or as rows; the oracle keeps the
+    residual form) against the oracle; odd and even L exercise both state
+    buffers of the premultiplied form."""
     ps, orc, rng = _build(2, DIMS, RATIOS)
+    premult = int(form != "residual")
     ps.set_option("premult", premult)
+    ps.set_option("premult_rows", int(form == "premult_rows"))
     ps.set_state(ps.join([rng.normal(size=(CHAINS, d)) * 0.2 for d in DIMS]))
     X0, chi0 = ps.get_state()
     # momenta live in float32 on the device: give the oracle the same values
@@ -114,11 +116,15 @@ def test_teacher_forced_proposal_matches_oracle(Ls, premult):
 
 @pytest.mark.parametrize("premult", [0, 1])
 def test_graph_replays_match_direct_launches_and_tile_shapes(premult):
+    """Graph replays against direct launches (bitwise), then other tile
+    shapes: rows per tile and waves per strip, and for the premultiplied form
+    rows of H against strips of H (the first two use rows of H)."""
     dims = [40, 200, 75, 310]
     out = []
-    for graph, rows, waves in [(True, 64, 4), (False, 64, 4), (True, 128, 8), (True, 256, 4)]:
+    for graph, rows, waves, hrow in [(True, 64, 4, 1), (False, 64, 4, 1), (True, 128, 8, 0), (True, 256, 4, 1)]:
         ps, _, rng = _build(3, dims)
         ps.set_option("premult", premult)
+        ps.set_option("premult_rows", hrow)
         ps.set_option("resid_rows", rows)
         ps.set_option("grad_waves", waves)
         X0, step = _start_and_steps(ps)
