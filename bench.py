@@ -606,7 +606,9 @@ def run_uneven(args):
         # one pass over H (d x d fp32) per step; x and p read and written
         traj_bytes = float(np.sum(L * (4.0 * d * d + 256.0 * d)))
         actual = traj_bytes / (step_ms / 1e3)
-        kernel = "k_set_grad premultiplied (g = H x - b, kick/drift; all trajectory launches of one round)"
+        kernel = ((f"k_set_resid_lds<{tiles['resid_rows'] // 64},1,{tiles['lds_chunk']}> (rows of H)"
+                   if tiles["premult_rows"] else f"k_set_grad<{tiles['grad_waves']},2> (column strips of H)")
+                  + ": g = H x - b, kick/drift; all trajectory launches of one round")
     else:
         # the algorithm needs one pass of G per step; the residual and gradient launches make two
         traj_bytes = float(np.sum(L * (4.0 * n * d + 256.0 * d)))

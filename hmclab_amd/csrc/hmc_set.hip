@@ -169,12 +169,11 @@ __global__ __launch_bounds__(256) void k_set_resid(const SetDesc* __restrict__ d
 
 
 // ---------------------------------------------------------------------------
-// residual, LDS-staged variant: the tile's 64*RW rows of G are read 64 columns
-// at a time, 16 lanes per row (256 contiguous bytes per row, as the gradient
-// tiles read G), into LDS through registers (the next chunk's loads are in
-// flight while the current chunk's MFMAs run); the MFMA sequence per
-// accumulator is the one of k_set_resid (k = 16 s + 4 q + e), so both give
-// identical results.
+// residual, LDS-staged variant: the tile's 64*RW rows of G are read KC (64 or
+// 128) columns at a time, KC/4 lanes per row (256 or 512 contiguous bytes per
+// row), into LDS through registers (the next chunk's loads are in flight
+// while the current chunk's MFMAs run); the MFMA sequence per accumulator is
+// the one of k_set_resid (k = 16 s + 4 q + e), so all give identical results.
 //
 // KICK = 1: the premultiplied step on rows of H instead of G.  H is symmetric,
 // so the tile's rows j give (H x)_j = sum_k H[j][k] x[k] for its 64*RW
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(256) void k_set_resid(const SetDesc* __restrict__ d
 // k_set_grad (g = H x - b + prior'(x), x read from buffer (L - step) mod 2 of
 // X / X2 and written to the other).
 // ---------------------------------------------------------------------------
-template <int RW, int KICK>
+template <int RW, int KICK, int KC>
 __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict__ desc,
                                                        const int2* __restrict__ tiles,
                                                        const float* __restrict__ G,
@@ -195,8 +194,11 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
                                                        const double* __restrict__ pb,
                                                        const double* __restrict__ hb,
                                                        const double* __restrict__ eps, int step) {
-  constexpr int ROWS = 64 * RW, KC = 64, LD = KC + 4;
-  constexpr int GPT = ROWS * (KC / 4) / 256;   // float4 loads of G per thread per chunk
+  constexpr int ROWS = 64 * RW, LD = KC + 4;
+  constexpr int LPR = KC / 4, RPP = 256 / LPR;   // loader lanes per row, rows per pass
+  constexpr int GPT = ROWS / RPP;                // float4 loads of G per thread per chunk
+  constexpr int XPT = SC / RPP;                  // ... of x (KC = 64: 1, 128: 2)
+  static_assert(KC == 64 || KC == 128, "chunk width");
   __shared__ __attribute__((aligned(16))) float Gs[ROWS * LD];
   __shared__ __attribute__((aligned(16))) float Xs[SC * LD];
   const int2 t = tiles[blockIdx.x];
@@ -210,23 +212,27 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
   const bool flip = KICK && ((D.L - step) & 1);
   const float* Xin = flip ? X2 : X;
   const float* Xp = Xin + D.x_off;
-  // loader mapping: thread tid covers column 4 * (tid % 16) of rows tid / 16 + 16 j
-  const int lc = 4 * (tid & 15), lr = tid >> 4;
-  floatx4 gr[GPT], xr;
+  // loader mapping: thread tid covers column 4 * (tid % LPR) of rows tid / LPR + RPP j
+  const int lc = 4 * (tid % LPR), lr = tid / LPR;
+  floatx4 gr[GPT], xr[XPT];
   auto gload = [&](int k0) {
     const int col = k0 + lc;
 #pragma unroll
     for (int j = 0; j < GPT; ++j) {
-      const int row = r0 + lr + 16 * j;
+      const int row = r0 + lr + RPP * j;
       gr[j] = (row < nrow && col < D.ldg) ? *(const floatx4*)(Gp + (int64_t)row * D.ldg + col)
                                            : floatx4{0.f, 0.f, 0.f, 0.f};
     }
-    xr = (col < D.ldg) ? *(const floatx4*)(Xp + (int64_t)lr * D.ldg + col) : floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < XPT; ++j)
+      xr[j] = (col < D.ldg) ? *(const floatx4*)(Xp + (int64_t)(lr + RPP * j) * D.ldg + col)
+                            : floatx4{0.f, 0.f, 0.f, 0.f};
   };
   auto sstore = [&]() {
 #pragma unroll
-    for (int j = 0; j < GPT; ++j) *(floatx4*)(&Gs[(lr + 16 * j) * LD + lc]) = gr[j];
-    *(floatx4*)(&Xs[lr * LD + lc]) = xr;
+    for (int j = 0; j < GPT; ++j) *(floatx4*)(&Gs[(lr + RPP * j) * LD + lc]) = gr[j];
+#pragma unroll
+    for (int j = 0; j < XPT; ++j) *(floatx4*)(&Xs[(lr + RPP * j) * LD + lc]) = xr[j];
   };
   floatx4 acc[RW];
   bool act[RW];
@@ -703,6 +709,7 @@ struct ProblemSet {
   int rw = 1, nw = 4;             // resid rows per wave (x16), grad waves per tile
   int rlds = 0;                   // residual tiles staged through LDS (k_set_resid_lds)
   int hrow = 0;                   // premultiplied step: 0 column strips (k_set_grad), 1 rows of H
+  int kc = 64;                    // columns per LDS chunk of the staged row tiles (64 or 128)
   // premultiplied trajectories: -1 automatic, 0 off, 1 on; pm = resolved, built
   int premult = -1;
   bool pm = false, pm_built = false;
@@ -825,12 +832,22 @@ static void build_tiles(ProblemSet& s) {
 
 static void launch_grad(ProblemSet& s, float* X, int step, int mode);
 
+template <int RW, int KICK, int KC>
+static void launch_rows_kc(ProblemSet& s, int64_t ntiles, const int2* tiles, const float* Gop, float* X,
+                           int step) {
+  hipLaunchKernelGGL((k_set_resid_lds<RW, KICK, KC>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc,
+                     tiles, Gop, X, s.dobs, s.ivar, s.w, KICK ? s.xt2 : nullptr, KICK ? s.pt : nullptr, s.pa,
+                     s.pb, KICK ? s.hb : nullptr, s.eps, step);
+}
+
+// chunk width s.kc (64 or 128 columns); 128 only for RW <= 2 (LDS size)
 template <int RW, int KICK>
 static void launch_rows_lds(ProblemSet& s, int64_t ntiles, const int2* tiles, const float* Gop, float* X,
                             int step) {
-  hipLaunchKernelGGL((k_set_resid_lds<RW, KICK>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
-                     Gop, X, s.dobs, s.ivar, s.w, KICK ? s.xt2 : nullptr, KICK ? s.pt : nullptr, s.pa, s.pb,
-                     KICK ? s.hb : nullptr, s.eps, step);
+  if (s.kc == 128 && RW <= 2)
+    launch_rows_kc<RW, KICK, (RW <= 2 ? 128 : 64)>(s, ntiles, tiles, Gop, X, step);
+  else
+    launch_rows_kc<RW, KICK, 64>(s, ntiles, tiles, Gop, X, step);
 }
 
 static void launch_resid(ProblemSet& s, int step, float* X) {
@@ -1047,15 +1064,27 @@ static void autotune_tiles(ProblemSet& s) {
   HMC_HIP(hipEventCreate(&a));
   HMC_HIP(hipEventCreate(&b));
   float best = 1e30f;
-  int brw = s.rw, bnw = 4, blds = s.rlds, bh = s.hrow;
-  // residual form: (rows staged or not, rows per tile, waves per gradient
-  // tile); premultiplied: (strips or rows of H, rows per tile, waves per strip)
+  int brw = s.rw, bnw = 4, blds = s.rlds, bh = s.hrow, bkc = s.kc;
+  // residual form: (rows staged or not, LDS chunk, rows per tile, waves per
+  // gradient tile); premultiplied: (strips or rows of H, LDS chunk, rows per
+  // tile, waves per strip)
+  struct Cand {
+    int lds, kc, rw, nw;
+  };
+  std::vector<Cand> cands;
   for (int lds : {0, 1})
-  for (int rw : {1, 2, 4})
-    for (int nw : {4, 8}) {
-      if (s.pm && ((lds == 0 && rw != 1) || (lds == 1 && nw != 4))) continue;   // unused knobs
+    for (int kc : {64, 128})
+      for (int rw : {1, 2, 4})
+        for (int nw : {4, 8}) {
+          if (kc == 128 && (!lds || rw > 2)) continue;
+          if (s.pm && ((lds == 0 && rw != 1) || (lds == 1 && nw != 4))) continue;   // unused knobs
+          cands.push_back({lds, kc, rw, nw});
+        }
+  for (const Cand& cd : cands) {
+      const int lds = cd.lds, rw = cd.rw, nw = cd.nw;
       if (s.pm) s.hrow = lds;
       else s.rlds = lds;
+      s.kc = cd.kc;
       s.rw = rw;
       s.nw = nw;
       build_tiles(s);
@@ -1078,14 +1107,16 @@ static void autotune_tiles(ProblemSet& s) {
         best = ms;
         brw = rw;
         bnw = nw;
+        bkc = cd.kc;
         if (s.pm) bh = lds;
         else blds = lds;
       }
-    }
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   s.rlds = blds;
   s.hrow = bh;
+  s.kc = bkc;
   s.rw = brw;
   s.nw = bnw;
   build_tiles(s);
@@ -1243,7 +1274,9 @@ int hmc_set_problem_prior(hmc_set* h, int64_t i, int kind, const double* a, cons
 // "resid_rows" (64, 128, 256 rows per residual tile), "grad_waves" (4 or 8),
 // "resid_lds" (1: residual tiles staged through LDS, 0: read into registers),
 // "premult" (-1 automatic, 0 residual form, 1 premultiplied trajectories),
-// "premult_rows" (premultiplied step over 1: rows of H, 0: column strips).
+// "premult_rows" (premultiplied step over 1: rows of H, 0: column strips),
+// "lds_chunk" (64 or 128 columns per LDS chunk of the staged row tiles; 128
+// applies to 64- and 128-row tiles).
 int hmc_set_set_option(hmc_set* h, const char* name, double value) {
   return guarded("hmc_set_set_option", [&] {
     HMC_REQUIRE(h && name, HMC_ERR_ARG, "NULL argument");
@@ -1258,6 +1291,10 @@ int hmc_set_set_option(hmc_set* h, const char* name, double value) {
     } else if (k == "premult") {
       HMC_REQUIRE(value == -1 || value == 0 || value == 1, HMC_ERR_ARG, "premult must be -1, 0 or 1");
       s.premult = (int)value;
+    } else if (k == "lds_chunk") {
+      HMC_REQUIRE(value == 64 || value == 128, HMC_ERR_ARG, "lds_chunk must be 64 or 128");
+      s.kc = (int)value;
+      s.autotune = 0;
     } else if (k == "premult_rows") {
       HMC_REQUIRE(value == 0 || value == 1, HMC_ERR_ARG, "premult_rows must be 0 or 1");
       s.hrow = (int)value;
@@ -1288,6 +1325,7 @@ int hmc_set_get_option(hmc_set* h, const char* name, double* value) {
     else if (k == "grad_waves") *value = s.nw;
     else if (k == "resid_lds") *value = s.rlds;
     else if (k == "premult_rows") *value = s.hrow;
+    else if (k == "lds_chunk") *value = s.kc;
     else if (k == "autotune") *value = s.autotune;
     else if (k == "premult") *value = s.pm ? 1.0 : 0.0;
     else throw Error{HMC_ERR_ARG, "unknown option '" + k + "'"};

@@ -83,7 +83,8 @@ class ProblemSet:
         r, w = C.c_int(0), C.c_int(0)
         call("hmc_set_tile_config", self._h, C.byref(r), C.byref(w))
         return dict(resid_rows=r.value, grad_waves=w.value, resid_lds=int(self.get_option("resid_lds")),
-                    premult=int(self.get_option("premult")), premult_rows=int(self.get_option("premult_rows")))
+                    premult=int(self.get_option("premult")), premult_rows=int(self.get_option("premult_rows")),
+                    lds_chunk=int(self.get_option("lds_chunk")))
 
     # ---- layout helpers ----------------------------------------------------------
     def split(self, flat):

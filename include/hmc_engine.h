@@ -320,9 +320,11 @@ int hmc_set_problem_prior(hmc_set* s, int64_t problem, int kind, const double* a
  * -1 (default) chooses it when one pass over every H reads fewer bytes than
  * the two passes over every G.  The FP64 accept test keeps the residual form;
  * "premult_rows" 1|0: the premultiplied step streams rows of H (H symmetric)
- * or 64-column strips of it (fix it, no autotune). */
+ * or 64-column strips of it; "lds_chunk" 64|128: columns per LDS chunk of the
+ * staged row tiles (128 for 64- and 128-row tiles; same results) (fix them,
+ * no autotune). */
 int hmc_set_set_option(hmc_set* s, const char* name, double value);
-/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "autotune", "premult"
+/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "lds_chunk", "autotune", "premult"
  * (the last is decided at the first run or proposal).  No reference
  * counterpart (engine configuration). */
 int hmc_set_get_option(hmc_set* s, const char* name, double* value);

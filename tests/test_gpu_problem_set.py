@@ -148,18 +148,42 @@ def test_graph_replays_match_direct_launches_and_tile_shapes(premult):
 
 @pytest.mark.parametrize("rows", [64, 128, 256])
 def test_lds_staged_residual_is_bitwise_identical(rows):
-    """Residual tiles staged through LDS run the same MFMA sequence per
-    accumulator as the register-fed tiles: states, misfits and decisions agree
-    bit for bit (d = 75 and 310 end in partial 64-column chunks, n = 465 in a
-    partial tile)."""
+    """Residual tiles staged through LDS (64- or 128-column chunks; 256-row
+    tiles use 64) run the same MFMA sequence per accumulator as the
+    register-fed tiles: states, misfits and decisions agree bit for bit
+    (d = 75 and 310 end in partial chunks, n = 465 in a partial tile)."""
     dims = [40, 200, 75, 310]
     out = []
-    for lds in (0, 1):
+    for lds, chunk in ((0, 64), (1, 64), (1, 128)):
         ps, _, _ = _build(3, dims)
         ps.set_option("resid_rows", rows)
         ps.set_option("grad_waves", 4)
         ps.set_option("resid_lds", lds)
+        ps.set_option("lds_chunk", chunk)
         ps.set_option("premult", 0)
+        X0, step = _start_and_steps(ps)
+        ps.set_state(X0)
+        acc = ps.run(4, [6, 11, 3, 8], step, seed=4)
+        X1, chi1 = ps.get_state()
+        out.append((acc, X1, chi1))
+        ps.close()
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0])
+        assert np.array_equal(out[0][1], o[1]) and np.array_equal(out[0][2], o[2])
+
+
+@pytest.mark.parametrize("rows", [64, 128, 256])
+def test_premultiplied_row_tiles_chunk_widths_are_bitwise_identical(rows):
+    """Rows of H through LDS in 64- or 128-column chunks: same MFMA sequence,
+    same bits (d = 75 and 310 end in partial chunks and tiles)."""
+    dims = [40, 200, 75, 310]
+    out = []
+    for chunk in (64, 128):
+        ps, _, _ = _build(3, dims)
+        ps.set_option("premult", 1)
+        ps.set_option("premult_rows", 1)
+        ps.set_option("resid_rows", rows)
+        ps.set_option("lds_chunk", chunk)
         X0, step = _start_and_steps(ps)
         ps.set_state(X0)
         acc = ps.run(4, [6, 11, 3, 8], step, seed=4)
