@@ -6,13 +6,22 @@ N(0, I), noise sigma 0.01, Normal prior std 1, Unit mass, 512 chains,
 25 leapfrog steps.  Two legs run on the same target and the same exact
 posterior draws as starting states:
 
-* plain batched FP64 GPU HMC (the comparator): trajectory GEMMs on FP64 MFMA;
+* plain batched FP64 GPU HMC (the comparator): trajectory GEMMs on FP64 MFMA
+  with the measured leg's trajectory form, and (default) also the
+  residual-form FP64 comparator;
 * surrogate+batched (the reported value): trajectory GEMMs on bf16 MFMA
   (hi/lo split operands, FP32 accumulation), endpoint Hamiltonian and
   Metropolis in FP64; an MLP surrogate of the likelihood (8192 -> 512 -> 512
   -> 1, SiLU) is trained on the GPU on (m, misfit, gradient) pairs from
   warm-up proposals and drives the trajectory until a chunk's acceptance
   falls below 0.5, after which the exact gradient takes over.
+
+Both legs default to premultiplied trajectories, the reference's algebra for
+n > d (LinearMatrix.py:163-177): H = G^T C^-1 G and b = G^T C^-1 d are built
+once in FP64; the bf16 leg keeps y = H x - b in float32 (hi/lo planes of H and
+x at a trajectory's first stage, then y += H_hi dx per stage), the FP64 leg
+computes g = H x - b per stage.  --premult-trajectory 0 runs both legs in the
+residual form (g = G^T (G x - d) / var).
 
 A "step" is one proposal for all chains.  W warm-up proposals run untimed,
 K proposals are timed; min-ESS is the rank-normalised split bulk ESS
@@ -21,8 +30,8 @@ errors of both legs are against the analytic FP64 posterior of the
 bf16-rounded G ("matched posterior error").
 
 The roofline line is measured live with HIP events on the engine's stream
-around every trajectory GEMM launch (algorithmic FLOP per launch =
-2 n d B).  The CPU baseline times the oracle (the reference's op sequence:
+around every trajectory GEMM launch (algorithmic FLOP per stage = 2 d^2 B
+premultiplied, 2 n d B in the residual form).  The CPU baseline times the oracle (the reference's op sequence:
 L gradients + 2 misfits per proposal, residual form, float64, one core) on
 a bounded sample and converts with the GPU's measured ESS per proposal.
 """
@@ -190,13 +199,16 @@ def _trajectory_gradient_error(post, prob, X, precision):
     return float(np.max(np.max(np.abs(g - g64), axis=1) / np.max(np.abs(g64), axis=1)))
 
 
-def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np):
-    """Comparator: plain batched FP64 GPU HMC (exact gradients on FP64 MFMA)."""
+def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, premult):
+    """Comparator: plain batched FP64 GPU HMC (exact gradients on FP64 MFMA),
+    trajectories premultiplied (g = H x - b, the reference's form for n > d)
+    or in the residual form (g = G^T (G x - d) / var)."""
     from hmclab_amd.Distributions import build_problem
     from hmclab_amd.engine import Chains
 
     B, L, d = args.chains, args.steps_per_traj, args.d
-    prob = build_problem(post, precision="f64", engine="gemm")
+    prob = build_problem(post, precision="f64", engine="gemm",
+                         options={"premult_trajectory": 1} if premult else None)
     ch = Chains(prob, B)
     ch.set_state(X0)
     sub = argparse.Namespace(**vars(args))
@@ -207,9 +219,11 @@ def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np):
     kt = _kernel_times(ch)
     ch.close()
     prob.close()
-    log(f"plain fp64: {args.plain_steps} proposals in {elapsed:.2f}s, acceptance {acc.mean() / args.plain_steps:.3f}, "
+    log(f"plain fp64 ({'premultiplied' if premult else 'residual form'}): {args.plain_steps} proposals in "
+        f"{elapsed:.2f}s, acceptance {acc.mean() / args.plain_steps:.3f}, "
         f"min ESS {min_ess:.0f} -> {min_ess / elapsed:.1f} ESS/s")
-    return dict(value=min_ess / elapsed, min_ess=min_ess, seconds=elapsed, proposals=args.plain_steps,
+    return dict(trajectory_form="premultiplied" if premult else "residual",
+                value=min_ess / elapsed, min_ess=min_ess, seconds=elapsed, proposals=args.plain_steps,
                 acceptance_rate=float(acc.mean() / args.plain_steps), stepsize=eps,
                 posterior_rms_mean_err_over_sd=em, posterior_rms_std_rel_err=es,
                 ms_per_proposal=elapsed / args.plain_steps * 1e3, _mean=mean, _std=std,
@@ -277,7 +291,12 @@ def run_dense(args):
     else:
         prior = D.Normal(np.zeros((d, 1)), 1.0)
     post = D.BayesRule([prior, D.LinearMatrix(Gf.astype(np.float64), dd, sigma * sigma, dtype="bfloat16")])
-    plain = _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np) if args.plain_steps > 0 else None
+    # comparator with the measured leg's trajectory form; with the
+    # premultiplied form also the residual-form FP64 comparator (--plain-residual)
+    plain = (_plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, args.premult_trajectory)
+             if args.plain_steps > 0 else None)
+    plain_res = (_plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, 0)
+                 if args.plain_steps > 0 and args.premult_trajectory and args.plain_residual else None)
     if plain is not None and mu_np is None:
         mu_np, sd_np = plain["_mean"], plain["_std"]
 
@@ -294,6 +313,8 @@ def run_dense(args):
             opts["tile_order"] = args.tile_order
         if not args.split_tail:
             opts["gemm_split_tail"] = 0
+    if args.premult_trajectory:
+        opts["premult_trajectory"] = 1
     prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)
@@ -340,7 +361,11 @@ def run_dense(args):
     log(f"timed {args.steps} proposals in {elapsed:.3f}s; residual gemm {res_ms:.1f} ms/{res_n}, "
         f"kick-drift gemm {leap_ms:.1f} ms/{leap_n}, endpoint {end_ms:.1f} ms/{end_n}, "
         f"elementwise {ew_ms:.1f} ms/{ew_n}")
-    if leap_ms >= res_ms:
+    if args.premult_trajectory:
+        gemm_ms, gemm_n, gemm_key = leap_ms, leap_n, "premult"
+        gemm_name = ("k_gemm_bf16<...,EpiLeap> + k_split_epilogue (premultiplied stage: y += H dx, kick/drift; "
+                     "a trajectory's first stage adds the hi/lo y = H x - b GEMMs to its timed region)")
+    elif leap_ms >= res_ms:
         gemm_ms, gemm_n, gemm_key = leap_ms, leap_n, "gradient"
         gemm_name = ("k_gemm_bf16<...,EpiLeap> + k_split_epilogue (gradient GEMM + kick/drift; "
                      "the timed region holds both kernels)")
@@ -352,7 +377,8 @@ def run_dense(args):
     acc_rate = float(acc.mean() / args.steps)
     value = min_ess / elapsed
 
-    flop_launch = 2.0 * n * d * B
+    # algorithmic FLOP of one stage: the gradient's GEMM (K = n residual form, K = d premultiplied)
+    flop_launch = 2.0 * d * d * B if args.premult_trajectory else 2.0 * n * d * B
     avg_s = gemm_ms / 1e3 / max(gemm_n, 1)
     achieved = flop_launch / avg_s
     peak = PEAK["bf16"] if args.precision == "bf16" else PEAK[args.precision]
@@ -367,7 +393,8 @@ def run_dense(args):
             ran = {"gemm_config": args.gemm_config, "gemm_config_grad": args.gemm_config_grad,
                    "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
                    "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
-                   "split_tail": args.split_tail, "prior": args.prior}
+                   "split_tail": args.split_tail, "prior": args.prior,
+                   "premult_trajectory": args.premult_trajectory}
             if all(tj.get("options", {}).get(k, object()) == v for k, v in ran.items()):
                 traffic = tj["bytes_per_launch"].get(gemm_key)
         except Exception:
@@ -378,7 +405,9 @@ def run_dense(args):
                  else "Normal(0, 1); errors are against the analytic fp64 posterior",
         "workload": f"dense linear {n}x{d} ({args.prior} prior), G stored bf16, {B} chains, L={L}: surrogate+batched "
                     f"({args.precision} trajectory GEMMs, fp64 endpoint Metropolis, MLP surrogate with "
-                    f"fallback) vs plain batched fp64 GPU HMC",
+                    f"fallback) vs plain batched fp64 GPU HMC; both legs "
+                    + ("premultiplied (H = G^T C^-1 G, the reference's form for n > d)" if args.premult_trajectory
+                       else "in the residual form"),
         "chains": B, "n_data": n, "n_params": d, "leapfrog_steps": L, "stepsize": eps,
         "min_ess": min_ess, "acceptance_rate": acc_rate,
         "posterior_rms_mean_err_over_sd": err_mean, "posterior_rms_std_rel_err": err_std,
@@ -387,7 +416,7 @@ def run_dense(args):
         "gemm": {"config": args.gemm_config, "config_grad": args.gemm_config_grad,
                  "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
                  "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
-                 "split_tail": args.split_tail},
+                 "split_tail": args.split_tail, "premult_trajectory": args.premult_trajectory},
     }
     if sur_info is not None:
         sur_info.update(chunk_acceptance=sur_acc, fallback_after_proposals=fallback_at,
@@ -398,6 +427,11 @@ def run_dense(args):
         plain.pop("_std", None)
         cfg["plain_fp64"] = plain
         cfg["speedup_vs_plain_fp64"] = value / plain["value"]
+    if plain_res is not None:
+        plain_res.pop("_mean", None)
+        plain_res.pop("_std", None)
+        cfg["plain_fp64_residual_form"] = plain_res
+        cfg["speedup_vs_plain_fp64_residual_form"] = value / plain_res["value"]
     result = {
         "metric": METRIC,
         "value": value,
@@ -743,6 +777,11 @@ def main():
     ap.add_argument("--split-tail", type=int, default=1, choices=[0, 1],
                     help="bf16 GEMMs: split the last partial round of tiles along K")
     ap.add_argument("--tile-order", type=int, default=0, choices=[0, 1])
+    ap.add_argument("--plain-residual", type=int, default=1, choices=[0, 1],
+                    help="with premultiplied trajectories, also time the residual-form FP64 comparator")
+    ap.add_argument("--premult-trajectory", type=int, default=1, choices=[0, 1],
+                    help="trajectories of both dense legs through H = G^T C^-1 G (the reference's "
+                         "premultiplication, its default when n > d); 0: residual form")
     ap.add_argument("--incremental-residual", type=int, default=1, choices=[0, 1],
                     help="bf16: advance a float32 residual by G dx after the first stage")
     ap.add_argument("--gemm-config-grad", type=int, default=None, choices=[0, 1, 2, 3],

@@ -640,6 +640,8 @@ struct EpiLeap {
   DevModel md;
   int M, N;
   int delta_planes;     // planes hold x_new - x_old (residual increments) instead of x
+  float* y;             // premultiplied bf16 trajectories: y += acc, g = y (float32, ld) (or null)
+  const double* bias;   // premultiplied FP64 trajectories: g = acc - bias[j] (or null)
   __device__ __forceinline__ void step(int64_t j, T& xv, T& pv, T g, T kk, T dd, T dd2) const {
     g += prior_grad<T>(md, j, xv);
     if (out_of_bounds(md, j, (double)xv)) g = (T)INFINITY;
@@ -663,12 +665,19 @@ struct EpiLeap {
         const int64_t o = n * ld + m0;
         floatx4 x4 = *reinterpret_cast<const floatx4*>(&x[o]);
         floatx4 p4 = *reinterpret_cast<const floatx4*>(&p[o]);
+        floatx4 g4 = {(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+        if (y) {
+          g4 += *reinterpret_cast<const floatx4*>(&y[o]);
+          *reinterpret_cast<floatx4*>(&y[o]) = g4;
+        }
         uint32_t h[4], l[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           T xv = x4[k], pv = p4[k];
           const T xold = xv;
-          step(m0 + k, xv, pv, (T)v[k], kk, dd, dd2);
+          T g = (T)g4[k];
+          if (bias) g -= (T)bias[m0 + k];
+          step(m0 + k, xv, pv, g, kk, dd, dd2);
           x4[k] = xv;
           p4[k] = pv;
           const float f = delta_planes ? (float)(xv - xold) : (float)xv;
@@ -691,7 +700,14 @@ struct EpiLeap {
       const int64_t o = n * ld + j;
       T xv = x[o], pv = p[o];
       const T xold = xv;
-      step(j, xv, pv, (T)v[k], kk, dd, dd2);
+      T g = (T)v[k];
+      if (y) {
+        const float yv = y[o] + (float)v[k];
+        y[o] = yv;
+        g = (T)yv;
+      }
+      if (bias) g -= (T)bias[j];
+      step(j, xv, pv, g, kk, dd, dd2);
       x[o] = xv;
       p[o] = pv;
       if (xh) {
@@ -700,6 +716,25 @@ struct EpiLeap {
         xh[o] = h;
         if (xl) xl[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
       }
+    }
+  }
+};
+
+// first stage of a premultiplied bf16 trajectory: y = acc - b (float32)
+struct EpiPmStart {
+  static constexpr bool kReduce = false;
+  float* y;
+  int64_t ld;
+  const double* b;
+  int M, N;
+  template <typename V>
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t m = m0 + (int64_t)e * ms;
+      if (m >= M) return;
+      y[n * ld + m] = (float)((double)v[e] - b[m]);
     }
   }
 };
@@ -913,6 +948,7 @@ size_t gemm_split_workspace(const Problem& p, int64_t B) {
   for (int planes = 1; planes <= 2; ++planes) {
     e = std::max(e, split_elems_cfg(p.gemm_cfg, planes, n, (int)B, (int)p.ldg, p.n_cu));
     e = std::max(e, split_elems_cfg(p.gemm_cfg_grad, planes, d, (int)B, (int)p.ldgt, p.n_cu));
+    e = std::max(e, split_elems_cfg(p.gemm_cfg_grad, planes, d, (int)B, (int)p.ldg, p.n_cu));   // H x
   }
   return e;
 }
@@ -921,12 +957,80 @@ int64_t gemm_row_pad() { return 256; }   // largest M-tile of the GEMM configs
 int64_t gemm_k_pad() { return 64; }      // k-tile multiple of every GEMM config
 int64_t gemm_chain_pad() { return 128; } // N-tile of every GEMM config
 
+static void build_premult(Chains& c);
+
+// bf16 hi/lo planes of the FP64 H (same [d_pad][ldg] layout, ldh = ldg)
+__global__ void k_split_planes(const double* __restrict__ H, int64_t count, uint16_t* __restrict__ hh,
+                               uint16_t* __restrict__ hl) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x) {
+    const double v = H[k];
+    const uint16_t h = f32_to_bf16_bits((float)v);
+    hh[k] = h;
+    hl[k] = f32_to_bf16_bits((float)(v - (double)bf16_bits_to_f32(h)));
+  }
+}
+
+// Premultiplied trajectories (LinearMatrix.py:184-207 with premultiplication:
+// gradient = H x - b): the gradient of a stage is one GEMM with K = d instead of
+// the residual and gradient GEMMs with K = d and K = n.
+bool traj_premult(const Problem& p) {
+  return p.premult_traj && endpoint_premult(p) && (p.precision == HMC_BF16 || p.precision == HMC_F64);
+}
+
+static void premult_kick_drift(Chains& c, double kick, double drift, double drift2, bool first) {
+  Problem& p = *c.p;
+  const int d = (int)p.d, B = (int)c.B;
+  const DevModel md = p.model();
+  if (!p.H) build_premult(c);
+  hipEvent_t ev;
+  time_begin(c, 3, ev);
+  if (p.precision == HMC_BF16) {
+    if (!p.Hh) {
+      const int64_t cnt = p.d_pad * p.ldh;
+      p.Hh = (uint16_t*)dev_alloc(p.owned, (size_t)cnt * 2);
+      p.Hl = (uint16_t*)dev_alloc(p.owned, (size_t)cnt * 2);
+      hipLaunchKernelGGL(k_split_planes, dim3((unsigned)std::min<int64_t>((cnt + 255) / 256, 8192)), dim3(256), 0,
+                         p.stream, p.H, cnt, p.Hh, p.Hl);
+      HMC_HIP(hipGetLastError());
+    }
+    const int K = (int)p.ldh;
+    if (first) {
+      // y = H_hi (x_hi + x_lo) + H_lo x_hi - b: the stage's gradient to about
+      // 16 bits of H and x (the float32 sum then cancels against b)
+      launch_bf16(c, p.gemm_cfg_grad, 2, p.Hh, p.ldh, c.xh, c.xl, c.ldx, d, B, K,
+                  EpiPmStart{c.yf, c.ldx, p.hb, d, B});
+      EpiLeap<float> e{(float*)c.xt, (float*)c.pt, c.dxp[0], nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
+                       md, d, B, 1, c.yf, nullptr};
+      launch_bf16(c, p.gemm_cfg_grad, 1, p.Hl, p.ldh, c.xh, nullptr, c.ldx, d, B, K, e);
+      c.dx_cur = 0;
+    } else {
+      // y += H_hi dx with one plane of the last stage's increment dx
+      const int cur = c.dx_cur;
+      EpiLeap<float> e{(float*)c.xt, (float*)c.pt, c.dxp[1 - cur], nullptr, c.ldx, c.eps, c.beta, kick, drift,
+                       drift2, md, d, B, 1, c.yf, nullptr};
+      launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, c.dxp[cur], nullptr, c.ldx, d, B, K, e);
+      c.dx_cur = 1 - cur;
+    }
+  } else {
+    // FP64: g = H x - b from a copy of the positions (the epilogue updates xt)
+    HMC_HIP(hipMemcpyAsync(c.xs, c.xt, (size_t)c.B_pad * c.ldx * 8, hipMemcpyDeviceToDevice, p.stream));
+    EpiLeap<double> e{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
+                      md, d, B, 0, nullptr, p.hb};
+    launch_f64(c, (const double*)p.H, p.ldh, (const double*)c.xs, c.ldx, d, B, (int)p.ldh, e);
+  }
+  time_end(c, 3, ev);
+}
+
 // Trajectory: residual GEMM then gradient GEMM with kick/drift epilogue.
 void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first) {
   Problem& p = *c.p;
   const int n = (int)p.n, d = (int)p.d, B = (int)c.B;
   const DevModel md = p.model();
   hipEvent_t ev1, ev2;
+  if (traj_premult(p)) {
+    premult_kick_drift(c, kick, drift, drift2, first);
+    return;
+  }
   if (p.precision == HMC_BF16) {
     const bool inc = p.resid_update && c.rf;
     time_begin(c, 0, ev1);

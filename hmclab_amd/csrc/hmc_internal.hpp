@@ -45,6 +45,11 @@ struct Problem {
   double* H = nullptr;      // [d_pad][ldh] FP64, built at the first endpoint
   double* hb = nullptr;     // [d_pad] G^T S^-1 d
   int64_t ldh = 0;
+  // trajectories through H as well (dense GEMM engine; needs the
+  // premultiplied endpoint): 0 residual form, 1 premultiplied
+  int premult_traj = 0;
+  uint16_t* Hh = nullptr;   // bf16 hi/lo planes of H [d_pad][ldg] (bf16 trajectories)
+  uint16_t* Hl = nullptr;
   double dtd = 0.0;         // d^T S^-1 d
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 
@@ -105,6 +110,14 @@ struct Chains {
   uint16_t* rl = nullptr;
   void* r = nullptr;        // residual [B_pad][ldr] (state dtype, F64/F32 GEMM)
   float* rf = nullptr;      // float32 running residual G x - d (bf16 GEMM, incremental mode)
+  // premultiplied trajectories: bf16 keeps y = H x - b in float32 [B_pad][ldx]
+  // and the position increments dx in two alternating bf16 planes (a stage's
+  // GEMM reads one while its epilogue writes the other); FP64 reads a copy xs
+  // of the positions while the epilogue updates xt
+  float* yf = nullptr;
+  uint16_t* dxp[2] = {nullptr, nullptr};
+  int dx_cur = 0;
+  void* xs = nullptr;
   double* partial = nullptr;// endpoint partial sums [mblocks][B_pad]
   float* split_ws = nullptr;// bf16 GEMM split-tail partial planes
   size_t split_ws_elems = 0;
@@ -205,6 +218,7 @@ double endpoint_const(const Problem& p);      // constant added to every misfit
 // Trajectory gradient + fused kick/drift for every chain:
 //   p -= kick*eps*g(xt); xt += drift*eps*Minv p; corrector; [xt += drift2*eps*Minv p; corrector]
 void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);
+bool traj_premult(const Problem& p);
 // Endpoint partial sums of the data misfit in FP64 into c.partial.
 void gemm_endpoint(Chains& c, const void* X, int x_dtype);
 // gradient (B x d) of model rows X into out (device, f64, row stride ldx)

@@ -32,8 +32,22 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
+_CACHE = {}
+
+
+def full_problem_cache():
+    """The full-size target, built once per test session."""
+    if "p" not in _CACHE:
+        _CACHE["p"] = _make_full_problem()
+    return _CACHE["p"]
+
+
 @pytest.fixture(scope="module")
 def full_problem():
+    return full_problem_cache()
+
+
+def _make_full_problem():
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(5)
     G = rng.standard_normal(size=(N_DATA, N_PARAMS), dtype=np.float32) * np.float32(1.0 / np.sqrt(N_PARAMS))
@@ -88,3 +102,30 @@ def test_full_size_proposal_hamiltonian(full_problem):
     assert _rel(res["H0"], chi0 + k0) < 1e-9
     ch.close()
     prob.close()
+
+
+def test_full_size_premultiplied_trajectory():
+    """bf16 trajectories through H (y = H x - b from hi/lo planes of H and x,
+    then y += H_hi dx per stage) against residual-form bf16 trajectories from
+    the same start and momenta: end points within 5e-3 of the largest
+    coordinate (both are bf16 approximations of one FP64 trajectory), and
+    the Hamiltonians are the FP64 Hamiltonians of the returned end points."""
+    torch, Gt, dd, post, X = full_problem_cache()
+    B = X.shape[0]
+    rng = np.random.default_rng(7)
+    P0 = rng.standard_normal(size=(B, N_PARAMS)).astype(np.float32).astype(np.float64)
+    ends = []
+    for premult in (0, 1):
+        prob = build_problem(post, precision="bf16", engine="gemm",
+                             options={"gemm_config": 3, "incremental_residual": 1, "bf16_split_grad": 1,
+                                      "premult_trajectory": premult})
+        ch = Chains(prob, B)
+        ch.set_state(X)
+        res = ch.propose(P0, np.full(B, 1e-3), rng.uniform(size=B), 25, "lf", trajectory=True)
+        chi1, _ = _reference(torch, Gt, dd, res["X1"])
+        k1 = 0.5 * np.sum(res["P1"] ** 2, axis=1)
+        assert _rel(res["H1"], chi1 + k1) < 1e-9
+        ends.append(res["X1"])
+        ch.close()
+        prob.close()
+    assert _rel(ends[1], ends[0]) < 5e-3

@@ -109,13 +109,14 @@ def test_bf16_gemm_gradient_and_fp64_misfit(cfg, n, d, B):
         assert _rel(g[b], orc.gradient(x)[:, 0]) < 2e-4
 
 
-@pytest.mark.parametrize("incremental,split_grad", [(0, 2), (1, 2), (1, 1)])
-def test_bf16_trajectory_close_to_fp64(incremental, split_grad):
+@pytest.mark.parametrize("incremental,split_grad,premult", [(0, 2, 0), (1, 2, 0), (1, 1, 0), (0, 2, 1)])
+def test_bf16_trajectory_close_to_fp64(incremental, split_grad, premult):
     """bf16 trajectories (hi/lo positions in the first residual GEMM; with
     incremental_residual the float32 residual then advances by G dx with one
-    bf16 plane of dx) stay within 2e-3 of the FP64 oracle trajectory
-    (relative to the largest coordinate) over 10 leapfrog steps; the
-    Hamiltonians are exact FP64 of those end points."""
+    bf16 plane of dx; premultiplied: y = H x - b from hi/lo planes of H and x
+    at the first stage, then y += H_hi dx) stay within 2e-3 of the FP64 oracle
+    trajectory (relative to the largest coordinate) over 10 leapfrog steps;
+    the Hamiltonians are exact FP64 of those end points."""
     rng = np.random.default_rng(11)
     n, d, B, L = 520, 384, 8, 10
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
@@ -124,7 +125,8 @@ def test_bf16_trajectory_close_to_fp64(incremental, split_grad):
     orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
                        O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
     prob = build_problem(post, precision="bf16",
-                         options={"incremental_residual": incremental, "bf16_split_grad": split_grad})
+                         options={"incremental_residual": incremental, "bf16_split_grad": split_grad,
+                                  "premult_trajectory": premult})
     ch = Chains(prob, B)
     ch.set_state((rng.normal(size=(B, d)) * 0.3).astype(np.float32).astype(np.float64))
     X0, _ = ch.get_state()
@@ -243,10 +245,11 @@ def test_bf16_single_plane_gradient_gemm_tolerance():
 
 
 # --------------------------------------------------------------------------- #
-def _teacher(post, orc, mass_g, mass_o, rng, B, L, integ, precision, engine, eps=0.01, x_scale=0.3):
+def _teacher(post, orc, mass_g, mass_o, rng, B, L, integ, precision, engine, eps=0.01, x_scale=0.3,
+             options=None):
     d = post.dimensions
     prob = build_problem(post, precision=precision, engine=engine,
-                         mass_diagonal=mass_g._device_diagonal())
+                         mass_diagonal=mass_g._device_diagonal(), options=options)
     ch = Chains(prob, B)
     X0 = rng.normal(size=(B, d)) * x_scale
     ch.set_state(X0)
@@ -280,6 +283,18 @@ def test_teacher_forced_proposal_f64(engine, integ, L):
     post, orc, rng = _dense_case(4, 150, 70, prior="both", bounds=True)
     d = post.dimensions
     _teacher(post, orc, MM.Unit(d), O.Unit(d, np.random.default_rng(5)), rng, 6, L, integ, "f64", engine)
+
+
+@pytest.mark.parametrize("integ", ["lf", "3s", "4s"])
+@pytest.mark.parametrize("L", [1, 10])
+def test_teacher_forced_premultiplied_trajectory_f64(integ, L):
+    """FP64 trajectories through H = G^T C^-1 G (one GEMM per stage, g = H x - b,
+    LinearMatrix.py:203-207) against the oracle's residual-form trajectory
+    (n = 150 > d = 70, both priors, bounds), FP64 tolerance."""
+    post, orc, rng = _dense_case(4, 150, 70, prior="both", bounds=True)
+    d = post.dimensions
+    _teacher(post, orc, MM.Unit(d), O.Unit(d, np.random.default_rng(5)), rng, 6, L, integ, "f64", "gemm",
+             options={"premult_trajectory": 1})
 
 
 @pytest.mark.parametrize("engine", ["gemm", "fused"])
