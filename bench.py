@@ -395,8 +395,9 @@ def run_dense(args):
                    "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
                    "split_tail": args.split_tail, "prior": args.prior,
                    "premult_trajectory": args.premult_trajectory}
-            if all(tj.get("options", {}).get(k, object()) == v for k, v in ran.items()):
-                traffic = tj["bytes_per_launch"].get(gemm_key)
+            for prof in tj.get("profiles", []):
+                if all(prof.get("options", {}).get(k, object()) == v for k, v in ran.items()):
+                    traffic = prof["bytes_per_launch"].get(gemm_key)
         except Exception:
             traffic = None
 
