@@ -139,6 +139,14 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 instead of n); default: when n > d, the reference's
  *                 premultiplication rule (LinearMatrix.py:164-167), and H
  *                 takes at most 16 GiB
+ *   "premult_trajectory" 0 or 1: trajectory gradients of the dense GEMM
+ *                 engine through H as well (LinearMatrix.py:203-207,
+ *                 g = H x - b): one GEMM with K = d per stage.  FP64: g from
+ *                 H directly; bf16: a float32 y = H x - b from hi/lo planes
+ *                 of H and x at a trajectory's first stage, then y += H_hi dx
+ *                 with one plane of the position change per stage.  Needs the
+ *                 premultiplied endpoint (chains creation fails otherwise);
+ *                 default 0
  *   "fused_tile_chains" 0, 1, 2 or 4: chains one workgroup of the fused
  *                 engine carries (0 = automatic: 2 if the LDS tile fits,
  *                 else 4 (float32) or 1; a value that does not fit the LDS
