@@ -487,6 +487,9 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
     } else if (key == "endpoint_premult") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "endpoint_premult must be 0 or 1");
       P.premult = (int)value;
+    } else if (key == "endpoint_symmetric") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "endpoint_symmetric must be 0 or 1");
+      P.premult_sym = (int)value;
     } else if (key == "premult_trajectory") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "premult_trajectory must be 0 or 1");
       P.premult_traj = (int)value;

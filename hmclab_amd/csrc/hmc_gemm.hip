@@ -855,6 +855,156 @@ static void launch_f64(const Chains& c, const TA* A, int64_t lda, const TB* Bt, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Premultiplied endpoint over the upper block triangle of the symmetric H, in
+// 128-row blocks a (T of them):
+//   x^T H x - 2 b^T x = sum_a [ x_a^T H_aa x_a - 2 b_a^T x_a + 2 sum_{c>a} x_a^T H_ac x_c ],
+// half the FP64 MFMA work of the full product.  Row blocks a and T-1-a are
+// paired (T + 1 k-blocks of work per pair) and each pair's concatenated
+// k-block list is cut into two halves, so the 2 ceil(T/2) items per chain
+// tile carry equal work.  An item's rows are reduced against x right after
+// each k-range (weight 1 on the diagonal block, which also takes the b term,
+// 2 beyond it), and item i writes partial[i][chain]; the sum over items is
+// the quadratic form.  The k loop is the one of k_gemm_f64.
+// ---------------------------------------------------------------------------
+template <typename TB>
+__global__ __launch_bounds__((F64_BM / F64_WM) * (F64_BN / F64_WN) * 64) void k_quad_sym(
+    const double* __restrict__ H, int64_t ldh, const TB* __restrict__ X, int64_t ldx,
+    const double* __restrict__ hb, int d, int N, double* __restrict__ partial, int64_t ldp) {
+  constexpr int BM = F64_BM, BN = F64_BN, WM = F64_WM, WN = F64_WN, BK = 16, LDS = BK + 2;
+  constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int WPR = BN / WN;
+  constexpr int KB = BM / BK;   // k steps per 128-wide k block
+  constexpr int A_CH = BM * 2 / NT, B_CH = BN * 2 / NT;
+  __shared__ double As[BM * LDS];
+  __shared__ double Bs[BN * LDS];
+  __shared__ double red[BM / WM][4][BN];
+
+  const int T = (d + BM - 1) / BM;
+  const int KT = (int)(ldh / BK);   // k steps of a row of H (ldh is a multiple of 16)
+  const int pr = (int)blockIdx.x >> 1, half = (int)blockIdx.x & 1;
+  const int64_t n_blk = (int64_t)blockIdx.y * BN;
+  const int a1 = pr, a2 = T - 1 - pr;   // a1 <= a2; equal for the middle block of an odd T
+  const int n1 = T - a1, n2 = (a2 != a1) ? T - a2 : 0;
+  const int U = n1 + n2, U0 = (U + 1) / 2;
+  const int lo = half ? U0 : 0, hi = half ? U : U0;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w / WPR, wn = w % WPR;
+  const int q = lane >> 4, r = lane & 15;
+  double ra[A_CH][8], rb[B_CH][8];
+  doublex4 acc[TM][TN];
+  double qs[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) qs[j] = 0.0;
+
+  // acc = H[row block][k blocks kb0..kb1) . X[chain tile][same k]^T
+  auto gemm = [&](int row, int kb0, int kb1) {
+    const double* A = H + (int64_t)row * BM * ldh;
+    const int kt0 = kb0 * KB, kt1 = min(kb1 * KB, KT);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+    auto gload = [&](int kt) {
+#pragma unroll
+      for (int i = 0; i < A_CH; ++i) {
+        const int c = tid + i * NT, rr = c >> 1, col = (c & 1) * 8;
+        ld8(A + (int64_t)rr * ldh + (int64_t)kt * BK + col, ra[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < B_CH; ++i) {
+        const int c = tid + i * NT, rr = c >> 1, col = (c & 1) * 8;
+        ld8(X + (n_blk + rr) * ldx + (int64_t)kt * BK + col, rb[i]);
+      }
+    };
+    gload(kt0);
+    for (int kt = kt0; kt < kt1; ++kt) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < A_CH; ++i) {
+        const int c = tid + i * NT, rr = c >> 1, col = (c & 1) * 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[rr * LDS + col + e] = ra[i][e];
+      }
+#pragma unroll
+      for (int i = 0; i < B_CH; ++i) {
+        const int c = tid + i * NT, rr = c >> 1, col = (c & 1) * 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) Bs[rr * LDS + col + e] = rb[i][e];
+      }
+      __syncthreads();
+      if (kt + 1 < kt1) gload(kt + 1);
+      double af[TM][4], bfr[TN][4];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const double* src = &As[(wm * WM + i * 16 + r) * LDS + 4 * q];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) af[i][s4] = src[s4];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const double* src = &Bs[(wn * WN + j * 16 + r) * LDS + 4 * q];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) bfr[j][s4] = src[s4];
+      }
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i][s4], bfr[j][s4], acc[i][j], 0, 0, 0);
+    }
+  };
+  // qs[j] += sum over the lane's rows m of x[n][m] (wt acc - 2 b_m [diagonal block])
+  auto flush = [&](int row, double wt, bool diag) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int64_t m0 = (int64_t)row * BM + wm * WM + i * 16 + q;
+        const int64_t n = n_blk + wn * WN + j * 16 + r;
+        if (n >= N) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t m = m0 + 4 * v;
+          if (m >= d) continue;
+          const double xv = (double)X[n * ldx + m];
+          qs[j] += xv * (wt * acc[i][j][v] - (diag ? 2.0 * hb[m] : 0.0));
+        }
+      }
+  };
+  for (int sg = 0; sg < 2; ++sg) {
+    const int row = sg ? a2 : a1, base = sg ? n1 : 0, cnt = sg ? n2 : n1;
+    const int s0 = max(lo, base) - base, s1 = min(hi, base + cnt) - base;
+    if (s0 >= s1) continue;
+    int kb0 = row + s0;
+    const int kb1 = row + s1;
+    if (kb0 == row) {
+      gemm(row, kb0, kb0 + 1);
+      flush(row, 1.0, true);
+      ++kb0;
+    }
+    if (kb0 < kb1) {
+      gemm(row, kb0, kb1);
+      flush(row, 2.0, false);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) red[wm][q][wn * WN + j * 16 + r] = qs[j];
+  __syncthreads();
+  for (int col = tid; col < BN; col += NT) {
+    double sum = 0.0;
+#pragma unroll
+    for (int a = 0; a < BM / WM; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) sum += red[a][b][col];
+    if (n_blk + col < N) partial[(int64_t)blockIdx.x * ldp + n_blk + col] = sum;
+  }
+}
+
 static size_t split_elems_f64(int M, int N, int K, int n_cu) {
   const SplitPlan pl = plan_split(F64_BM, F64_BN, M, N, K, n_cu, true, 16);
   return pl.split() ? (size_t)pl.SPL * N * pl.TR : 0;
@@ -1083,7 +1233,9 @@ bool endpoint_premult(const Problem& p) {
 
 int64_t endpoint_blocks(const Chains& c) {
   const Problem& p = *c.p;
-  return endpoint_premult(p) ? (p.d + F64_BM - 1) / F64_BM : (p.n + F64_BM - 1) / F64_BM;
+  if (!endpoint_premult(p)) return (p.n + F64_BM - 1) / F64_BM;
+  const int64_t T = (p.d + F64_BM - 1) / F64_BM;
+  return p.premult_sym ? 2 * ((T + 1) / 2) : T;   // k_quad_sym items, or row blocks of H
 }
 
 double endpoint_const(const Problem& p) { return p.misfit_const + (endpoint_premult(p) ? 0.5 * p.dtd : 0.0); }
@@ -1154,12 +1306,24 @@ void gemm_endpoint(Chains& c, const void* X, int x_dtype) {
     const int d = (int)p.d;
     hipEvent_t ev;
     time_begin(c, 1, ev);
-    if (x_dtype == HMC_F32)
+    if (p.premult_sym) {
+      constexpr int NT = (F64_BM / F64_WM) * (F64_BN / F64_WN) * 64;
+      const int64_t T = (p.d + F64_BM - 1) / F64_BM;
+      const dim3 grid((unsigned)(2 * ((T + 1) / 2)), (unsigned)((B + F64_BN - 1) / F64_BN));
+      if (x_dtype == HMC_F32)
+        hipLaunchKernelGGL(k_quad_sym<float>, grid, dim3(NT), 0, p.stream, (const double*)p.H, p.ldh,
+                           (const float*)X, c.ldx, p.hb, d, B, c.partial, c.B_pad);
+      else
+        hipLaunchKernelGGL(k_quad_sym<double>, grid, dim3(NT), 0, p.stream, (const double*)p.H, p.ldh,
+                           (const double*)X, c.ldx, p.hb, d, B, c.partial, c.B_pad);
+      HMC_HIP(hipGetLastError());
+    } else if (x_dtype == HMC_F32) {
       launch_f64(c, (const double*)p.H, p.ldh, (const float*)X, c.ldx, d, B, (int)p.ldh,
                  EpiQuad<float>{(const float*)X, c.ldx, p.hb, c.partial, c.B_pad, d, B});
-    else
+    } else {
       launch_f64(c, (const double*)p.H, p.ldh, (const double*)X, c.ldx, d, B, (int)p.ldh,
                  EpiQuad<double>{(const double*)X, c.ldx, p.hb, c.partial, c.B_pad, d, B});
+    }
     time_end(c, 1, ev);
     return;
   }

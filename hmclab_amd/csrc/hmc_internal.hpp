@@ -48,6 +48,8 @@ struct Problem {
   // trajectories through H as well (dense GEMM engine; needs the
   // premultiplied endpoint): 0 residual form, 1 premultiplied
   int premult_traj = 0;
+  // premultiplied endpoint over the upper block triangle of H (half the work)
+  int premult_sym = 1;
   uint16_t* Hh = nullptr;   // bf16 hi/lo planes of H [d_pad][ldg] (bf16 trajectories)
   uint16_t* Hl = nullptr;
   double dtd = 0.0;         // d^T S^-1 d

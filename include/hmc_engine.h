@@ -139,6 +139,10 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 instead of n); default: when n > d, the reference's
  *                 premultiplication rule (LinearMatrix.py:164-167), and H
  *                 takes at most 16 GiB
+ *   "endpoint_symmetric" 0 or 1: the premultiplied endpoint over the upper
+ *                 block triangle of the symmetric H (x_a^T H_aa x_a + 2 x_a^T
+ *                 H_ac x_c, c > a: half the FP64 work; default 1) or over all
+ *                 of H (0)
  *   "premult_trajectory" 0 or 1: trajectory gradients of the dense GEMM
  *                 engine through H as well (LinearMatrix.py:203-207,
  *                 g = H x - b): one GEMM with K = d per stage.  FP64: g from

@@ -189,15 +189,19 @@ def test_split_tail_matches_unsplit(precision, cfg, B):
         assert _rel(grad["split"][b], grad["plain"][b]) < same
 
 
-@pytest.mark.parametrize("precision,g_dtype", [("bf16", "bfloat16"), ("f64", None)])
-def test_premultiplied_endpoint_matches_residual_form(precision, g_dtype):
+@pytest.mark.parametrize("precision,g_dtype,d,sym", [("bf16", "bfloat16", 300, 1), ("f64", None, 300, 1),
+                                                     ("f64", None, 300, 0), ("f64", None, 100, 1),
+                                                     ("f64", None, 520, 1), ("bf16", "bfloat16", 640, 1)])
+def test_premultiplied_endpoint_matches_residual_form(precision, g_dtype, d, sym):
     """With n > d the GEMM engine evaluates the FP64 endpoint misfit as
     0.5 (x^T H x - 2 b^T x + d^T S^-1 d), H = G^T S^-1 G built once in FP64
-    (the reference's premultiplied algebra, LinearMatrix.py:164-191).  It
-    equals the residual form and the oracle to 1e-11 relative, for misfits
-    and for the H0/H1 of a proposal; vector variances enter H and b."""
+    (the reference's premultiplied algebra, LinearMatrix.py:164-191), over the
+    full H (sym 0) or its upper block triangle (sym 1, the default; d = 100,
+    300, 520, 640 give 1, 3, 5 and 5 blocks of 128 rows with partial last
+    blocks).  It equals the residual form and the oracle to 1e-11 relative, for
+    misfits and for the H0/H1 of a proposal; vector variances enter H and b."""
     rng = np.random.default_rng(31)
-    n, d, B = 700, 300, 6
+    n, B = 700, 6
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
     var = rng.uniform(0.005, 0.02, size=(n, 1))
@@ -209,7 +213,7 @@ def test_premultiplied_endpoint_matches_residual_form(precision, g_dtype):
     e = 0.002 * rng.uniform(0.5, 1.5, size=B)
     u = rng.uniform(size=B)
     chi, res = {}, {}
-    for name, opt in (("premult", {}), ("residual", {"endpoint_premult": 0})):
+    for name, opt in (("premult", {"endpoint_symmetric": sym}), ("residual", {"endpoint_premult": 0})):
         prob = build_problem(post, precision=precision, engine="gemm", options=opt)
         chi[name] = prob.misfit_batch(X)
         ch = Chains(prob, B)
