@@ -207,8 +207,7 @@ def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, premult):
     from hmclab_amd.engine import Chains
 
     B, L, d = args.chains, args.steps_per_traj, args.d
-    prob = build_problem(post, precision="f64", engine="gemm",
-                         options={"premult_trajectory": 1} if premult else None)
+    prob = build_problem(post, precision="f64", engine="gemm", options={"premult_trajectory": int(premult)})
     ch = Chains(prob, B)
     ch.set_state(X0)
     sub = argparse.Namespace(**vars(args))
@@ -313,8 +312,7 @@ def run_dense(args):
             opts["tile_order"] = args.tile_order
         if not args.split_tail:
             opts["gemm_split_tail"] = 0
-    if args.premult_trajectory:
-        opts["premult_trajectory"] = 1
+    opts["premult_trajectory"] = args.premult_trajectory
     prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)

@@ -1124,7 +1124,7 @@ __global__ void k_split_planes(const double* __restrict__ H, int64_t count, uint
 // gradient = H x - b): the gradient of a stage is one GEMM with K = d instead of
 // the residual and gradient GEMMs with K = d and K = n.
 bool traj_premult(const Problem& p) {
-  return p.premult_traj && endpoint_premult(p) && (p.precision == HMC_BF16 || p.precision == HMC_F64);
+  return p.premult_traj != 0 && endpoint_premult(p) && (p.precision == HMC_BF16 || p.precision == HMC_F64);
 }
 
 static void premult_kick_drift(Chains& c, double kick, double drift, double drift2, bool first) {

@@ -46,8 +46,8 @@ struct Problem {
   double* hb = nullptr;     // [d_pad] G^T S^-1 d
   int64_t ldh = 0;
   // trajectories through H as well (dense GEMM engine; needs the
-  // premultiplied endpoint): 0 residual form, 1 premultiplied
-  int premult_traj = 0;
+  // premultiplied endpoint): -1 with the endpoint, 0 residual form, 1 premultiplied
+  int premult_traj = -1;
   // premultiplied endpoint over the upper block triangle of H (half the work)
   int premult_sym = 1;
   uint16_t* Hh = nullptr;   // bf16 hi/lo planes of H [d_pad][ldg] (bf16 trajectories)

@@ -144,8 +144,9 @@ def test_bf16_trajectory_close_to_fp64(incremental, split_grad, premult):
         assert abs(res["H1"][c] - (chi1 + k1)) <= 1e-6 * abs(chi1 + k1), c
 
 
-@pytest.mark.parametrize("precision,cfg,B", [("bf16", 1, 256), ("bf16", 3, 256), ("bf16", 3, 200), ("f64", 1, 256)])
-def test_split_tail_matches_unsplit(precision, cfg, B):
+@pytest.mark.parametrize("precision,cfg,B,pm", [("bf16", 1, 256, 0), ("bf16", 3, 256, 0), ("bf16", 3, 200, 0),
+                                                ("f64", 1, 256, 0), ("bf16", 3, 256, 1), ("f64", 1, 200, 1)])
+def test_split_tail_matches_unsplit(precision, cfg, B, pm):
     """Planning for 16 compute units makes both GEMMs of a 2500 x 2200 problem
     with 256 chains leave a partial last round (residual 40 tiles of 128 rows
     or 20 of 256, gradient 36 or 18; 200 chains leave a partly empty chain
@@ -155,7 +156,9 @@ def test_split_tail_matches_unsplit(precision, cfg, B):
     coordinate after 5 leapfrog steps), its end-point Hamiltonians are FP64
     of those end points, and gradients stay within the bf16 tolerance of the
     oracle.  The FP64 endpoint GEMM (128-row tiles: 80 of them) splits its
-    tail the same way, and so do both GEMMs of FP64 trajectories (to 1e-12)."""
+    tail the same way, and so do both GEMMs of FP64 trajectories (to 1e-12).
+    pm = 1: premultiplied trajectories, whose stage GEMMs (M = d = 2200)
+    split their tails the same way."""
     rng = np.random.default_rng(21)
     n, d, L = 2500, 2200, 5
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
@@ -170,6 +173,7 @@ def test_split_tail_matches_unsplit(precision, cfg, B):
     res, grad = {}, {}
     for name, opts in (("split", {"gemm_split_cus": 16}), ("plain", {"gemm_split_tail": 0})):
         base = {"gemm_config": cfg, "incremental_residual": 1} if precision == "bf16" else {}
+        base["premult_trajectory"] = pm
         prob = build_problem(post, precision=precision, engine="gemm", options={**base, **opts})
         ch = Chains(prob, B)
         ch.set_state(X0)
@@ -213,7 +217,9 @@ def test_premultiplied_endpoint_matches_residual_form(precision, g_dtype, d, sym
     e = 0.002 * rng.uniform(0.5, 1.5, size=B)
     u = rng.uniform(size=B)
     chi, res = {}, {}
-    for name, opt in (("premult", {"endpoint_symmetric": sym}), ("residual", {"endpoint_premult": 0})):
+    # trajectories in the residual form in both runs: only the endpoint differs
+    for name, opt in (("premult", {"endpoint_symmetric": sym, "premult_trajectory": 0}),
+                      ("residual", {"endpoint_premult": 0})):
         prob = build_problem(post, precision=precision, engine="gemm", options=opt)
         chi[name] = prob.misfit_batch(X)
         ch = Chains(prob, B)
