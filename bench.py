@@ -717,7 +717,10 @@ def run_small(args):
     ch = Chains(prob, 1)
     ch.set_state(mu.reshape(1, 50))
     L = 10
-    # stated step 0.05: latency only (it exceeds the stability limit 2/sqrt(lam_max))
+    # stated step 0.05: latency only (it exceeds the stability limit 2/sqrt(lam_max));
+    # one untimed proposal first builds what the first call builds (H when premultiplied)
+    ch.run(1, L, np.full(1, 0.05), seed=args.seed + 1)
+    ch.sync()
     ch.enable_timing(True)
     t0 = time.perf_counter()
     ch.run(args.steps, L, np.full(1, 0.05), seed=args.seed)

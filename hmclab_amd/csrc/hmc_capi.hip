@@ -562,7 +562,7 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
         c.partial = (double*)dev_alloc(c.owned, (size_t)c.n_mblocks * c.B_pad * 8);
         c.split_ws64_elems = gemm_split_workspace_f64(P, c.B);
         if (c.split_ws64_elems) c.split_ws64 = (double*)dev_alloc(c.owned, c.split_ws64_elems * 8);
-        HMC_REQUIRE(P.premult_traj <= 0 || traj_premult(P), HMC_ERR_ARG,
+        HMC_REQUIRE(P.premult_traj <= 0 || traj_premult(P) || fused_premult(P), HMC_ERR_ARG,
                     "premult_trajectory needs the GEMM engine with the premultiplied endpoint "
                     "(n > d or endpoint_premult = 1)");
         if (traj_premult(P) && P.precision == HMC_BF16) {

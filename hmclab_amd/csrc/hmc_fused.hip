@@ -136,6 +136,11 @@ struct Ops {
   const T* bwd_val;
   const VT* fwd_tgt;    // target precision values of the forward operator
   int64_t n, d;
+  // premultiplied trajectories (dense): H = G^T C^-1 G in trajectory precision,
+  // row-major [d_pad][ldh] (symmetric, so also its k-major view), and b; or null
+  const T* h_val;
+  int64_t ldh;
+  const double* hb;
 
   template <typename Acc, typename V, typename Tin, int CT, class F>
   __device__ void forward(const V* val, const Tin* in, int w, int lane, F&& f) const {
@@ -276,6 +281,56 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     const int step = st / a.P, k = st - step * a.P;
     const double* sc = (step + 1 < a.L) ? a.pat[k] : a.pat_last[k];
     const double kick = sc[0], drift = sc[1], drift2 = sc[2];
+    // kick (gradient g without the prior), drift, reflect for owned slice ks
+    auto update = [&](int ks, int64_t j, const T* g) {
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        const double e = s_eps[c];
+        T xv = Xs[j * CT + c], pv = preg[ks][c];
+        T gg = g[c] + prior_grad<T>(md, j, xv);
+        if (out_of_bounds(md, j, (double)xv)) gg = (T)INFINITY;
+        pv -= (T)(a.beta ? kick * e * s_beta[c] : kick * e) * gg;
+        xv += (T)(drift * e) * kgrad<T>(md, j, pv);
+        corrector<T>(md, j, xv, pv);
+        if (drift2 != 0.0) {
+          xv += (T)(drift2 * e) * kgrad<T>(md, j, pv);
+          corrector<T>(md, j, xv, pv);
+        }
+        Xs[j * CT + c] = xv;
+        preg[ks][c] = pv;
+      }
+    };
+    if constexpr (!SPARSE && KMAX <= 2) {   // the gradients of more slices would spill
+      if (op.h_val) {
+        // premultiplied: g = H x - b for every owned parameter first (all
+        // threads read all of x), then the updates
+        T gk[KMAX][CT];
+#pragma unroll
+        for (int ks = 0; ks < KMAX; ++ks) {
+          const int64_t s = w + (int64_t)ks * FU_NW;
+#pragma unroll
+          for (int c = 0; c < CT; ++c) gk[ks][c] = T(0);
+          const int64_t row = s * 64 + lane;
+          if (s < nps && row < op.d) {
+            dense_row<T, T, T, CT>(op.h_val + row, op.ldh, op.d, Xs, gk[ks]);
+            const T bj = (T)op.hb[row];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) gk[ks][c] -= bj;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KMAX; ++ks) {
+          const int64_t s = w + (int64_t)ks * FU_NW;
+          if (s < nps) {
+            const int64_t j = op.param_of(s, lane);
+            if (j >= 0) update(ks, j, gk[ks]);
+          }
+        }
+        __syncthreads();
+        continue;
+      }
+    }
     op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, lane, [&](int64_t, int64_t i, T* acc) {
       if (i >= 0) {
         const T di = (T)a.dobs[i], vi = (T)a.var[i];
@@ -299,24 +354,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
           if (row < op.d) dense_row<T, T, T, CT>(op.bwd_val + row, op.bwd_d.ld, op.n, Rs, g);
         }
         const int64_t j = op.param_of(s, lane);
-        if (j >= 0) {
-#pragma unroll
-          for (int c = 0; c < CT; ++c) {
-            const double e = s_eps[c];
-            T xv = Xs[j * CT + c], pv = preg[ks][c];
-            T gg = g[c] + prior_grad<T>(md, j, xv);
-            if (out_of_bounds(md, j, (double)xv)) gg = (T)INFINITY;
-            pv -= (T)(a.beta ? kick * e * s_beta[c] : kick * e) * gg;
-            xv += (T)(drift * e) * kgrad<T>(md, j, pv);
-            corrector<T>(md, j, xv, pv);
-            if (drift2 != 0.0) {
-              xv += (T)(drift2 * e) * kgrad<T>(md, j, pv);
-              corrector<T>(md, j, xv, pv);
-            }
-            Xs[j * CT + c] = xv;
-            preg[ks][c] = pv;
-          }
-        }
+        if (j >= 0) update(ks, j, g);
       }
     }
     __syncthreads();
@@ -581,8 +619,39 @@ static Ops<T, VT, SPARSE> make_ops(const Problem& p) {
     op.fwd_val = (const T*)p.Gt;
     op.bwd_val = (const T*)p.G;
     op.fwd_tgt = (const VT*)p.Gt_tgt;
+    if (fused_premult(p) && p.H) {
+      op.h_val = sizeof(T) == 8 ? (const T*)p.H : (const T*)p.Hf;
+      op.ldh = p.ldh;
+      op.hb = p.hb;
+    }
   }
   return op;
+}
+
+// Premultiplied fused trajectories (LinearMatrix.py:203-207, g = H x - b): the
+// dense operator when premult_trajectory is 1, or -1 with n > d (the
+// reference's rule); the endpoint keeps the FP64 residual form.
+bool fused_premult(const Problem& p) {
+  return p.op == 1 && p.n > 0 && p.premult_traj != 0 && (p.premult_traj > 0 || p.n > p.d) &&
+         (p.d + 63) / 64 <= 2 * FU_NW;   // at most 2 slices per wave (KMAX = 2)
+}
+
+__global__ void k_h_to_f32(const double* __restrict__ H, int64_t count, float* __restrict__ out) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = (float)H[k];
+}
+
+static void fused_premult_build(Chains& c) {
+  Problem& p = *c.p;
+  if (!fused_premult(p)) return;
+  if (!p.H) premult_build(c);
+  if (p.state_dtype != HMC_F64 && !p.Hf) {
+    const int64_t cnt = p.d_pad * p.ldh;
+    p.Hf = (float*)dev_alloc(p.owned, (size_t)cnt * 4);
+    hipLaunchKernelGGL(k_h_to_f32, dim3((unsigned)std::min<int64_t>((cnt + 255) / 256, 4096)), dim3(256), 0,
+                       p.stream, p.H, cnt, p.Hf);
+    HMC_HIP(hipGetLastError());
+  }
 }
 
 template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
@@ -651,6 +720,7 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
   integrator_schedule(integrator, L, first, st);
   const int P = (int)st.size() / L;
   if (trace_slot >= 0) HMC_REQUIRE(trace_slot < c.tr_cap, HMC_ERR_STATE, "trace capacity exceeded");
+  fused_premult_build(c);
 
   FusedArgs a{};
   a.x0 = c.x0;

@@ -1286,6 +1286,10 @@ static void build_premult_t(Chains& c) {
   for (void* q : tmp) (void)hipFree(q);
 }
 
+void premult_build(Chains& c) {
+  if (!c.p->H) build_premult(c);
+}
+
 static void build_premult(Chains& c) {
   Problem& p = *c.p;
   p.ldh = round_up(p.d, 64);

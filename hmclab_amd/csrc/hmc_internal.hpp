@@ -51,6 +51,7 @@ struct Problem {
   // premultiplied endpoint over the upper block triangle of H (half the work)
   int premult_sym = 1;
   uint16_t* Hh = nullptr;   // bf16 hi/lo planes of H [d_pad][ldg] (bf16 trajectories)
+  float* Hf = nullptr;      // float32 H (fused float32 trajectories)
   uint16_t* Hl = nullptr;
   double dtd = 0.0;         // d^T S^-1 d
   int tgt_dtype = HMC_F64;  // element type defining the target's G
@@ -221,6 +222,8 @@ double endpoint_const(const Problem& p);      // constant added to every misfit
 //   p -= kick*eps*g(xt); xt += drift*eps*Minv p; corrector; [xt += drift2*eps*Minv p; corrector]
 void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);
 bool traj_premult(const Problem& p);
+void premult_build(Chains& c);            // H, b in FP64 (dense), once
+bool fused_premult(const Problem& p);
 // Endpoint partial sums of the data misfit in FP64 into c.partial.
 void gemm_endpoint(Chains& c, const void* X, int x_dtype);
 // gradient (B x d) of model rows X into out (device, f64, row stride ldx)

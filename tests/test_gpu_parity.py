@@ -307,6 +307,20 @@ def test_teacher_forced_premultiplied_trajectory_f64(integ, L):
              options={"premult_trajectory": 1})
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("premult", [0, 1])
+@pytest.mark.parametrize("integ", ["lf", "4s"])
+def test_teacher_forced_fused_dense_forms(precision, premult, integ):
+    """Fused dense engine (n = 150 > d = 70, both priors, bounds): trajectories
+    in the residual form (premult_trajectory 0) and through H = G^T C^-1 G
+    (the default for n > d: one pass over H per stage, the gradient of every
+    owned parameter before any update) against the oracle's residual form."""
+    post, orc, rng = _dense_case(4, 150, 70, prior="both", bounds=True)
+    d = post.dimensions
+    _teacher(post, orc, MM.Unit(d), O.Unit(d, np.random.default_rng(5)), rng, 6, 10, integ, precision, "fused",
+             options={"premult_trajectory": premult})
+
+
 @pytest.mark.parametrize("engine", ["gemm", "fused"])
 def test_teacher_forced_diagonal_mass_f64(engine):
     post, orc, rng = _dense_case(6, 120, 64, prior="normal")
