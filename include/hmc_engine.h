@@ -143,9 +143,10 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 block triangle of the symmetric H (x_a^T H_aa x_a + 2 x_a^T
  *                 H_ac x_c, c > a: half the FP64 work; default 1) or over all
  *                 of H (0)
- *   "premult_trajectory" -1, 0 or 1: trajectory gradients of the dense GEMM
- *                 engine through H as well (LinearMatrix.py:203-207,
- *                 g = H x - b): one GEMM with K = d per stage.  FP64: g from
+ *   "premult_trajectory" -1, 0 or 1: trajectory gradients of dense targets
+ *                 through H as well (LinearMatrix.py:203-207, g = H x - b):
+ *                 the GEMM engine runs one GEMM with K = d per stage, the
+ *                 fused engine one pass over H per stage (d <= 1024).  FP64: g from
  *                 H directly; bf16: a float32 y = H x - b from hi/lo planes
  *                 of H and x at a trajectory's first stage, then y += H_hi dx
  *                 with one plane of the position change per stage.  1 needs the
