@@ -109,8 +109,9 @@ def test_bf16_gemm_gradient_and_fp64_misfit(cfg, n, d, B):
         assert _rel(g[b], orc.gradient(x)[:, 0]) < 2e-4
 
 
-@pytest.mark.parametrize("incremental,split_grad,premult", [(0, 2, 0), (1, 2, 0), (1, 1, 0), (0, 2, 1)])
-def test_bf16_trajectory_close_to_fp64(incremental, split_grad, premult):
+@pytest.mark.parametrize("incremental,split_grad,premult,integ", [(0, 2, 0, "lf"), (1, 2, 0, "lf"), (1, 1, 0, "lf"),
+                                                                (0, 2, 1, "lf"), (0, 2, 1, "4s")])
+def test_bf16_trajectory_close_to_fp64(incremental, split_grad, premult, integ):
     """bf16 trajectories (hi/lo positions in the first residual GEMM; with
     incremental_residual the float32 residual then advances by G dx with one
     bf16 plane of dx; premultiplied: y = H x - b from hi/lo planes of H and x
@@ -133,10 +134,10 @@ def test_bf16_trajectory_close_to_fp64(incremental, split_grad, premult):
     P0 = rng.normal(size=(B, d)).astype(np.float32).astype(np.float64)
     e = 0.002 * rng.uniform(0.5, 1.5, size=B)
     u = rng.uniform(size=B)
-    res = ch.propose(P0, e, u, L, "lf", trajectory=True)
+    res = ch.propose(P0, e, u, L, integ, trajectory=True)
     unit = O.Unit(d, np.random.default_rng(0))
     for c in range(B):
-        ref = O.propose(orc, unit, X0[c].reshape(d, 1), None, P0[c].reshape(d, 1), e[c], L, u[c])
+        ref = O.propose(orc, unit, X0[c].reshape(d, 1), None, P0[c].reshape(d, 1), e[c], L, u[c], integ)
         assert _rel(res["X1"][c], ref["x1"][:, 0]) < 2e-3, c
         assert _rel(res["H0"][c], ref["h0"]) < 1e-10
         chi1 = orc.misfit(res["X1"][c].reshape(d, 1))
