@@ -15,10 +15,13 @@ Differences from the reference, by design:
   float32 (LinearMatrix.py:25-31,148,152).  Pass ``dtype=numpy.float32`` for
   that behaviour, or ``dtype="bfloat16"`` to define the target by
   bf16-rounded G.
-* The likelihood is always evaluated in the residual form
-  0.5*||(G m - d)/sigma||^2 with gradient G^T((G m - d)/sigma^2); the
-  ``premultiplication`` keyword is accepted and recorded but does not switch
-  to the premultiplied algebra (LinearMatrix.py:164-191).
+* ``premultiplication`` selects the algebra of a dense likelihood on the
+  device as in LinearMatrix.py:163-207: True evaluates the Metropolis misfit
+  and the trajectory gradients through H = G^T C^-1 G and b = G^T C^-1 d,
+  False in the residual form 0.5*||(G m - d)/sigma||^2 with gradient
+  G^T((G m - d)/sigma^2), None (default) premultiplies when n > d, the
+  reference's rule.  Host-side ``misfit()``/``gradient()`` use the residual
+  form.
 * A full data covariance (n x n) is handled by pre-whitening with its
   Cholesky factor (G' = L^-1 G, d' = L^-1 d, unit variance), which gives the
   same misfit and gradient as LinearMatrix.py:226-305.
@@ -348,7 +351,7 @@ class LinearMatrix(_AbstractDistribution):
                  **kwargs):
         if kwargs:
             raise TypeError(f"Unidentified argument(s): {kwargs}")
-        self.premultiplication = premultiplication   # recorded; residual form is always used
+        self.premultiplication = premultiplication   # device algebra of a dense G (None: n > d)
         self.use_mkl = use_mkl
         self.g_dtype = None
         if dtype is not None and str(dtype) in ("bfloat16", "bf16"):
@@ -441,9 +444,14 @@ def build_problem(dist: _AbstractDistribution, precision: str = "f64", engine: s
     spec = _Spec()
     dist._contribute(spec)
     prob = Problem(dist.dimensions, precision=precision, engine=engine)
+    lik = spec.likelihood
+    if lik is not None and lik.premultiplication is not None and not sp.issparse(lik.G):
+        # LinearMatrix.py:163-167: an explicit choice of the algebra
+        pm = int(bool(lik.premultiplication))
+        prob.set_option("endpoint_premult", pm)
+        prob.set_option("premult_trajectory", pm)
     for k, v in (options or {}).items():
         prob.set_option(k, v)
-    lik = spec.likelihood
     if lik is not None:
         G, d, var = lik._whitened()
         if sp.issparse(G):

@@ -466,3 +466,38 @@ def test_set_state_rejects_infinite_misfit():
     ch = Chains(build_problem(post), 2)
     with pytest.raises(ValueError, match="inf"):
         ch.set_state(np.full((2, d), 2.0))
+
+
+def test_premultiplication_keyword_selects_the_algebra():
+    """LinearMatrix(premultiplication=True/False) selects the device algebra
+    as the reference's keyword does (LinearMatrix.py:163-167): True on an
+    n < d problem (where the default keeps the residual form) gives the same
+    misfits and proposal as the explicit premultiplied options, False on an
+    n > d problem the same as the explicit residual options; both match the
+    oracle."""
+    rng = np.random.default_rng(41)
+    B = 4
+    for n, d, flag in ((60, 90, True), (150, 70, False)):
+        G = rng.normal(size=(n, d)) / np.sqrt(d)
+        dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+        orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
+                           O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
+        X = rng.normal(size=(B, d)) * 0.3
+        P0 = rng.normal(size=(B, d))
+        e = np.full(B, 0.003)
+        u = rng.uniform(size=B)
+        out = []
+        for kw, opts in ((dict(premultiplication=flag), None),
+                         ({}, {"endpoint_premult": int(flag), "premult_trajectory": int(flag)})):
+            post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, **kw)])
+            prob = build_problem(post, precision="f64", engine="gemm", options=opts)
+            chi = prob.misfit_batch(X)
+            ch = Chains(prob, B)
+            ch.set_state(X)
+            res = ch.propose(P0, e, u, 5, "lf", trajectory=True)
+            out.append((chi, res["X1"], res["H1"]))
+            ch.close()
+            prob.close()
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        for b in range(B):
+            assert _rel(out[0][0][b], orc.misfit(X[b].reshape(d, 1))) < 1e-11
