@@ -503,7 +503,23 @@ __global__ __launch_bounds__(256) void k_set_endpoint(const SetDesc* __restrict_
     const float* grow = G + D.g_off + (int64_t)(rbase + i) * D.ldg + 4 * q;
     const float* xrow = X + D.x_off + (int64_t)i * D.ldg + 4 * q;
     doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < D.ldg; k0 += 16) {
+    // the loads of SET_U consecutive 16-wide k blocks are issued before their
+    // MFMAs (same accumulation order)
+    int k0 = 0;
+    for (; k0 + 16 * SET_U <= D.ldg; k0 += 16 * SET_U) {
+      floatx4 aa[SET_U], bb[SET_U];
+#pragma unroll
+      for (int u = 0; u < SET_U; ++u) {
+        aa[u] = *(const floatx4*)(grow + k0 + 16 * u);
+        bb[u] = *(const floatx4*)(xrow + k0 + 16 * u);
+      }
+#pragma unroll
+      for (int u = 0; u < SET_U; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)aa[u][e], (double)bb[u][e], acc, 0, 0, 0);
+    }
+    for (; k0 < D.ldg; k0 += 16) {
       const floatx4 a = *(const floatx4*)(grow + k0);
       const floatx4 b = *(const floatx4*)(xrow + k0);
 #pragma unroll
