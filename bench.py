@@ -10,17 +10,22 @@ posterior draws as starting states:
   with the measured leg's trajectory form, and (default) also the
   residual-form FP64 comparator;
 * surrogate+batched (the reported value): trajectory GEMMs on bf16 MFMA
-  (hi/lo split operands, FP32 accumulation), endpoint Hamiltonian and
-  Metropolis in FP64; an MLP surrogate of the likelihood (8192 -> 512 -> 512
-  -> 1, SiLU) is trained on the GPU on (m, misfit, gradient) pairs from
-  warm-up proposals and drives the trajectory until a chunk's acceptance
-  falls below 0.5, after which the exact gradient takes over.
+  (FP32 accumulation), endpoint Hamiltonian and Metropolis in FP64; an MLP
+  surrogate of the likelihood (8192 -> 512 -> 512 -> 1, SiLU) is trained on
+  the GPU on (m, misfit, gradient) pairs from warm-up proposals.  Whether it
+  drives the timed trajectories is decided during warm-up, before timing
+  starts: it is attached only if its held-out gradient error is below
+  --sur-max-grad-err and a warm-up probe of --sur-probe proposals keeps
+  acceptance above the fallback threshold; attached, it still falls back to
+  the exact gradient when a chunk's acceptance drops below the threshold.
+  The line reports the fraction of timed proposals the surrogate drove.
 
 Both legs default to premultiplied trajectories, the reference's algebra for
 n > d (LinearMatrix.py:163-177): H = G^T C^-1 G and b = G^T C^-1 d are built
-once in FP64; the bf16 leg keeps y = H x - b in float32 (hi/lo planes of H and
-x at a trajectory's first stage, then y += H_hi dx per stage), the FP64 leg
-computes g = H x - b per stage.  --premult-trajectory 0 runs both legs in the
+once in FP64; the FP64 leg computes g = H x - b per stage, the bf16 leg the
+anchored field g = (H x_ref - b) + bf16(H) bf16(x - x_ref) with x_ref the
+mean of the starting draws (a function of the position alone, so the
+trajectories stay reversible).  --premult-trajectory 0 runs both legs in the
 residual form (g = G^T (G x - d) / var).
 
 A "step" is one proposal for all chains.  W warm-up proposals run untimed,
@@ -30,8 +35,9 @@ errors of both legs are against the analytic FP64 posterior of the
 bf16-rounded G ("matched posterior error").
 
 The roofline line is measured live with HIP events on the engine's stream
-around every trajectory GEMM launch (algorithmic FLOP per stage = 2 d^2 B
-premultiplied, 2 n d B in the residual form).  The CPU baseline times the oracle (the reference's op sequence:
+around every trajectory stage GEMM launch and its split-tail epilogue
+(algorithmic FLOP per stage = 2 d^2 B premultiplied, 2 n d B per GEMM in the
+residual form).  The CPU baseline times the oracle (the reference's op sequence:
 L gradients + 2 misfits per proposal, residual form, float64, one core) on
 a bounded sample and converts with the GPU's measured ESS per proposal.
 """
@@ -149,6 +155,15 @@ def cpu_baseline(args, eps):
     return rates, secs
 
 
+def _cpu_model():
+    """Host CPU model name and logical CPU count (from /proc/cpuinfo)."""
+    try:
+        names = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs visible)" if names else None
+    except OSError:
+        return None
+
+
 def trace_to_torch(torch, ch, K):
     """Device trace (K, B, d) -> torch float32 tensor (B, K, d) on the GPU."""
     out = torch.empty((ch.B, K, ch.d), dtype=torch.float32, device="cuda")
@@ -180,23 +195,26 @@ def _posterior_errors(torch, ch, K, mu_np, sd_np, d):
 
 
 def _kernel_times(ch):
-    return {k: ch.timing(k) for k in range(4)}
+    return {k: ch.timing(k) for k in range(8)}
 
 
-def _trajectory_gradient_error(post, prob, X, precision):
-    """Relative error (max-norm per row, worst row) of the trajectory gradient
-    in `precision` against the FP64 gradient of the same target."""
+def _trajectory_gradient_error(post, ch, X, precision):
+    """Relative error (max-norm per row; worst row and RMS over rows) of the
+    gradient field the chains' trajectories integrate (for bf16 the anchored
+    field, hmc_chains_trajectory_gradient) against the FP64 gradient of the
+    same target, at positions X (B, d)."""
     from hmclab_amd.Distributions import build_problem
 
     if precision == "f64":
-        return 0.0
-    ref = build_problem(post, precision="f64", engine="gemm")
+        return 0.0, 0.0
+    ref = build_problem(post, precision="f64", engine="gemm", options={"premult_trajectory": 0})
     try:
         g64 = ref.gradient_batch(X)
     finally:
         ref.close()
-    g = prob.gradient_batch(X)
-    return float(np.max(np.max(np.abs(g - g64), axis=1) / np.max(np.abs(g64), axis=1)))
+    g = ch.trajectory_gradient(X)
+    e = np.max(np.abs(g - g64), axis=1) / np.max(np.abs(g64), axis=1)
+    return float(np.max(e)), float(np.sqrt(np.mean(e ** 2)))
 
 
 def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, premult):
@@ -226,7 +244,9 @@ def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, premult):
                 acceptance_rate=float(acc.mean() / args.plain_steps), stepsize=eps,
                 posterior_rms_mean_err_over_sd=em, posterior_rms_std_rel_err=es,
                 ms_per_proposal=elapsed / args.plain_steps * 1e3, _mean=mean, _std=std,
-                gemm_fp64_avg_us=(kt[0][0] + kt[3][0]) / max(kt[0][1] + kt[3][1], 1) * 1e3)
+                stage_fp64_avg_us=(kt[4][0] / max(kt[4][1], 1) * 1e3 if premult
+                                   else (kt[0][0] + kt[3][0]) / max(kt[0][1] + kt[3][1], 1) * 1e3),
+                endpoint_fp64_avg_us=kt[1][0] / max(kt[1][1], 1) * 1e3)
 
 
 def _train_surrogate(args, torch, ch, post, B, L, eps, done):
@@ -316,33 +336,53 @@ def run_dense(args):
     prob = build_problem(post, precision=args.precision, engine="gemm", options=opts)
     ch = Chains(prob, B)
     ch.set_state(X0)
+    # bf16 trajectories: the anchor of the position-only gradient field is the
+    # mean of the starting draws, fixed for the whole run (warm-up included)
+    ch.set_anchor()
     log(f"device problem ready ({time.perf_counter() - t_setup:.1f}s)")
     eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), args, args.seed)
-    grad_err = _trajectory_gradient_error(post, prob, X0[:8], args.precision)
 
+    # surrogate: trained on warm-up pairs, then attached for the timed phase only
+    # if its held-out gradient error and a warm-up probe pass (decided here,
+    # before timing, so the timed chain is one fixed Markov kernel)
     sur, sur_info = None, None
     if args.surrogate:
         sur, sur_info = _train_surrogate(args, torch, ch, post, B, L, eps, done)
         done = sur_info["proposals_done"]
-        ch.set_surrogate(sur)
+        decision = "rejected: held-out gradient error %.3f > %.3g" % (sur_info["heldout_grad_rel_err"],
+                                                                      args.sur_max_grad_err)
+        if sur_info["heldout_grad_rel_err"] <= args.sur_max_grad_err:
+            ch.set_surrogate(sur)
+            _, a = ch.run(args.sur_probe, L, np.full(B, eps), seed=args.seed + 5, first_proposal=done)
+            done += args.sur_probe
+            rate = float(a.sum()) / (args.sur_probe * B)
+            sur_info["probe_acceptance"] = rate
+            if rate < args.sur_fallback:
+                ch.set_surrogate(None)
+                decision = f"rejected: warm-up probe acceptance {rate:.3f} < {args.sur_fallback}"
+            else:
+                decision = f"attached: warm-up probe acceptance {rate:.3f}"
+        sur_info["decision"] = decision
+        log(f"surrogate {decision}")
+    attached = sur is not None and sur_info["decision"].startswith("attached")
 
-    # timed sampling phase: chunks of proposals; the surrogate is detached when
-    # a chunk's acceptance falls below the fallback threshold
+    # timed sampling phase: chunks of proposals; an attached surrogate is
+    # detached when a chunk's acceptance falls below the fallback threshold
     ch.set_trace(args.steps)
     ch.enable_timing(True)
     ch.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     acc = np.zeros(B, dtype=np.int64)
-    fallback_at, sur_acc, k = None, [], 0
+    fallback_at, sur_acc, sur_props, k = None, [], 0, 0
     while k < args.steps:
-        # while the surrogate drives the trajectories, short probe chunks keep a
-        # failing surrogate from costing more than sur_probe proposals
-        m = min(args.sur_probe if (sur is not None and fallback_at is None) else args.chunk, args.steps - k)
+        m = min(args.chunk, args.steps - k)
+        driving = attached and fallback_at is None
         _, a = ch.run(m, L, np.full(B, eps), seed=args.seed, first_proposal=done + k)
         acc += a
         k += m
-        if sur is not None and fallback_at is None:
+        if driving:
+            sur_props += m
             rate = float(a.sum()) / (m * B)
             sur_acc.append(rate)
             if rate < args.sur_fallback:
@@ -352,33 +392,38 @@ def run_dense(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     if fallback_at is not None:
-        log(f"surrogate fallback after {fallback_at} proposals (chunk acceptance {sur_acc[-1]:.3f} < "
+        log(f"surrogate fallback after {fallback_at} timed proposals (chunk acceptance {sur_acc[-1]:.3f} < "
             f"{args.sur_fallback})")
     kt = _kernel_times(ch)
-    (res_ms, res_n), (end_ms, end_n), (ew_ms, ew_n), (leap_ms, leap_n) = kt[0], kt[1], kt[2], kt[3]
-    log(f"timed {args.steps} proposals in {elapsed:.3f}s; residual gemm {res_ms:.1f} ms/{res_n}, "
-        f"kick-drift gemm {leap_ms:.1f} ms/{leap_n}, endpoint {end_ms:.1f} ms/{end_n}, "
-        f"elementwise {ew_ms:.1f} ms/{ew_n}")
+    names = {0: "residual gemm", 1: "endpoint", 2: "elementwise", 3: "gradient gemm", 4: "premultiplied stage gemm",
+             5: "surrogate gemms", 6: "surrogate kick/drift"}
+    log(f"timed {args.steps} proposals in {elapsed:.3f}s; " +
+        ", ".join(f"{names[k_]} {kt[k_][0]:.1f} ms/{kt[k_][1]}" for k_ in names if kt[k_][1]))
     if args.premult_trajectory:
-        gemm_ms, gemm_n, gemm_key = leap_ms, leap_n, "premult"
-        gemm_name = ("k_gemm_bf16<...,EpiLeap> + k_split_epilogue (premultiplied stage: y += H dx, kick/drift; "
-                     "a trajectory's first stage adds the hi/lo y = H x - b GEMMs to its timed region)")
-    elif leap_ms >= res_ms:
-        gemm_ms, gemm_n, gemm_key = leap_ms, leap_n, "gradient"
+        slot, gemm_key = 4, "premult"
+        gemm_name = ("k_gemm_bf16<256,128,64,64,1,3,EpiLeap> + k_split_epilogue (premultiplied stage: "
+                     "g = y_ref + bf16(H) bf16(x - x_ref), kick/drift; timing slot 4 holds only these launches)")
+    elif kt[3][0] >= kt[0][0]:
+        slot, gemm_key = 3, "gradient"
         gemm_name = ("k_gemm_bf16<...,EpiLeap> + k_split_epilogue (gradient GEMM + kick/drift; "
                      "the timed region holds both kernels)")
     else:
-        gemm_ms, gemm_n, gemm_key = res_ms, res_n, "residual"
-        gemm_name = ("k_gemm_bf16<...,EpiResidRaw> + k_split_epilogue (residual GEMM; the timed region "
+        slot, gemm_key = 0, "residual"
+        gemm_name = ("k_gemm_bf16<...,EpiResidAnchor> + k_split_epilogue (residual GEMM; the timed region "
                      "holds both kernels)")
+    gemm_ms, gemm_n = kt[slot]
     min_ess, err_mean, err_std, _, _ = _posterior_errors(torch, ch, args.steps, mu_np, sd_np, d)
     acc_rate = float(acc.mean() / args.steps)
     value = min_ess / elapsed
+    # the gradient field the timed trajectories integrated, at the chains' final positions
+    Xf, _ = ch.get_state()
+    grad_err, grad_err_rms = (_trajectory_gradient_error(post, ch, Xf, args.precision)
+                              if not (attached and fallback_at is None) else (None, None))
 
-    # algorithmic FLOP of one stage: the gradient's GEMM (K = n residual form, K = d premultiplied)
+    # algorithmic FLOP of one stage GEMM (K = n residual form, K = d premultiplied)
     flop_launch = 2.0 * d * d * B if args.premult_trajectory else 2.0 * n * d * B
     avg_s = gemm_ms / 1e3 / max(gemm_n, 1)
-    achieved = flop_launch / avg_s
+    achieved = flop_launch / avg_s if gemm_n else 0.0
     peak = PEAK["bf16"] if args.precision == "bf16" else PEAK[args.precision]
     # HBM bytes per launch of the dominant kernel from the committed counter
     # profile (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel); only for
@@ -392,9 +437,9 @@ def run_dense(args):
                    "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
                    "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
                    "split_tail": args.split_tail, "prior": args.prior,
-                   "premult_trajectory": args.premult_trajectory}
+                   "premult_trajectory": args.premult_trajectory, "gradient_field": "anchored"}
             for prof in tj.get("profiles", []):
-                if all(prof.get("options", {}).get(k, object()) == v for k, v in ran.items()):
+                if all(prof.get("options", {}).get(k_, object()) == v for k_, v in ran.items()):
                     traffic = prof["bytes_per_launch"].get(gemm_key)
         except Exception:
             traffic = None
@@ -403,29 +448,41 @@ def run_dense(args):
         "prior": "Laplace(0, 0.3); errors are against the plain fp64 leg" if args.prior == "laplace"
                  else "Normal(0, 1); errors are against the analytic fp64 posterior",
         "workload": f"dense linear {n}x{d} ({args.prior} prior), G stored bf16, {B} chains, L={L}: surrogate+batched "
-                    f"({args.precision} trajectory GEMMs, fp64 endpoint Metropolis, MLP surrogate with "
-                    f"fallback) vs plain batched fp64 GPU HMC; both legs "
+                    f"({args.precision} trajectory GEMMs, fp64 endpoint Metropolis, MLP surrogate decided in "
+                    f"warm-up) vs plain batched fp64 GPU HMC; both legs "
                     + ("premultiplied (H = G^T C^-1 G, the reference's form for n > d)" if args.premult_trajectory
                        else "in the residual form"),
         "chains": B, "n_data": n, "n_params": d, "leapfrog_steps": L, "stepsize": eps,
         "min_ess": min_ess, "acceptance_rate": acc_rate,
         "posterior_rms_mean_err_over_sd": err_mean, "posterior_rms_std_rel_err": err_std,
         "proposals_per_s": args.steps / elapsed,
+        "surrogate_fraction_of_timed_proposals": sur_props / args.steps,
         "trajectory_gradient_rel_err_vs_fp64": grad_err,
+        "trajectory_gradient_rel_err_vs_fp64_rms": grad_err_rms,
+        "trajectory_gradient_field": ("anchored: y_ref + bf16(H) bf16(x - x_ref), x_ref = mean of the starting draws"
+                                      if args.premult_trajectory and args.precision == "bf16" else
+                                      "exact" if args.precision == "f64" else
+                                      "residual form (" + ("anchored" if args.incremental_residual else "hi/lo x planes")
+                                      + ")"),
         "gemm": {"config": args.gemm_config, "config_grad": args.gemm_config_grad,
                  "bf16_split": args.bf16_split, "bf16_split_grad": args.bf16_split_grad,
                  "incremental_residual": args.incremental_residual, "tile_order": args.tile_order,
                  "split_tail": args.split_tail, "premult_trajectory": args.premult_trajectory},
+        "kernel_avg_us": {names[k_]: kt[k_][0] / kt[k_][1] * 1e3 for k_ in names if kt[k_][1]},
     }
     if sur_info is not None:
-        sur_info.update(chunk_acceptance=sur_acc, fallback_after_proposals=fallback_at,
-                        fallback_threshold=args.sur_fallback)
+        sur_info.update(timed_chunk_acceptance=sur_acc, timed_fallback_after_proposals=fallback_at,
+                        fallback_threshold=args.sur_fallback, max_heldout_grad_err=args.sur_max_grad_err)
+        if sur_props:
+            sur_info["value_including_training_s"] = min_ess / (elapsed + sur_info["train_s"]
+                                                                + sur_info["warmup_and_pairs_s"])
         cfg["surrogate"] = sur_info
     if plain is not None:
         plain.pop("_mean", None)
         plain.pop("_std", None)
         cfg["plain_fp64"] = plain
         cfg["speedup_vs_plain_fp64"] = value / plain["value"]
+        cfg["ms_per_proposal_ratio_vs_plain_fp64"] = plain["ms_per_proposal"] / (elapsed / args.steps * 1e3)
     if plain_res is not None:
         plain_res.pop("_mean", None)
         plain_res.pop("_std", None)
@@ -447,9 +504,9 @@ def run_dense(args):
         "config": cfg,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
-                     "kernel": gemm_name, "avg_launch_us": avg_s * 1e6,
-                     "other_gemm_avg_us": (min(res_ms, leap_ms) / 1e3 / max(gemm_n, 1)) * 1e6,
-                     "endpoint_fp64_avg_us": end_ms / max(end_n, 1) * 1e3},
+                     "kernel": gemm_name, "avg_launch_us": avg_s * 1e6, "launches": gemm_n,
+                     "algorithmic_flop_per_launch": flop_launch,
+                     "endpoint_fp64_avg_us": kt[1][0] / max(kt[1][1], 1) * 1e3},
     }
 
     if args.cpu_baseline:
@@ -458,6 +515,7 @@ def run_dense(args):
         agg = float(np.sum(rates))
         result["cpu_baseline"] = {
             "value": ess_per_prop_chain * agg, "unit": "ESS/s", "cores": len(rates), "kind": "port",
+            "cpu_model": _cpu_model(),
             "sample": f"{len(rates)} single-chain oracle processes at once (one core each, BLAS pinned to 1 "
                       f"thread, float64 G, residual form, L={L}), {args.cpu_proposals} proposals each in "
                       f"{min(secs):.1f}-{max(secs):.1f}s: {agg:.4g} proposals/s in total "
@@ -804,8 +862,12 @@ def main():
     ap.add_argument("--sur-epochs", type=int, default=8)
     ap.add_argument("--sur-lr", type=float, default=1e-3)
     ap.add_argument("--sur-fallback", type=float, default=0.5)
-    ap.add_argument("--sur-probe", type=int, default=2,
-                    help="proposals per acceptance check while the surrogate is attached")
+    ap.add_argument("--sur-probe", type=int, default=4,
+                    help="warm-up proposals with the surrogate attached that decide whether it drives the "
+                         "timed phase")
+    ap.add_argument("--sur-max-grad-err", type=float, default=0.5,
+                    help="largest held-out gradient error (relative to the gradient's spread) at which the "
+                         "surrogate is tried")
     args = ap.parse_args()
     if args.plain_steps is None:
         args.plain_steps = args.steps

@@ -460,8 +460,9 @@ class HMC:
                             else np.empty((B, d + 1, 0)))
         if self.diagnostic_mode:
             total = (self.end_time - self.start_time).total_seconds()
-            names = ["trajectory gradient kernels", "endpoint misfit", "element-wise / Metropolis",
-                     "trajectory kick/drift GEMM"]
+            names = ["residual GEMM / fused proposal", "endpoint misfit", "element-wise / Metropolis",
+                     "gradient GEMM + kick/drift", "premultiplied stage GEMM + kick/drift",
+                     "surrogate GEMMs", "surrogate kick/drift"]
             self.diagnostics = {}
             print("Detailed statistics:")
             print(f"Total runtime: {total:.2f} seconds")

@@ -69,6 +69,8 @@ SIGNATURES = {
     "hmc_chains_destroy": [_p],
     "hmc_chains_set_state": [_p, _dp],
     "hmc_chains_get_state": [_p, _dp, _dp],
+    "hmc_chains_set_anchor": [_p, _dp],
+    "hmc_chains_trajectory_gradient": [_p, _dp, _dp],
     "hmc_chains_propose": [_p, _dp, _dp, _dp, _i32, _i32, _ip, _dp, _dp, _dp, _dp],
     "hmc_chains_run": [_p, _i64, _i32, _i32, _dp, _i32, _i32, C.c_double, C.c_double,
                        C.c_uint64, _i64, _i64, _lp],

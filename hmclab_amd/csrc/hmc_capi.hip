@@ -558,7 +558,10 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
       c.ex_tmp = (int32_t*)dev_alloc(c.owned, (size_t)c.B_pad * 2 * 4);
       c.ex_u = (double*)dev_alloc(c.owned, c.B_pad * 8);
       if (P.op == 1) {
-        c.n_mblocks = std::max(P.n_pad, P.d_pad) / 128;   // residual (n) or premultiplied (d) endpoint
+        // endpoint partial rows: residual form (n_pad / 128 row blocks), premultiplied
+        // over all of H (d_pad / 128) or over its upper block triangle (2 ceil(T/2) items)
+        const int64_t T = (P.d + 127) / 128;
+        c.n_mblocks = std::max({P.n_pad / 128, P.d_pad / 128, 2 * ((T + 1) / 2)});
         c.partial = (double*)dev_alloc(c.owned, (size_t)c.n_mblocks * c.B_pad * 8);
         c.split_ws64_elems = gemm_split_workspace_f64(P, c.B);
         if (c.split_ws64_elems) c.split_ws64 = (double*)dev_alloc(c.owned, c.split_ws64_elems * 8);
@@ -566,17 +569,19 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
                     "premult_trajectory needs the GEMM engine with the premultiplied endpoint "
                     "(n > d or endpoint_premult = 1)");
         if (traj_premult(P) && P.precision == HMC_BF16) {
-          c.yf = (float*)dev_alloc(c.owned, rows * 4);
           c.dxp[0] = (uint16_t*)dev_alloc(c.owned, rows * 2);
           c.dxp[1] = (uint16_t*)dev_alloc(c.owned, rows * 2);
         } else if (traj_premult(P)) {
           c.xs = dev_alloc(c.owned, rows * es);
         }
+        if (gemm_anchored(P)) {
+          c.xa = (float*)dev_alloc(c.owned, (size_t)c.ldx * 4);
+          c.yref = (float*)dev_alloc(c.owned, (size_t)std::max(c.ldx, c.ldr) * 4);
+        }
         if (P.precision == HMC_BF16) {
           c.xh = (uint16_t*)dev_alloc(c.owned, rows * 2);
           c.xl = (uint16_t*)dev_alloc(c.owned, rows * 2);
           c.rh = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
-          if (P.resid_update) c.rf = (float*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 4);
           c.split_ws_elems = gemm_split_workspace(P, c.B);
           if (c.split_ws_elems) c.split_ws = (float*)dev_alloc(c.owned, c.split_ws_elems * 4);
           c.rl = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
@@ -773,6 +778,33 @@ int hmc_gradient_batch(hmc_problem* hp, const double* M, int64_t B, double* out)
   });
 }
 
+int hmc_chains_set_anchor(hmc_chains* hc, const double* x_ref) {
+  return guarded("hmc_chains_set_anchor", [&] {
+    HMC_REQUIRE(hc, HMC_ERR_ARG, "NULL chains");
+    Chains& c = hc->c;
+    if (!gemm_anchored(*c.p)) return;   // no anchored trajectories for this problem
+    gemm_set_anchor(c, x_ref);
+    HMC_HIP(hipStreamSynchronize(c.p->stream));
+  });
+}
+
+int hmc_chains_trajectory_gradient(hmc_chains* hc, const double* X, double* out) {
+  return guarded("hmc_chains_trajectory_gradient", [&] {
+    HMC_REQUIRE(hc && X && out, HMC_ERR_ARG, "NULL argument");
+    Chains& c = hc->c;
+    Problem& P = *c.p;
+    copy_rows_to_device(c, X, c.B);
+    elem_load_rows(c, c.p_host, c.xt, c.xh, c.xl);
+    if (P.resolved_engine() == HMC_ENGINE_GEMM) {
+      HMC_REQUIRE(c.surrogate == nullptr, HMC_ERR_STATE, "a surrogate drives these chains' trajectories");
+      gemm_traj_gradient(c, c.xt, c.p_host);
+    } else {
+      fused_gradient(c, c.xt, c.p_host);
+    }
+    copy_rows_to_host(c, out, c.B);
+  });
+}
+
 int hmc_chains_propose(hmc_chains* hc, const double* P0, const double* eps, const double* u, int L,
                        int integrator, int32_t* accepted, double* H0, double* H1, double* X1,
                        double* P1) {
@@ -918,7 +950,7 @@ int hmc_chains_enable_timing(hmc_chains* hc, int on) {
     Chains& c = hc->c;
     time_collect(c);
     c.timing = on != 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kTimingKinds; ++k) {
       c.t_ms[k] = 0.0;
       c.t_n[k] = 0;
     }
@@ -927,7 +959,7 @@ int hmc_chains_enable_timing(hmc_chains* hc, int on) {
 
 int hmc_chains_timing(hmc_chains* hc, int kind, double* total_ms, int64_t* launches) {
   return guarded("hmc_chains_timing", [&] {
-    HMC_REQUIRE(hc && kind >= 0 && kind < 4, HMC_ERR_ARG, "invalid argument");
+    HMC_REQUIRE(hc && kind >= 0 && kind < kTimingKinds, HMC_ERR_ARG, "invalid argument");
     Chains& c = hc->c;
     time_collect(c);
     if (total_ms) *total_ms = c.t_ms[kind];

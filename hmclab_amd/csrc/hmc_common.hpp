@@ -149,6 +149,7 @@ __device__ inline double philox_exchange(uint64_t seed, uint32_t chain, uint64_t
 // Passed by value to kernels.
 // ----------------------------------------------------------------------------
 constexpr int kMaxPriors = 4;
+constexpr int kTimingKinds = 8;   // kernel timing groups of a chain set
 
 struct DevPrior {
   int kind;           // HMC_PRIOR_NORMAL (a = means, b = inverse variances)

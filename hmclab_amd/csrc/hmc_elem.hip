@@ -11,13 +11,15 @@ namespace hmc {
 
 constexpr int EW_NT = 256;
 
+// bf16 planes for the GEMM engine: hi/lo of v, or (anchored, l null) one
+// plane of v - a
 template <typename T>
-__device__ inline void put_planes(uint16_t* h, uint16_t* l, int64_t o, T v) {
+__device__ inline void put_planes(uint16_t* h, uint16_t* l, int64_t o, T v, float a = 0.0f) {
   if (h) {
-    const float f = (float)v;
+    const float f = (float)v - a;
     const uint16_t hb = f32_to_bf16_bits(f);
     h[o] = hb;
-    l[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(hb));
+    if (l) l[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(hb));
   }
 }
 
@@ -30,7 +32,7 @@ __global__ __launch_bounds__(EW_NT) void k_refresh(T* __restrict__ xt, T* __rest
                                                    int64_t d, double* k0, double* eps, double* u,
                                                    const double* step, double drift0, uint64_t seed,
                                                    int64_t prop, int host_p, int device_scalars,
-                                                   int randomize, DevModel md) {
+                                                   int randomize, const float* __restrict__ xa, DevModel md) {
   __shared__ double scratch[EW_NT / 64 + 1];
   const int64_t c = blockIdx.x;
   double e;
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(EW_NT) void k_refresh(T* __restrict__ xt, T* __rest
       corrector<T>(md, j, xv, pv);
       xt[o] = xv;
       pt[o] = pv;
-      put_planes<T>(xh, xl, o, xv);
+      put_planes<T>(xh, xl, o, xv, xa ? xa[j] : 0.0f);
     }
   }
   const double s = block_sum<EW_NT>(ks, scratch);
@@ -257,18 +259,32 @@ __global__ __launch_bounds__(EW_NT) void k_oob(const T* __restrict__ X, int64_t 
 void elem_refresh(Chains& c, bool host_p, double drift0, uint64_t seed, int64_t prop,
                   bool device_scalars, int randomize) {
   Problem& p = *c.p;
+  // planes for the first GEMM of the trajectory: hi/lo of x, or one plane of
+  // x - x_ref for anchored trajectories (premultiplied: the stage buffer 0)
+  uint16_t* ph = c.xh;
+  uint16_t* pl = c.xl;
+  const float* xa = nullptr;
+  if (c.surrogate == nullptr && gemm_anchored(p)) {
+    if (!c.anchor_ready) gemm_set_anchor(c, nullptr);
+    xa = c.xa;
+    pl = nullptr;
+    if (traj_premult(p)) {
+      ph = c.dxp[0];
+      c.dx_cur = 0;
+    }
+  }
   hipEvent_t ev;
   time_begin(c, 2, ev);
   if (p.state_dtype == HMC_F64) {
     hipLaunchKernelGGL(k_refresh<double>, dim3(c.B), dim3(EW_NT), 0, p.stream, (double*)c.xt,
-                       (double*)c.pt, c.xh, c.xl, (const double*)c.x0, c.p_host, c.ldx, p.d, c.k0,
+                       (double*)c.pt, ph, pl, (const double*)c.x0, c.p_host, c.ldx, p.d, c.k0,
                        c.eps, c.u, c.step, drift0, seed, prop, host_p ? 1 : 0,
-                       device_scalars ? 1 : 0, randomize, p.model());
+                       device_scalars ? 1 : 0, randomize, xa, p.model());
   } else {
     hipLaunchKernelGGL(k_refresh<float>, dim3(c.B), dim3(EW_NT), 0, p.stream, (float*)c.xt,
-                       (float*)c.pt, c.xh, c.xl, (const float*)c.x0, c.p_host, c.ldx, p.d, c.k0,
+                       (float*)c.pt, ph, pl, (const float*)c.x0, c.p_host, c.ldx, p.d, c.k0,
                        c.eps, c.u, c.step, drift0, seed, prop, host_p ? 1 : 0,
-                       device_scalars ? 1 : 0, randomize, p.model());
+                       device_scalars ? 1 : 0, randomize, xa, p.model());
   }
   HMC_HIP(hipGetLastError());
   time_end(c, 2, ev);

@@ -574,37 +574,32 @@ struct EpiResid {
   }
 };
 
-// residual with a float32 running copy: raw = acc - d (full product G x) or
-// raw += acc (increment G dx); writes raw and the bf16 hi/lo planes of raw/var
-// (ol null: the gradient GEMM reads the hi plane only).
-struct EpiResidRaw {
+// anchored residual (bf16 trajectories): r = rref_m + acc with acc = G_hi .
+// bf16(x - x_ref); writes the bf16 hi/lo planes of r/var (ol null: the
+// gradient GEMM reads the hi plane only)
+struct EpiResidAnchor {
   static constexpr bool kReduce = false;
-  float* raw;       // [N_pad][ld]
+  const float* rref;   // [M] G x_ref - d (float32 of an FP64 product)
   uint16_t* oh;
   uint16_t* ol;
   int64_t ld;
-  const double* dobs;
   const double* var;
-  int accumulate;
   int M, N;
   template <typename V>
   __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
     if (ms == 1 && m0 + 3 < M) {
-      // 4 consecutive rows of one chain: one 16-byte access of the residual
-      // and one 8-byte store per bf16 plane (o is a multiple of 4)
+      // 4 consecutive rows of one chain: one 8-byte store per bf16 plane
+      // (o is a multiple of 4)
       const int64_t o = n * ld + m0;
-      floatx4 r4 = accumulate ? *reinterpret_cast<const floatx4*>(&raw[o]) : floatx4{0.f, 0.f, 0.f, 0.f};
+      const floatx4 r4 = *reinterpret_cast<const floatx4*>(&rref[m0]);
       uint32_t h[4], l[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float r = accumulate ? r4[e] + (float)v[e] : (float)v[e] - (float)dobs[m0 + e];
-        r4[e] = r;
-        const float f = r / (float)var[m0 + e];
+        const float f = (r4[e] + (float)v[e]) / (float)var[m0 + e];
         h[e] = f32_to_bf16_bits(f);
         l[e] = f32_to_bf16_bits(f - bf16_bits_to_f32((uint16_t)h[e]));
       }
-      *reinterpret_cast<floatx4*>(&raw[o]) = r4;
       *reinterpret_cast<uint2*>(&oh[o]) = uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
       if (ol) *reinterpret_cast<uint2*>(&ol[o]) = uint2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
       return;
@@ -614,9 +609,7 @@ struct EpiResidRaw {
       const int64_t m = m0 + (int64_t)e * ms;
       if (m >= M) return;
       const int64_t o = n * ld + m;
-      const float r = accumulate ? raw[o] + (float)v[e] : (float)v[e] - (float)dobs[m];
-      raw[o] = r;
-      const float f = r / (float)var[m];
+      const float f = (rref[m] + (float)v[e]) / (float)var[m];
       const uint16_t h = f32_to_bf16_bits(f);
       oh[o] = h;
       if (ol) ol[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
@@ -624,8 +617,10 @@ struct EpiResidRaw {
   }
 };
 
-// trajectory step: g = acc + prior'(x); p -= kick*eps*g; x += drift*eps*kgrad(p);
-// corrector; optional second drift + corrector.
+// trajectory step: g = acc [+ yref_j] [- bias_j] + prior'(x); p -= kick*eps*g;
+// x += drift*eps*kgrad(p); corrector; optional second drift + corrector.
+// The bf16 planes written for the next GEMM hold x (hi/lo), or x - xa (one
+// plane) for anchored trajectories.
 template <typename T>
 struct EpiLeap {
   static constexpr bool kReduce = false;
@@ -639,8 +634,8 @@ struct EpiLeap {
   double kick, drift, drift2;
   DevModel md;
   int M, N;
-  int delta_planes;     // planes hold x_new - x_old (residual increments) instead of x
-  float* y;             // premultiplied bf16 trajectories: y += acc, g = y (float32, ld) (or null)
+  const float* xa;      // anchor: planes hold x - xa (or null)
+  const float* yref;    // premultiplied bf16: g = yref + acc (or null)
   const double* bias;   // premultiplied FP64 trajectories: g = acc - bias[j] (or null)
   __device__ __forceinline__ void step(int64_t j, T& xv, T& pv, T g, T kk, T dd, T dd2) const {
     g += prior_grad<T>(md, j, xv);
@@ -666,21 +661,19 @@ struct EpiLeap {
         floatx4 x4 = *reinterpret_cast<const floatx4*>(&x[o]);
         floatx4 p4 = *reinterpret_cast<const floatx4*>(&p[o]);
         floatx4 g4 = {(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
-        if (y) {
-          g4 += *reinterpret_cast<const floatx4*>(&y[o]);
-          *reinterpret_cast<floatx4*>(&y[o]) = g4;
-        }
+        if (yref) g4 += *reinterpret_cast<const floatx4*>(&yref[m0]);
+        floatx4 a4 = {0.f, 0.f, 0.f, 0.f};
+        if (xa) a4 = *reinterpret_cast<const floatx4*>(&xa[m0]);
         uint32_t h[4], l[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           T xv = x4[k], pv = p4[k];
-          const T xold = xv;
           T g = (T)g4[k];
           if (bias) g -= (T)bias[m0 + k];
           step(m0 + k, xv, pv, g, kk, dd, dd2);
           x4[k] = xv;
           p4[k] = pv;
-          const float f = delta_planes ? (float)(xv - xold) : (float)xv;
+          const float f = (float)xv - a4[k];
           h[k] = f32_to_bf16_bits(f);
           l[k] = f32_to_bf16_bits(f - bf16_bits_to_f32((uint16_t)h[k]));
         }
@@ -699,19 +692,14 @@ struct EpiLeap {
       if (j >= M) return;
       const int64_t o = n * ld + j;
       T xv = x[o], pv = p[o];
-      const T xold = xv;
       T g = (T)v[k];
-      if (y) {
-        const float yv = y[o] + (float)v[k];
-        y[o] = yv;
-        g = (T)yv;
-      }
+      if (yref) g = (T)((float)v[k] + yref[j]);
       if (bias) g -= (T)bias[j];
       step(j, xv, pv, g, kk, dd, dd2);
       x[o] = xv;
       p[o] = pv;
       if (xh) {
-        const float f = delta_planes ? (float)(xv - xold) : (float)xv;
+        const float f = xa ? (float)xv - xa[j] : (float)xv;
         const uint16_t h = f32_to_bf16_bits(f);
         xh[o] = h;
         if (xl) xl[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
@@ -720,26 +708,8 @@ struct EpiLeap {
   }
 };
 
-// first stage of a premultiplied bf16 trajectory: y = acc - b (float32)
-struct EpiPmStart {
-  static constexpr bool kReduce = false;
-  float* y;
-  int64_t ld;
-  const double* b;
-  int M, N;
-  template <typename V>
-  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
-    if (n >= N) return;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int64_t m = m0 + (int64_t)e * ms;
-      if (m >= M) return;
-      y[n * ld + m] = (float)((double)v[e] - b[m]);
-    }
-  }
-};
-
-// gradient output: out[n][m] = acc + prior'(x) (+inf for a chain out of bounds)
+// gradient output: out[n][m] = acc [+ yref_m] [- bias_m] + prior'(x) (+inf for a
+// chain out of bounds), in the arithmetic of EpiLeap's stage gradient
 template <typename T>
 struct EpiGrad {
   static constexpr bool kReduce = false;
@@ -749,6 +719,8 @@ struct EpiGrad {
   const int32_t* oob;
   DevModel md;
   int M, N;
+  const float* yref;    // anchored premultiplied bf16 field (or null)
+  const double* bias;   // premultiplied FP64 field (or null)
   template <typename V>
   __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
     if (n >= N) return;
@@ -757,7 +729,10 @@ struct EpiGrad {
       const int64_t j = m0 + (int64_t)k * ms;
       if (j >= M) return;
       const int64_t o = n * ld + j;
-      double g = (double)((T)v[k] + prior_grad<T>(md, j, x[o]));
+      T gv = (T)v[k];
+      if (yref) gv = (T)((float)v[k] + yref[j]);
+      if (bias) gv -= (T)bias[j];
+      double g = (double)(gv + prior_grad<T>(md, j, x[o]));
       if (oob[n]) g += INFINITY;
       out[o] = g;
     }
@@ -1109,15 +1084,10 @@ int64_t gemm_chain_pad() { return 128; } // N-tile of every GEMM config
 
 static void build_premult(Chains& c);
 
-// bf16 hi/lo planes of the FP64 H (same [d_pad][ldg] layout, ldh = ldg)
-__global__ void k_split_planes(const double* __restrict__ H, int64_t count, uint16_t* __restrict__ hh,
-                               uint16_t* __restrict__ hl) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x) {
-    const double v = H[k];
-    const uint16_t h = f32_to_bf16_bits((float)v);
-    hh[k] = h;
-    hl[k] = f32_to_bf16_bits((float)(v - (double)bf16_bits_to_f32(h)));
-  }
+// bf16 plane of the FP64 H (same [d_pad][ldh] layout)
+__global__ void k_split_planes(const double* __restrict__ H, int64_t count, uint16_t* __restrict__ hh) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x)
+    hh[k] = f32_to_bf16_bits((float)H[k]);
 }
 
 // Premultiplied trajectories (LinearMatrix.py:184-207 with premultiplication:
@@ -1127,48 +1097,107 @@ bool traj_premult(const Problem& p) {
   return p.premult_traj != 0 && endpoint_premult(p) && (p.precision == HMC_BF16 || p.precision == HMC_F64);
 }
 
-static void premult_kick_drift(Chains& c, double kick, double drift, double drift2, bool first) {
+bool gemm_anchored(const Problem& p) {
+  return p.op == 1 && p.precision == HMC_BF16 && p.resolved_engine() == HMC_ENGINE_GEMM &&
+         (traj_premult(p) || p.resid_update);
+}
+
+// xa[j] = mean over chains of X[c][j] (fixed order, FP64 sums)
+template <typename T>
+__global__ void k_chain_mean(const T* __restrict__ X, int64_t ldx, int64_t d, int64_t B, float* __restrict__ xa) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= d) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < B; ++c) s += (double)X[c * ldx + j];
+  xa[j] = (float)(s / (double)B);
+}
+
+// out[m] = (float)(sum_k A[m][k] x[k] - sub[m]) for m < M: one wavefront per
+// row, lanes over k, fixed butterfly (FP64)
+template <typename TA>
+__global__ __launch_bounds__(256) void k_anchor_matvec(const TA* __restrict__ A, int64_t lda, int64_t M, int64_t K,
+                                                       const float* __restrict__ x, const double* __restrict__ sub,
+                                                       float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t m = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (m >= M) return;
+  double s = 0.0;
+  for (int64_t k = lane; k < K; k += 64) {
+    double a;
+    if constexpr (sizeof(TA) == 2) a = (double)bf16_bits_to_f32((uint16_t)A[m * lda + k]);
+    else a = (double)A[m * lda + k];
+    s += a * (double)x[k];
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[m] = (float)(s - sub[m]);
+}
+
+void gemm_set_anchor(Chains& c, const double* x_ref) {
+  Problem& p = *c.p;
+  if (!gemm_anchored(p)) return;
+  HMC_REQUIRE(c.xa && c.yref, HMC_ERR_STATE, "anchor buffers missing");
+  const int64_t d = p.d;
+  if (x_ref) {
+    std::vector<float> v((size_t)c.ldx, 0.0f);
+    for (int64_t j = 0; j < d; ++j) {
+      HMC_REQUIRE(std::isfinite(x_ref[j]), HMC_ERR_ARG, "anchor must be finite");
+      v[j] = (float)x_ref[j];
+    }
+    HMC_HIP(hipMemcpyAsync(c.xa, v.data(), (size_t)c.ldx * 4, hipMemcpyHostToDevice, p.stream));
+    HMC_HIP(hipStreamSynchronize(p.stream));
+  } else {
+    hipLaunchKernelGGL(k_chain_mean<float>, dim3((unsigned)((d + 255) / 256)), dim3(256), 0, p.stream,
+                       (const float*)c.x0, c.ldx, d, c.B, c.xa);
+    HMC_HIP(hipGetLastError());
+  }
+  if (traj_premult(p)) {
+    if (!p.H) build_premult(c);
+    hipLaunchKernelGGL(k_anchor_matvec<double>, dim3((unsigned)((d + 3) / 4)), dim3(256), 0, p.stream,
+                       (const double*)p.H, p.ldh, d, d, c.xa, p.hb, c.yref);
+  } else {
+    // the trajectory's G (bf16 values) at the anchor: r_ref = G x_ref - d
+    hipLaunchKernelGGL(k_anchor_matvec<uint16_t>, dim3((unsigned)((p.n + 3) / 4)), dim3(256), 0, p.stream,
+                       (const uint16_t*)p.G, p.ldg, p.n, d, c.xa, p.dobs, c.yref);
+  }
+  HMC_HIP(hipGetLastError());
+  c.anchor_ready = true;
+}
+
+static void ensure_h_plane(Problem& p) {
+  if (p.Hh) return;
+  const int64_t cnt = p.d_pad * p.ldh;
+  p.Hh = (uint16_t*)dev_alloc(p.owned, (size_t)cnt * 2);
+  hipLaunchKernelGGL(k_split_planes, dim3((unsigned)std::min<int64_t>((cnt + 255) / 256, 8192)), dim3(256), 0,
+                     p.stream, p.H, cnt, p.Hh);
+  HMC_HIP(hipGetLastError());
+}
+
+static void premult_kick_drift(Chains& c, double kick, double drift, double drift2) {
   Problem& p = *c.p;
   const int d = (int)p.d, B = (int)c.B;
   const DevModel md = p.model();
   if (!p.H) build_premult(c);
+  if (p.precision == HMC_BF16) ensure_h_plane(p);
   hipEvent_t ev;
-  time_begin(c, 3, ev);
+  time_begin(c, 4, ev);
   if (p.precision == HMC_BF16) {
-    if (!p.Hh) {
-      const int64_t cnt = p.d_pad * p.ldh;
-      p.Hh = (uint16_t*)dev_alloc(p.owned, (size_t)cnt * 2);
-      p.Hl = (uint16_t*)dev_alloc(p.owned, (size_t)cnt * 2);
-      hipLaunchKernelGGL(k_split_planes, dim3((unsigned)std::min<int64_t>((cnt + 255) / 256, 8192)), dim3(256), 0,
-                         p.stream, p.H, cnt, p.Hh, p.Hl);
-      HMC_HIP(hipGetLastError());
-    }
-    const int K = (int)p.ldh;
-    if (first) {
-      // y = H_hi (x_hi + x_lo) + H_lo x_hi - b: the stage's gradient to about
-      // 16 bits of H and x (the float32 sum then cancels against b)
-      launch_bf16(c, p.gemm_cfg_grad, 2, p.Hh, p.ldh, c.xh, c.xl, c.ldx, d, B, K,
-                  EpiPmStart{c.yf, c.ldx, p.hb, d, B});
-      EpiLeap<float> e{(float*)c.xt, (float*)c.pt, c.dxp[0], nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
-                       md, d, B, 1, c.yf, nullptr};
-      launch_bf16(c, p.gemm_cfg_grad, 1, p.Hl, p.ldh, c.xh, nullptr, c.ldx, d, B, K, e);
-      c.dx_cur = 0;
-    } else {
-      // y += H_hi dx with one plane of the last stage's increment dx
-      const int cur = c.dx_cur;
-      EpiLeap<float> e{(float*)c.xt, (float*)c.pt, c.dxp[1 - cur], nullptr, c.ldx, c.eps, c.beta, kick, drift,
-                       drift2, md, d, B, 1, c.yf, nullptr};
-      launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, c.dxp[cur], nullptr, c.ldx, d, B, K, e);
-      c.dx_cur = 1 - cur;
-    }
+    HMC_REQUIRE(c.anchor_ready, HMC_ERR_STATE, "anchored trajectory without an anchor");
+    // g = (H x_ref - b) + H_hi bf16(x - x_ref): one plane of the offset from
+    // the anchor, written by the previous stage (or the momentum refresh)
+    const int cur = c.dx_cur;
+    EpiLeap<float> e{(float*)c.xt, (float*)c.pt, c.dxp[1 - cur], nullptr, c.ldx, c.eps, c.beta, kick, drift,
+                     drift2, md, d, B, c.xa, c.yref, nullptr};
+    launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, c.dxp[cur], nullptr, c.ldx, d, B, (int)p.ldh, e);
+    c.dx_cur = 1 - cur;
   } else {
     // FP64: g = H x - b from a copy of the positions (the epilogue updates xt)
     HMC_HIP(hipMemcpyAsync(c.xs, c.xt, (size_t)c.B_pad * c.ldx * 8, hipMemcpyDeviceToDevice, p.stream));
     EpiLeap<double> e{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
-                      md, d, B, 0, nullptr, p.hb};
+                      md, d, B, nullptr, nullptr, p.hb};
     launch_f64(c, (const double*)p.H, p.ldh, (const double*)c.xs, c.ldx, d, B, (int)p.ldh, e);
   }
-  time_end(c, 3, ev);
+  time_end(c, 4, ev);
 }
 
 // Trajectory: residual GEMM then gradient GEMM with kick/drift epilogue.
@@ -1176,29 +1205,29 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
   Problem& p = *c.p;
   const int n = (int)p.n, d = (int)p.d, B = (int)c.B;
   const DevModel md = p.model();
+  (void)first;
   hipEvent_t ev1, ev2;
   if (traj_premult(p)) {
-    premult_kick_drift(c, kick, drift, drift2, first);
+    premult_kick_drift(c, kick, drift, drift2);
     return;
   }
   if (p.precision == HMC_BF16) {
-    const bool inc = p.resid_update && c.rf;
+    const bool anch = gemm_anchored(p);
     time_begin(c, 0, ev1);
-    if (inc) {
-      // first stage: full product G x (hi/lo planes of x); later stages: the
-      // float32 residual advances by G dx with one plane of dx
-      EpiResidRaw e1{c.rf, c.rh, p.bf16_split_grad == 2 ? c.rl : nullptr, c.ldr, p.dobs, p.var, first ? 0 : 1, n, B};
-      launch_bf16(c, p.gemm_cfg, first ? p.bf16_split : 1, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B,
-                  (int)p.ldg, e1);
+    if (anch) {
+      HMC_REQUIRE(c.anchor_ready, HMC_ERR_STATE, "anchored trajectory without an anchor");
+      // r = (G x_ref - d) + G_hi bf16(x - x_ref) with one plane of the offset
+      EpiResidAnchor e1{c.yref, c.rh, p.bf16_split_grad == 2 ? c.rl : nullptr, c.ldr, p.var, n, B};
+      launch_bf16(c, p.gemm_cfg, 1, (const uint16_t*)p.G, p.ldg, c.xh, nullptr, c.ldx, n, B, (int)p.ldg, e1);
     } else {
       EpiResid<float> e1{nullptr, c.rh, c.rl, c.ldr, p.dobs, p.var, n, B};
       launch_bf16(c, p.gemm_cfg, p.bf16_split, (const uint16_t*)p.G, p.ldg, c.xh, c.xl, c.ldx, n, B, (int)p.ldg,
                   e1);
     }
     time_end(c, 0, ev1);
-    // incremental: the planes carry dx for the next one-plane increment (no lo plane)
-    EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, inc ? nullptr : c.xl, c.ldx, c.eps, c.beta, kick, drift,
-                      drift2, md, d, B, inc ? 1 : 0};
+    // anchored: the plane carries x - x_ref for the next residual GEMM (no lo plane)
+    EpiLeap<float> e2{(float*)c.xt, (float*)c.pt, c.xh, anch ? nullptr : c.xl, c.ldx, c.eps, c.beta, kick, drift,
+                      drift2, md, d, B, anch ? c.xa : nullptr, nullptr, nullptr};
     time_begin(c, 3, ev2);
     launch_bf16(c, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
@@ -1208,7 +1237,7 @@ void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool f
     launch_f64(c, (const double*)p.G, p.ldg, (const double*)c.xt, c.ldx, n, B, (int)p.ldg, e1);
     time_end(c, 0, ev1);
     EpiLeap<double> e2{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
-                       md, d, B, 0};
+                       md, d, B};
     time_begin(c, 3, ev2);
     launch_f64(c, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
     time_end(c, 3, ev2);
@@ -1305,6 +1334,7 @@ static void build_premult(Chains& c) {
 void gemm_endpoint(Chains& c, const void* X, int x_dtype) {
   Problem& p = *c.p;
   const int n = (int)p.n, B = (int)c.B;
+  HMC_REQUIRE(endpoint_blocks(c) <= c.n_mblocks, HMC_ERR_STATE, "endpoint partial buffer too small");
   if (endpoint_premult(p)) {
     if (!p.H) build_premult(c);
     const int d = (int)p.d;
@@ -1373,6 +1403,51 @@ void gemm_gradient(Chains& c, const void* X, double* out) {
   } else {
     throw Error{HMC_ERR_UNSUPPORTED, "GEMM engine supports float64 and bfloat16 trajectories"};
   }
+}
+
+// plane[c][j] = bf16(X[c][j] - xa[j]) for the chains' rows
+__global__ void k_anchor_plane(const float* __restrict__ X, int64_t ldx, int64_t d, const float* __restrict__ xa,
+                               uint16_t* __restrict__ plane) {
+  const int64_t c = blockIdx.y;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < d; j += (int64_t)gridDim.x * blockDim.x)
+    plane[c * ldx + j] = f32_to_bf16_bits(X[c * ldx + j] - xa[j]);
+}
+
+// The gradient field a trajectory stage applies, at model rows X (state
+// dtype, chain rows of the set): the anchored bf16 field, the premultiplied
+// FP64 field, or the residual form otherwise (gemm_gradient).
+void gemm_traj_gradient(Chains& c, const void* X, double* out) {
+  Problem& p = *c.p;
+  const int n = (int)p.n, d = (int)p.d, B = (int)c.B;
+  const DevModel md = p.model();
+  if (p.precision == HMC_BF16 && gemm_anchored(p)) {
+    if (!c.anchor_ready) gemm_set_anchor(c, nullptr);
+    elem_oob_flags(c, X);
+    uint16_t* plane = traj_premult(p) ? c.dxp[0] : c.xh;
+    const dim3 grid((unsigned)std::min<int64_t>((p.d + 255) / 256, 64), (unsigned)B);
+    hipLaunchKernelGGL(k_anchor_plane, grid, dim3(256), 0, p.stream, (const float*)X, c.ldx, p.d, c.xa, plane);
+    HMC_HIP(hipGetLastError());
+    if (traj_premult(p)) {
+      ensure_h_plane(p);
+      EpiGrad<float> e{(const float*)X, out, c.ldx, c.oob, md, d, B, c.yref, nullptr};
+      launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, plane, nullptr, c.ldx, d, B, (int)p.ldh, e);
+    } else {
+      EpiResidAnchor e1{c.yref, c.rh, p.bf16_split_grad == 2 ? c.rl : nullptr, c.ldr, p.var, n, B};
+      launch_bf16(c, p.gemm_cfg, 1, (const uint16_t*)p.G, p.ldg, plane, nullptr, c.ldx, n, B, (int)p.ldg, e1);
+      EpiGrad<float> e2{(const float*)X, out, c.ldx, c.oob, md, d, B};
+      launch_bf16(c, p.gemm_cfg_grad, p.bf16_split_grad, (const uint16_t*)p.Gt, p.ldgt, c.rh, c.rl, c.ldr, d, B,
+                  (int)p.ldgt, e2);
+    }
+    return;
+  }
+  if (p.precision == HMC_F64 && traj_premult(p)) {
+    if (!p.H) build_premult(c);
+    elem_oob_flags(c, X);
+    EpiGrad<double> e{(const double*)X, out, c.ldx, c.oob, md, d, B, nullptr, p.hb};
+    launch_f64(c, (const double*)p.H, p.ldh, (const double*)X, c.ldx, d, B, (int)p.ldh, e);
+    return;
+  }
+  gemm_gradient(c, X, out);
 }
 
 }  // namespace hmc

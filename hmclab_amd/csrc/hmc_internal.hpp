@@ -32,7 +32,10 @@ struct Problem {
   int engine = HMC_ENGINE_AUTO;
   int bf16_split = 2;       // bf16 planes of the positions in the residual GEMM (1 or 2)
   int bf16_split_grad = 2;  // bf16 planes of the residuals in the gradient GEMM (1 or 2)
-  int resid_update = 0;     // bf16: advance a float32 residual by G dx after the first stage
+  // bf16 residual-form trajectories: r = r_ref + G_hi bf16(x - x_ref) with the
+  // anchor's residual r_ref kept in float32 (one bf16 plane of x - x_ref per
+  // stage) instead of hi/lo planes of x
+  int resid_update = 0;
   int tile_order = 0;       // bf16 GEMM tiles per XCD: 0 m-major, 1 n-major
   int gemm_cfg = 1;         // bf16 GEMM tile configuration of the residual GEMM
   int gemm_cfg_grad = 1;    // ... of the gradient GEMM
@@ -50,9 +53,8 @@ struct Problem {
   int premult_traj = -1;
   // premultiplied endpoint over the upper block triangle of H (half the work)
   int premult_sym = 1;
-  uint16_t* Hh = nullptr;   // bf16 hi/lo planes of H [d_pad][ldg] (bf16 trajectories)
+  uint16_t* Hh = nullptr;   // bf16 plane of H [d_pad][ldh] (bf16 trajectories)
   float* Hf = nullptr;      // float32 H (fused float32 trajectories)
-  uint16_t* Hl = nullptr;
   double dtd = 0.0;         // d^T S^-1 d
   int tgt_dtype = HMC_F64;  // element type defining the target's G
 
@@ -112,12 +114,20 @@ struct Chains {
   uint16_t* rh = nullptr;   // bf16 hi/lo planes of the residual
   uint16_t* rl = nullptr;
   void* r = nullptr;        // residual [B_pad][ldr] (state dtype, F64/F32 GEMM)
-  float* rf = nullptr;      // float32 running residual G x - d (bf16 GEMM, incremental mode)
-  // premultiplied trajectories: bf16 keeps y = H x - b in float32 [B_pad][ldx]
-  // and the position increments dx in two alternating bf16 planes (a stage's
-  // GEMM reads one while its epilogue writes the other); FP64 reads a copy xs
-  // of the positions while the epilogue updates xt
-  float* yf = nullptr;
+  // Anchored bf16 trajectories (premultiplied, or residual form with
+  // resid_update): the stage gradient is a function of the position alone,
+  //   premultiplied  g = (H x_ref - b) + H_hi bf16(x - x_ref) + prior'(x)
+  //   residual       r = (G x_ref - d) + G_hi bf16(x - x_ref),  g = G^T (r / var) + prior'(x)
+  // with one anchor x_ref per chain set, fixed until it is set again, so a
+  // trajectory is reversible and volume preserving (a deterministic gradient
+  // field).  xa = x_ref [ldx]; yref = H x_ref - b [ldx] or G x_ref - d [ldr],
+  // float32 values of FP64 products.
+  float* xa = nullptr;
+  float* yref = nullptr;
+  bool anchor_ready = false;
+  // premultiplied bf16: the planes of x - x_ref alternate between two buffers
+  // (a stage's GEMM reads one while its epilogue writes the other); FP64 reads
+  // a copy xs of the positions while the epilogue updates xt
   uint16_t* dxp[2] = {nullptr, nullptr};
   int dx_cur = 0;
   void* xs = nullptr;
@@ -144,8 +154,13 @@ struct Chains {
 
   // timing
   bool timing = false;
-  double t_ms[4] = {0, 0, 0, 0};
-  int64_t t_n[4] = {0, 0, 0, 0};
+  // kinds: 0 residual GEMM (fused engine: the whole-proposal kernel),
+  // 1 endpoint misfit, 2 element-wise / Metropolis, 3 gradient GEMM with the
+  // kick/drift epilogue (residual form), 4 premultiplied stage GEMM with its
+  // kick/drift epilogue, 5 surrogate forward + input-gradient GEMMs,
+  // 6 surrogate kick/drift
+  double t_ms[kTimingKinds] = {};
+  int64_t t_n[kTimingKinds] = {};
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   size_t ev_next = 0;
@@ -223,6 +238,14 @@ double endpoint_const(const Problem& p);      // constant added to every misfit
 void gemm_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);
 bool traj_premult(const Problem& p);
 void premult_build(Chains& c);            // H, b in FP64 (dense), once
+// bf16 trajectories of this problem use an anchor (see Chains::xa)
+bool gemm_anchored(const Problem& p);
+// set the anchor of a chain set: x_ref from the host (d values) or, when
+// null, the mean of the chains' current positions; computes yref in FP64
+void gemm_set_anchor(Chains& c, const double* x_ref);
+// the gradient a trajectory stage applies at model rows X (state dtype; for
+// bf16 the anchored field) into out (device, f64, row stride ldx)
+void gemm_traj_gradient(Chains& c, const void* X, double* out);
 bool fused_premult(const Problem& p);
 // Endpoint partial sums of the data misfit in FP64 into c.partial.
 void gemm_endpoint(Chains& c, const void* X, int x_dtype);

@@ -427,7 +427,7 @@ void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2) {
   } swap{s, s.stream};
   s.stream = p.stream;
   hipEvent_t ev;
-  time_begin(c, 0, ev);
+  time_begin(c, 5, ev);
   if (p.state_dtype == HMC_F64)
     hipLaunchKernelGGL(k_norm<double>, g1(rows * s.Dp), dim3(256), 0, s.stream, (const double*)c.xt, c.ldx,
                        s.Xn, s.Dp, s.D, s.xm, s.xs, rows);
@@ -435,9 +435,9 @@ void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2) {
     hipLaunchKernelGGL(k_norm<float>, g1(rows * s.Dp), dim3(256), 0, s.stream, (const float*)c.xt, c.ldx,
                        s.Xn, s.Dp, s.D, s.xm, s.xs, rows);
   forward_grad(s, rows);
-  time_end(c, 0, ev);
+  time_end(c, 5, ev);
   hipEvent_t ev2;
-  time_begin(c, 3, ev2);
+  time_begin(c, 6, ev2);
   if (p.state_dtype == HMC_F64)
     hipLaunchKernelGGL(k_sur_leap<double>, g1(c.B * p.d), dim3(256), 0, s.stream, (double*)c.xt, (double*)c.pt,
                        c.ldx, s.G, s.Dp, p.d, s.gscale, c.eps, (const double*)c.beta, kick, drift, drift2, c.B,
@@ -447,7 +447,7 @@ void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2) {
                        c.ldx, s.G, s.Dp, p.d, s.gscale, c.eps, (const double*)c.beta, kick, drift, drift2, c.B,
                        p.model());
   HMC_HIP(hipGetLastError());
-  time_end(c, 3, ev2);
+  time_end(c, 6, ev2);
 }
 
 }  // namespace hmc

@@ -153,6 +153,19 @@ class Chains:
         call("hmc_chains_get_state", self._h, dptr(X), dptr(chi))
         return X, chi
 
+    def set_anchor(self, x_ref=None):
+        """Anchor of bf16 trajectories (see hmc_chains_set_anchor): x_ref (d,)
+        or None for the mean of the chains' current positions."""
+        x = None if x_ref is None else np.ascontiguousarray(np.asarray(x_ref, dtype=np.float64).reshape(self.d))
+        call("hmc_chains_set_anchor", self._h, dptr(x))
+
+    def trajectory_gradient(self, X_rows):
+        """Gradient field the trajectories integrate, at B models (B, d)."""
+        X = _lib.as_f64_rows(X_rows, self.B, self.d)
+        out = np.empty_like(X)
+        call("hmc_chains_trajectory_gradient", self._h, dptr(X), dptr(out))
+        return out
+
     def propose(self, P0_rows, eps, u, amount_of_steps: int, integrator: str = "lf",
                 trajectory: bool = False):
         P0 = _lib.as_f64_rows(P0_rows, self.B, self.d)

@@ -120,11 +120,11 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 GEMMs (2 = hi/lo split, ~16-bit operand precision; default 2)
  *   "bf16_split_grad" 1 or 2: the same for the gradient GEMM only (its operand
  *                 is the scaled residual)
- *   "incremental_residual" 0 or 1: bf16 trajectories keep a float32
- *                 residual G x - d; the first stage of a trajectory computes it
- *                 from hi/lo planes of x, later stages add G dx with one bf16
- *                 plane of the position change dx (half the MFMA work of the
- *                 residual GEMM; default 0)
+ *   "incremental_residual" 0 or 1: bf16 residual-form trajectories take
+ *                 r = (G x_ref - d) + G_hi bf16(x - x_ref) with a float32
+ *                 residual at the chain set's anchor x_ref (one bf16 plane of
+ *                 x - x_ref per stage instead of hi/lo planes of x: half the
+ *                 MFMA work of the residual GEMM; default 0)
  *   "gemm_config" 0-3: bf16 GEMM tile of both trajectory GEMMs
  *                 (0 = 128x128, 4 waves of 64x64; 1 = 128x128, 8 waves of
  *                 64x32; 2 = 256x128, 4 waves of 128x64; 3 = 256x128, 8 waves
@@ -147,9 +147,9 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 through H as well (LinearMatrix.py:203-207, g = H x - b):
  *                 the GEMM engine runs one GEMM with K = d per stage, the
  *                 fused engine one pass over H per stage (d <= 1024).  FP64: g from
- *                 H directly; bf16: a float32 y = H x - b from hi/lo planes
- *                 of H and x at a trajectory's first stage, then y += H_hi dx
- *                 with one plane of the position change per stage.  1 needs the
+ *                 H directly; bf16: g = (H x_ref - b) + H_hi bf16(x - x_ref)
+ *                 with the anchor's FP64 product kept in float32 and one bf16
+ *                 plane of H and of x - x_ref.  1 needs the
  *                 premultiplied endpoint (chains creation fails otherwise);
  *                 -1 (default) follows the endpoint (the reference's n > d
  *                 rule); 0 keeps the residual form
@@ -203,6 +203,28 @@ int hmc_chains_propose(hmc_chains* c, const double* P0, const double* eps, const
                        int amount_of_steps, int integrator, int32_t* accepted, double* H0,
                        double* H1, double* X1, double* P1);
 
+/* Anchor of the bf16 trajectories (premultiplied, or residual form with
+ * "incremental_residual"): the stage gradient is
+ *   g(x) = g(x_ref) + A_hi bf16(x - x_ref) + prior'(x)
+ * (A = H, or G and G^T through the residual) with g(x_ref) computed in FP64,
+ * a function of the position alone, so trajectories stay reversible and
+ * volume preserving.  x_ref (dimensions values) or NULL: the mean of the
+ * chains' current positions.  The anchor stays until it is set again; a
+ * chain set without one takes the mean at its first trajectory.  Set it
+ * before a sampling phase, not between its proposals (a kernel that changes
+ * with the chain's own state is not a Markov kernel of the target).  A no-op
+ * for other precisions and engines.  No counterpart in the reference (its
+ * trajectories are FP64). */
+int hmc_chains_set_anchor(hmc_chains* c, const double* x_ref);
+
+/* The gradient field the trajectories of this chain set integrate, at B
+ * models X (B x dimensions) -> out (B x dimensions): the anchored bf16 field,
+ * the premultiplied FP64 field H x - b + prior', or the residual form.  For
+ * measuring the trajectory's gradient error against hmc_gradient_batch of
+ * an FP64 problem.  Replaces: the gradient calls inside
+ * _propagate_leapfrog (Samplers.py:1549-1569). */
+int hmc_chains_trajectory_gradient(hmc_chains* c, const double* X, double* out);
+
 /* Run `proposals` proposals entirely on the device with the Philox RNG
  * (seed, counter offset `first_proposal`).  Per-chain step sizes stepsize[B]
  * (host, in/out when autotuning).  randomize: multiply by uniform(0.5,1.5).
@@ -230,11 +252,14 @@ int hmc_chains_trace_device(hmc_chains* c, void** x_ptr, double** chi_ptr, int* 
 int hmc_chains_reset_trace(hmc_chains* c);
 
 /* Accumulated kernel times since the last reset, from HIP events recorded on
- * the engine's stream around each launch.  kind: 0 = first trajectory pass
- * (GEMM engine: residual GEMM; fused engine: the whole-proposal kernel),
- * 1 = endpoint misfit, 2 = element-wise / Metropolis, 3 = second trajectory
- * pass (GEMM engine: gradient GEMM with the kick/drift epilogue).  total_ms
- * and launches may be NULL.  Timing is off until hmc_chains_enable_timing(c,1). */
+ * the engine's stream around each launch (a launch and its split-tail
+ * epilogue kernel count as one).  kind: 0 = residual GEMM (fused engine: the
+ * whole-proposal kernel), 1 = endpoint misfit, 2 = element-wise / Metropolis,
+ * 3 = gradient GEMM with the kick/drift epilogue (residual form),
+ * 4 = premultiplied stage GEMM with the kick/drift epilogue, 5 = surrogate
+ * forward and input-gradient GEMMs, 6 = surrogate kick/drift, 7 = unused.
+ * total_ms and launches may be NULL.  Timing is off until
+ * hmc_chains_enable_timing(c,1). */
 int hmc_chains_enable_timing(hmc_chains* c, int on);
 int hmc_chains_timing(hmc_chains* c, int kind, double* total_ms, int64_t* launches);
 

@@ -112,12 +112,12 @@ def test_bf16_gemm_gradient_and_fp64_misfit(cfg, n, d, B):
 @pytest.mark.parametrize("incremental,split_grad,premult,integ", [(0, 2, 0, "lf"), (1, 2, 0, "lf"), (1, 1, 0, "lf"),
                                                                 (0, 2, 1, "lf"), (0, 2, 1, "4s")])
 def test_bf16_trajectory_close_to_fp64(incremental, split_grad, premult, integ):
-    """bf16 trajectories (hi/lo positions in the first residual GEMM; with
-    incremental_residual the float32 residual then advances by G dx with one
-    bf16 plane of dx; premultiplied: y = H x - b from hi/lo planes of H and x
-    at the first stage, then y += H_hi dx) stay within 2e-3 of the FP64 oracle
-    trajectory (relative to the largest coordinate) over 10 leapfrog steps;
-    the Hamiltonians are exact FP64 of those end points."""
+    """bf16 trajectories (hi/lo planes of x in every residual GEMM; with
+    incremental_residual the anchored residual r = (G x_ref - d) + G_hi
+    bf16(x - x_ref); premultiplied: g = (H x_ref - b) + H_hi bf16(x - x_ref))
+    stay within 2e-3 of the FP64 oracle trajectory (relative to the largest
+    coordinate) over 10 leapfrog steps; the Hamiltonians are exact FP64 of
+    those end points."""
     rng = np.random.default_rng(11)
     n, d, B, L = 520, 384, 8, 10
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
@@ -143,6 +143,104 @@ def test_bf16_trajectory_close_to_fp64(incremental, split_grad, premult, integ):
         chi1 = orc.misfit(res["X1"][c].reshape(d, 1))
         k1 = 0.5 * float(np.sum(res["P1"][c] ** 2))
         assert abs(res["H1"][c] - (chi1 + k1)) <= 1e-6 * abs(chi1 + k1), c
+
+
+def _bf16(a):
+    return bf16_to_f32(bf16_round(np.asarray(a, dtype=np.float32))).astype(np.float64)
+
+
+def _anchored_case(seed=12, n=520, d=384):
+    rng = np.random.default_rng(seed)
+    G = _bf16(rng.normal(size=(n, d)) / np.sqrt(d))
+    m_true = rng.normal(size=(d, 1))
+    dd = G @ m_true + rng.normal(0, 0.1, size=(n, 1))
+    var = 0.01
+    pv = 1.0
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), pv), D.LinearMatrix(G, dd, var, dtype="bfloat16")])
+    H = G.T @ G / var
+    b = (G.T @ dd / var)[:, 0]
+    mu = np.linalg.solve(H + np.eye(d) / pv, b)
+    return rng, G, dd, var, pv, post, H, b, mu
+
+
+@pytest.mark.parametrize("premult", [1, 0])
+def test_bf16_anchored_gradient_field_matches_its_definition(premult):
+    """The gradient field of anchored bf16 trajectories, evaluated through the
+    C-ABI at arbitrary positions, equals its definition computed here in
+    float64 from the same bf16 roundings:
+      premultiplied  g = f32(H x_ref - b) + bf16(H) bf16(x - x_ref) + prior'(x)
+      residual form  r = f32(G x_ref - d) + G bf16(x - x_ref),
+                     g = G^T (hi + lo planes of r / var) + prior'(x)
+    to float32 accumulation (1e-5 of the largest entry).  At x = x_ref the
+    offset plane is zero and the field is the FP64 gradient rounded once to
+    float32 (1e-6); away from it the field stays within 1e-2 of the FP64
+    gradient (bf16 rounding of H, or of G and r / var, and of the offset: each
+    2^-9 relative; a float64 emulation of this case gives 2-6e-3).  The anchor
+    is set from the host."""
+    rng, G, dd, var, pv, post, H, b, mu = _anchored_case()
+    n, d = G.shape
+    B = 6
+    prob = build_problem(post, precision="bf16", engine="gemm",
+                         options={"premult_trajectory": premult, "incremental_residual": 1, "bf16_split_grad": 2})
+    ch = Chains(prob, B)
+    x_ref = (mu + 0.01 * rng.normal(size=d)).astype(np.float32).astype(np.float64)
+    ch.set_state(np.tile(x_ref, (B, 1)))
+    ch.set_anchor(x_ref)
+    X = (x_ref[None, :] + 0.05 * rng.normal(size=(B, d))).astype(np.float32).astype(np.float64)
+    X[0] = x_ref
+    g = ch.trajectory_gradient(X)
+    g64 = X @ H - b[None, :] + X / pv
+    for c in range(B):
+        off = _bf16(X[c] - x_ref)
+        prior = X[c] / pv
+        if premult:
+            yref = (H @ x_ref - b).astype(np.float32).astype(np.float64)
+            want = yref + _bf16(H) @ off + prior
+        else:
+            rref = (G @ x_ref - dd[:, 0]).astype(np.float32).astype(np.float64)
+            w = ((rref + G @ off) / var).astype(np.float32)
+            hi = _bf16(w)
+            lo = _bf16(w.astype(np.float64) - hi)
+            want = G.T @ (hi + lo) + prior
+        assert _rel(g[c], want) < 1e-5, c
+        # at x_ref: one float32 rounding of the FP64 gradient (premultiplied), or
+        # the hi/lo planes of r / var (16 bits, 2^-17 relative; residual form)
+        assert _rel(g[c], g64[c]) < ((1e-6 if premult else 2e-5) if c == 0 else 1e-2), c
+
+
+@pytest.mark.parametrize("premult,integ", [(1, "lf"), (0, "lf"), (1, "4s")])
+def test_bf16_anchored_trajectory_is_reversible(premult, integ):
+    """A bf16 trajectory whose stage gradient depends on the position alone is
+    reversible: integrating (x1, -p1) with the same step returns to x0.  A
+    chain returns to float32 rounding of the state (below 1e-6 of the largest
+    coordinate after 25 steps) unless float32 rounding on the way back moves a
+    coordinate of x - x_ref across a bf16 rounding boundary; the field then
+    differs by one bf16 ulp of that offset and the chain returns to within
+    5e-4 (measured: at most 1.3e-4).  A float64 emulation of this case with a
+    gradient carried along the path instead (y += H_hi bf16(dx) from the
+    trajectory's start) misses by 2.5-3.7e-3 on every chain."""
+    rng, G, dd, var, pv, post, H, b, mu = _anchored_case(seed=13)
+    n, d = G.shape
+    B, L = 8, 25
+    lam = np.linalg.eigvalsh(H + np.eye(d) / pv).max()
+    prob = build_problem(post, precision="bf16", engine="gemm",
+                         options={"premult_trajectory": premult, "incremental_residual": 1, "bf16_split_grad": 1})
+    ch = Chains(prob, B)
+    X0 = (mu[None, :] + 0.05 * rng.normal(size=(B, d))).astype(np.float32).astype(np.float64)
+    ch.set_state(X0)
+    ch.set_anchor()
+    P0 = rng.normal(size=(B, d)).astype(np.float32).astype(np.float64)
+    e = np.full(B, 0.5 / np.sqrt(lam))
+    u = np.zeros(B)          # accept every finite proposal
+    fwd = ch.propose(P0, e, u, L, integ, trajectory=True)
+    assert fwd["accepted"].all()
+    X1, P1 = fwd["X1"], fwd["P1"]
+    assert _rel(X1, X0) > 1e-2                 # the trajectory moved
+    ch.set_state(X1)
+    back = ch.propose(-P1, e, u, L, integ, trajectory=True)
+    err = np.array([_rel(back["X1"][c], X0[c]) for c in range(B)])
+    assert err.min() < 1e-6, err
+    assert err.max() < 5e-4, err
 
 
 @pytest.mark.parametrize("precision,cfg,B,pm", [("bf16", 1, 256, 0), ("bf16", 3, 256, 0), ("bf16", 3, 200, 0),
@@ -180,7 +278,9 @@ def test_split_tail_matches_unsplit(precision, cfg, B, pm):
         ch.set_state(X0)
         res[name] = ch.propose(P0, e, u, L, "lf", trajectory=True)
         grad[name] = prob.gradient_batch(X0[:4])
-    same = 1e-5 if precision == "bf16" else 1e-12
+    # bf16: float32 summation order, and where it moves a coordinate of
+    # x - x_ref across a bf16 rounding boundary, one bf16 ulp of that offset
+    same = 1e-4 if precision == "bf16" else 1e-12
     assert _rel(res["split"]["X1"], res["plain"]["X1"]) < same
     assert _rel(res["split"]["P1"], res["plain"]["P1"]) < same
     assert _rel(res["split"]["H0"], res["plain"]["H0"]) < 1e-12

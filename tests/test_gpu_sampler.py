@@ -79,7 +79,9 @@ def test_in_memory_results_and_diagnostics(capsys):
     out = capsys.readouterr().out
     assert "Detailed statistics" in out
     d = s.get_diagnostics()
-    assert d["trajectory gradient kernels"][1] > 0
+    # a 5-parameter target runs on the fused engine: one kernel per proposal
+    assert d["residual GEMM / fused proposal"][1] > 0
+    assert d["element-wise / Metropolis"][1] == 0
 
 
 def test_gpu_misfit_gradient_methods():
