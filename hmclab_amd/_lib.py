@@ -64,6 +64,7 @@ SIGNATURES = {
     "hmc_problem_set_option": [_p, C.c_char_p, C.c_double],
     "hmc_problem_destroy": [_p],
     "hmc_misfit_batch": [_p, _dp, _i64, _dp],
+    "hmc_misfit_batch_noalloc": [_p, _dp, _i64, _dp],
     "hmc_gradient_batch": [_p, _dp, _i64, _dp],
     "hmc_chains_create": [_p, _i64, C.POINTER(_p)],
     "hmc_chains_destroy": [_p],

@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libhmc.so")
-SOURCES = ["hmc_capi.hip", "hmc_elem.hip", "hmc_gemm.hip", "hmc_fused.hip", "hmc_surrogate.hip", "hmc_set.hip"]
+SOURCES = ["hmc_capi.hip", "hmc_elem.hip", "hmc_gemm.hip", "hmc_fused.hip", "hmc_surrogate.hip", "hmc_set.hip",
+           "hmc_noalloc.hip"]
 ARCH = os.environ.get("HMC_OFFLOAD_ARCH", "gfx950")
 
 # packed FP32 instructions are disabled for all device code

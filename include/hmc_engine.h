@@ -172,6 +172,20 @@ int hmc_problem_destroy(hmc_problem* p);
  * Replaces: BayesRule.misfit (base.py:1036-1043). */
 int hmc_misfit_batch(hmc_problem* p, const double* M, int64_t B, double* out);
 
+/* hmc_misfit_batch for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  M_dev (B x dimensions,
+ * float64, row-major) and out_dev (B) are DEVICE pointers owned by the
+ * caller; every other buffer lives in a static 32 MiB device array per GPU,
+ * so the call makes no hipMalloc/hipFree.  Fails with HMC_ERR_ARG when the
+ * batch needs more than 32 MiB (the models in the padded state layout plus
+ * the endpoint partial sums: use fewer models per call), and with
+ * HMC_ERR_STATE when the problem's premultiplied operator H (dense, n > d)
+ * has not been built yet by an earlier misfit evaluation.  Runs on the
+ * problem's stream and returns after it finished.  Calls that overlap on one
+ * GPU share the array and race. */
+int hmc_misfit_batch_noalloc(hmc_problem* p, const double* M_dev, int64_t B, double* out_dev);
+
 /* Batched gradient for B models -> G_out (B x dimensions).
  * Replaces: BayesRule.gradient (base.py:1045-1054). */
 int hmc_gradient_batch(hmc_problem* p, const double* M, int64_t B, double* out);
