@@ -593,7 +593,7 @@ def run_tomography(args):
     dtype = np.float32 if args.precision == "f32" else np.float64
     post = D.BayesRule([D.Normal(np.full((d, 1), prob_d["prior_mean"]), sm2),
                         D.LinearMatrix(G.astype(dtype), dd.astype(dtype) if dtype == np.float32 else dd,
-                                       sd2, dtype=dtype if dtype == np.float32 else None)])
+                                       sd2, dtype=dtype)])
     prob = build_problem(post, precision=args.precision, mass_diagonal=mass,
                          options={"fused_tile_chains": args.fused_tile_chains} if args.fused_tile_chains else None)
     ch = Chains(prob, B)
@@ -769,7 +769,7 @@ def run_small(args):
     rng = np.random.default_rng(args.seed)
     G = rng.normal(size=(100, 50))
     dd = G @ rng.normal(size=(50, 1)) + rng.normal(0, 0.1, size=(100, 1))
-    post = D.BayesRule([D.Normal(np.zeros((50, 1)), 1.0), D.LinearMatrix(G, dd, 0.01)])
+    post = D.BayesRule([D.Normal(np.zeros((50, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, dtype=np.float64)])
     mu, Sigma = O.analytic_linear_gaussian(G, dd, 0.01, np.zeros(50), 1.0)
     prob = build_problem(post, precision="f64")
     ch = Chains(prob, 1)

@@ -9,12 +9,20 @@ bounds handling and error messages follow the reference; ``misfit`` and
 ``gradient`` run on the GPU through libhmc.so, and ``misfit_batch`` /
 ``gradient_batch`` evaluate many models (columns) in one call.
 
-Differences from the reference, by design:
-* ``LinearMatrix(..., dtype=...)`` forwards ``dtype`` to the stored G and d.
-  The reference wrapper drops it and its backends silently round G and d to
-  float32 (LinearMatrix.py:25-31,148,152).  Pass ``dtype=numpy.float32`` for
-  that behaviour, or ``dtype="bfloat16"`` to define the target by
-  bf16-rounded G.
+Storage type of ``LinearMatrix``: by default G, d and an array covariance
+are stored in float32, as the reference's backends do (their ``dtype``
+defaults to ``numpy.single`` and the wrapper never forwards one,
+LinearMatrix.py:30-34,148-155,318-329), so ``LinearMatrix(G, d, var)``
+defines the same target as in the reference.  Differences by design:
+* ``dtype=numpy.float64`` keeps float64 storage (the reference has no way to
+  ask for it: its wrapper reinterprets the array's memory instead,
+  LinearMatrix.py:30-31); ``dtype="bfloat16"`` defines the target by
+  bf16-rounded G with float64 data.
+* The dense residual gradient uses the stored G; the reference's uses the
+  caller's uncast G (LinearMatrix.py:206-208), about 1e-7 relative apart in
+  float32 storage.  The premultiplied b = G^T C^-1 d is built from the
+  stored G as well (the reference mixes the caller's G into it, :176), so
+  both forms define the same target.
 * ``premultiplication`` selects the algebra of a dense likelihood on the
   device as in LinearMatrix.py:163-207: True evaluates the Metropolis misfit
   and the trajectory gradients through H = G^T C^-1 G and b = G^T C^-1 d,
@@ -24,7 +32,12 @@ Differences from the reference, by design:
   form.
 * A full data covariance (n x n) is handled by pre-whitening with its
   Cholesky factor (G' = L^-1 G, d' = L^-1 d, unit variance), which gives the
-  same misfit and gradient as LinearMatrix.py:226-305.
+  same misfit and gradient as LinearMatrix.py:226-305 and 444-481.
+* A full-covariance ``Normal`` prior (base.py:531,553-574) becomes extra rows
+  of the dense likelihood: with the precision P = R R^T (R lower Cholesky),
+  0.5 (m - mu)^T P (m - mu) = 0.5 ||R^T m - R^T mu||^2, rows R^T, data R^T mu,
+  unit variance.  A sparse likelihood cannot take these dense rows
+  (NotImplementedError).
 """
 from __future__ import annotations
 
@@ -144,7 +157,7 @@ class _AbstractDistribution:
     def _contribute(self, spec: "_Spec"):
         raise TypeError(
             f"{type(self).__name__} has no GPU implementation; the engine supports "
-            "Normal (diagonal), Laplace, Uniform, LinearMatrix and BayesRule of those."
+            "Normal, Laplace, Uniform, LinearMatrix and BayesRule of those."
         )
 
 
@@ -214,13 +227,21 @@ class Normal(_AbstractDistribution):
 
     def _contribute(self, spec):
         if not self.diagonal:
-            inv = np.asarray(self.inverse_covariance)
+            inv = np.asarray(self.inverse_covariance, dtype=np.float64)
             if inv.ndim == 2 and inv.shape == (self.dimensions, self.dimensions) and np.count_nonzero(
                 inv - np.diag(np.diag(inv))
             ) == 0:
                 spec.priors.append((_lib.PRIOR_NORMAL, self.means, np.diag(inv), self.normalization_constant))
                 return
-            raise NotImplementedError("full-covariance Normal priors are not supported by the GPU engine")
+            # 0.5 (m - mu)^T P (m - mu) with P = inverse_covariance = R R^T
+            # (base.py:553-574): whitened rows R^T, data R^T mu
+            R = np.linalg.cholesky(0.5 * (inv + inv.T))
+            spec.rows.append((np.ascontiguousarray(R.T), R.T @ self.means[:, 0]))
+            if self.normalization_constant != 0.0:
+                # the constant rides on a zero-weight diagonal term
+                spec.priors.append((_lib.PRIOR_NORMAL, self.means, np.zeros(self.dimensions),
+                                    self.normalization_constant))
+            return
         inv = np.broadcast_to(np.asarray(self.inverse_covariance, dtype=np.float64).reshape(-1), (self.dimensions,))
         spec.priors.append((_lib.PRIOR_NORMAL, self.means, inv, self.normalization_constant))
 
@@ -354,7 +375,11 @@ class LinearMatrix(_AbstractDistribution):
         self.premultiplication = premultiplication   # device algebra of a dense G (None: n > d)
         self.use_mkl = use_mkl
         self.g_dtype = None
-        if dtype is not None and str(dtype) in ("bfloat16", "bf16"):
+        if dtype is None:
+            # the reference's backends store G, d and an array covariance in
+            # float32 (dtype=numpy.single, LinearMatrix.py:148-155,318-329)
+            dtype = np.float32
+        if str(dtype) in ("bfloat16", "bf16"):
             # the target is defined by bf16-rounded G; data and covariance stay float64
             self.g_dtype = "bf16"
             G = G.astype(np.float32)
@@ -436,6 +461,28 @@ class _Spec:
     def __init__(self):
         self.priors = []
         self.likelihood = None
+        self.rows = []        # whitened dense rows of full-covariance priors: (R^T, R^T mu)
+
+
+def _stacked(lik, rows):
+    """One dense whitened operator from the likelihood and the prior rows:
+    G = [G_w; R_k^T], d = [d_w; R_k^T mu_k], variances [var_w; 1]."""
+    d = rows[0][0].shape[1]
+    Gs, ds, vs = [], [], []
+    if lik is not None:
+        if sp.issparse(lik.G):
+            raise NotImplementedError(
+                "a full-covariance Normal prior with a sparse LinearMatrix is not supported by the GPU engine")
+        Gw, dw, varw = lik._whitened()
+        n = Gw.shape[0]
+        Gs.append(np.asarray(Gw, dtype=np.float64))
+        ds.append(np.asarray(dw, dtype=np.float64).reshape(n, 1))
+        vs.append(np.broadcast_to(np.asarray(varw, dtype=np.float64).reshape(-1), (n,)))
+    for Rt, b in rows:
+        Gs.append(Rt)
+        ds.append(b.reshape(d, 1))
+        vs.append(np.ones(d))
+    return np.vstack(Gs), np.vstack(ds), np.concatenate(vs)
 
 
 def build_problem(dist: _AbstractDistribution, precision: str = "f64", engine: str = "auto",
@@ -452,7 +499,12 @@ def build_problem(dist: _AbstractDistribution, precision: str = "f64", engine: s
         prob.set_option("premult_trajectory", pm)
     for k, v in (options or {}).items():
         prob.set_option(k, v)
-    if lik is not None:
+    if spec.rows:
+        # full-covariance priors: extra whitened rows of one dense operator,
+        # float64 (bf16- or float32-exact likelihood rows stay exact)
+        G, d, var = _stacked(lik, spec.rows)
+        prob.set_dense(G, d, var)
+    elif lik is not None:
         G, d, var = lik._whitened()
         if sp.issparse(G):
             prob.set_csr(G, d, var)

@@ -244,6 +244,10 @@ class HMC:
         if self.autotuning:
             self.acceptance_rates = np.empty((proposals, B))
             self.stepsizes = np.empty((proposals, B))
+            if self.rng_mode == "device":
+                # the device records exp(H0 - H1) and the pre-update step of every
+                # proposal, as HMC.autotune does (Samplers.py:1496-1497)
+                self._chains.set_tuning_history(proposals)
         self._chunk = chunk
         if self.diagnostic_mode:
             self._chains.enable_timing(True)
@@ -438,7 +442,11 @@ class HMC:
         self.current_model = X[0].reshape(d, 1) if B == 1 else X.T.copy()
         self.current_x = float(chi[0]) if B == 1 else chi
         self.stepsize = float(self._step[0]) if B == 1 else self._step.copy()
-        if self.autotuning and self.rng_mode == "host":
+        if self.autotuning:
+            if self.rng_mode == "device":
+                rates, steps = self._chains.tuning_history()
+                self.acceptance_rates, self.stepsizes = rates, steps
+            # Samplers.py:1421-1422 keeps the entries before current_proposal
             self.acceptance_rates = self.acceptance_rates[: self.current_proposal]
             self.stepsizes = self.stepsizes[: self.current_proposal]
         rate = np.sum(self.accepted_proposals) / B / (self.current_proposal + 1)
@@ -449,11 +457,8 @@ class HMC:
             w.write_attribute("runtime", str(self.end_time - self.start_time))
             w.write_attribute("runtime_seconds", (self.end_time - self.start_time).total_seconds())
             if self.autotuning:
-                if self.rng_mode == "host":
-                    w.write_attribute("acceptance_rates", self.acceptance_rates[:, c:c + 1])
-                    w.write_attribute("stepsizes", self.stepsizes[:, c:c + 1])
-                else:
-                    w.write_attribute("stepsizes", np.array([[self._step[c]]]))
+                w.write_attribute("acceptance_rates", self.acceptance_rates[:, c:c + 1])
+                w.write_attribute("stepsizes", self.stepsizes[:, c:c + 1])
         self._close_files()
         if self._memory is not None:
             self.results = (np.stack(self._memory, axis=2) if self._memory

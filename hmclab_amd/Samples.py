@@ -4,37 +4,58 @@ A ``.npy`` samples file holds an (N, d+1) float64 array on disk (row k =
 sample k, last column = misfit) and is read back transposed to (d+1, N)
 (Samples.py:161,325).  Attributes live in a pickle side-file
 ``<filename>.pkl`` (Samples.py:165-174).  The writer appends rows and
-rewrites a fixed-size header in place, so the file is a valid ``.npy``
-after every append.  HDF5 (``.h5``) output needs h5py, which this image
-does not provide; asking for it raises a clear error.
+rewrites the header in place, so the file is a valid ``.npy`` after every
+append.  The header is NPY version 2.0 with 64 spare bytes for the shape to
+grow, the layout the reference's appender reopens (it accepts version 2.0
+only, with exactly that header length, hmclab/Helpers/AppendNPY.py:65-85),
+so a reference process can keep appending to a file written here.  HDF5
+(``.h5``) output needs h5py, which this image does not provide; asking for it
+raises a clear error.
 """
 from __future__ import annotations
 
+import io
 import os
 import pickle
 import struct
 
 import numpy as np
 
-_HEADER_BYTES = 256   # magic + length + padded dict; room for any shape
+SPARE_HEADER_BYTES = 64
 
 
-def _header(rows: int, cols: int) -> bytes:
-    d = "{'descr': '<f8', 'fortran_order': False, 'shape': (%d, %d), }" % (rows, cols)
-    body_len = _HEADER_BYTES - 10
-    d = d.ljust(body_len - 1) + "\n"
-    return b"\x93NUMPY" + bytes([1, 0]) + struct.pack("<H", body_len) + d.encode("latin1")
+def npy_header(rows: int, cols: int, spare: bool = True) -> bytes:
+    """NPY 2.0 header of a C-order float64 (rows, cols) array; with `spare`,
+    64 more bytes of blank padding (counted in the header length) so the
+    shape can grow without moving the data."""
+    buf = io.BytesIO()
+    np.lib.format.write_array_header_2_0(buf, {"descr": "<f8", "fortran_order": False, "shape": (rows, cols)})
+    h = bytearray(buf.getvalue())
+    if spare:
+        h[-1:] = b" " * SPARE_HEADER_BYTES + b"\n"      # the closing newline moves to the end
+        struct.pack_into("<I", h, 8, len(h) - 12)
+    return bytes(h)
 
 
 class _AppendableNPY:
-    """Row-appendable float64 .npy file with a fixed-size version-1.0 header."""
+    """Row-appendable float64 .npy file whose header length stays fixed."""
 
     def __init__(self, filename: str, cols: int):
         self.filename = filename
         self.cols = cols
         self.rows = 0
         self.fp = open(filename, "wb")
-        self.fp.write(_header(0, cols))
+        self.header = npy_header(0, cols)
+        self.fp.write(self.header)
+
+    def _header_for(self, rows: int) -> bytes:
+        h = npy_header(rows, self.cols)
+        if len(h) != len(self.header):
+            # the shape grew into the spare bytes
+            h = npy_header(rows, self.cols, spare=False)
+        if len(h) != len(self.header):
+            raise OSError(f"samples file {self.filename}: no header room left for shape ({rows}, {self.cols})")
+        return h
 
     def append(self, rows: np.ndarray):
         rows = np.ascontiguousarray(rows, dtype="<f8")
@@ -43,7 +64,7 @@ class _AppendableNPY:
         rows.tofile(self.fp)
         self.rows += rows.shape[0]
         self.fp.seek(0)
-        self.fp.write(_header(self.rows, self.cols))
+        self.fp.write(self._header_for(self.rows))
         self.fp.flush()
 
     def close(self):

@@ -75,6 +75,11 @@ SIGNATURES = {
     "hmc_chains_propose": [_p, _dp, _dp, _dp, _i32, _i32, _ip, _dp, _dp, _dp, _dp],
     "hmc_chains_run": [_p, _i64, _i32, _i32, _dp, _i32, _i32, C.c_double, C.c_double,
                        C.c_uint64, _i64, _i64, _lp],
+    "hmc_chains_set_tuning_history": [_p, _i64],
+    "hmc_chains_tuning_history_count": [_p, _lp],
+    "hmc_chains_read_tuning_history": [_p, _i64, _i64, _dp, _dp],
+    "hmc_integrator_schedule": [_i32, _i32, _dp, _dp, _i64, _lp],
+    "hmc_philox4x32_10": [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
     "hmc_chains_set_trace": [_p, _i64, _i64],
     "hmc_chains_trace_count": [_p, _lp],
     "hmc_chains_read_trace": [_p, _i64, _i64, _dp, _dp],

@@ -210,7 +210,8 @@ void time_collect(Chains& c) {
 
 // ---- proposal drivers ----------------------------------------------------------
 static void gemm_proposal(Chains& c, bool host_p, int L, int integ, uint64_t seed, int64_t prop,
-                          int randomize, int autotune, double target, double lr, int64_t slot) {
+                          int randomize, int autotune, double target, double lr, int64_t slot,
+                          int64_t hist_slot = -1) {
   double first = 0.0;
   std::vector<Stage> st;
   integrator_schedule(integ, L, first, st);
@@ -221,7 +222,7 @@ static void gemm_proposal(Chains& c, bool host_p, int L, int integ, uint64_t see
     else gemm_kick_drift(c, s.kick, s.drift, s.drift2, k == 0);
   }
   gemm_endpoint(c, c.xt, c.p->state_dtype);
-  elem_metropolis(c, true, autotune, target, lr, prop, slot);
+  elem_metropolis(c, true, autotune, target, lr, prop, slot, hist_slot);
 }
 
 static void evaluate_misfit(Chains& c, const void* X, double* out_dev) {
@@ -854,10 +855,17 @@ int hmc_chains_run(hmc_chains* hc, int64_t proposals, int L, int integrator, dou
         HMC_REQUIRE(c.tr_n < c.tr_cap, HMC_ERR_STATE, "trace capacity exceeded");
         slot = c.tr_n++;
       }
+      int64_t hslot = -1;
+      if (autotune && c.hist_cap > 0) {
+        HMC_REQUIRE(c.hist_n < c.hist_cap, HMC_ERR_STATE, "tuning history capacity exceeded");
+        hslot = c.hist_n++;
+      }
       if (gemm)
-        gemm_proposal(c, false, L, integrator, seed, prop, randomize, autotune, target, learning_rate, slot);
+        gemm_proposal(c, false, L, integrator, seed, prop, randomize, autotune, target, learning_rate, slot,
+                      hslot);
       else
-        fused_propose(c, false, L, integrator, seed, prop, randomize, autotune, target, learning_rate, slot);
+        fused_propose(c, false, L, integrator, seed, prop, randomize, autotune, target, learning_rate, slot,
+                      hslot);
       // replica exchange after the proposal, as in the reference's loop (Samplers.py:589-595)
       if (c.ex_pairs && prop % c.ex_interval == 0 && prop / c.ex_interval < c.ex_events)
         elem_exchange(c, c.ex_pairs + (prop / c.ex_interval) * 2 * c.ex_npairs, c.ex_npairs, nullptr, seed, prop,
@@ -869,6 +877,73 @@ int hmc_chains_run(hmc_chains* hc, int64_t proposals, int L, int integrator, dou
       HMC_HIP(hipMemcpyAsync(accepted_total, c.acc_count, c.B * 8, hipMemcpyDeviceToHost, P.stream));
     HMC_HIP(hipStreamSynchronize(P.stream));
     time_collect(c);
+  });
+}
+
+int hmc_chains_set_tuning_history(hmc_chains* hc, int64_t capacity) {
+  return guarded("hmc_chains_set_tuning_history", [&] {
+    HMC_REQUIRE(hc && capacity >= 0, HMC_ERR_ARG, "invalid argument");
+    Chains& c = hc->c;
+    HMC_HIP(hipStreamSynchronize(c.p->stream));
+    for (double* ptr : {c.hist_rate, c.hist_step}) {
+      if (!ptr) continue;
+      (void)hipFree(ptr);
+      c.owned.erase(std::remove(c.owned.begin(), c.owned.end(), (void*)ptr), c.owned.end());
+    }
+    c.hist_rate = c.hist_step = nullptr;
+    c.hist_cap = c.hist_n = 0;
+    if (capacity > 0) {
+      c.hist_rate = (double*)dev_alloc(c.owned, (size_t)capacity * c.B * 8);
+      c.hist_step = (double*)dev_alloc(c.owned, (size_t)capacity * c.B * 8);
+      c.hist_cap = capacity;
+    }
+  });
+}
+
+int hmc_chains_tuning_history_count(hmc_chains* hc, int64_t* count) {
+  return guarded("hmc_chains_tuning_history_count", [&] {
+    HMC_REQUIRE(hc && count, HMC_ERR_ARG, "NULL argument");
+    *count = hc->c.hist_n;
+  });
+}
+
+int hmc_chains_read_tuning_history(hmc_chains* hc, int64_t first, int64_t count, double* rates, double* steps) {
+  return guarded("hmc_chains_read_tuning_history", [&] {
+    HMC_REQUIRE(hc, HMC_ERR_ARG, "NULL chains");
+    Chains& c = hc->c;
+    HMC_REQUIRE(first >= 0 && count >= 0 && first + count <= c.hist_n, HMC_ERR_ARG,
+                "tuning history range out of bounds");
+    if (count == 0) return;
+    const size_t off = (size_t)first * c.B, len = (size_t)count * c.B * 8;
+    if (rates) HMC_HIP(hipMemcpyAsync(rates, c.hist_rate + off, len, hipMemcpyDeviceToHost, c.p->stream));
+    if (steps) HMC_HIP(hipMemcpyAsync(steps, c.hist_step + off, len, hipMemcpyDeviceToHost, c.p->stream));
+    HMC_HIP(hipStreamSynchronize(c.p->stream));
+  });
+}
+
+int hmc_integrator_schedule(int integrator, int L, double* first_drift, double* stages, int64_t capacity,
+                            int64_t* count) {
+  return guarded("hmc_integrator_schedule", [&] {
+    HMC_REQUIRE(first_drift && count && L > 0, HMC_ERR_ARG, "invalid argument");
+    std::vector<Stage> st;
+    integrator_schedule(integrator, L, *first_drift, st);
+    *count = (int64_t)st.size();
+    HMC_REQUIRE(stages == nullptr || (int64_t)st.size() <= capacity, HMC_ERR_ARG, "stage buffer too small");
+    if (stages)
+      for (size_t k = 0; k < st.size(); ++k) {
+        stages[3 * k] = st[k].kick;
+        stages[3 * k + 1] = st[k].drift;
+        stages[3 * k + 2] = st[k].drift2;
+      }
+  });
+}
+
+int hmc_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* out) {
+  return guarded("hmc_philox4x32_10", [&] {
+    HMC_REQUIRE(counter && key && out, HMC_ERR_ARG, "NULL argument");
+    U32x4 c{{counter[0], counter[1], counter[2], counter[3]}};
+    const U32x4 r = philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = r.v[i];
   });
 }
 

@@ -118,7 +118,8 @@ __global__ __launch_bounds__(EW_NT) void k_metropolis(
     int64_t ldx, int64_t d, const double* partial, int64_t nmb, int64_t ldp, double mconst,
     const double* k0, const double* u, double* h0o, double* h1o, int32_t* acc, int64_t* acc_count,
     double* step, int autotune, double target, double lr, int64_t prop_index, T* tr_x,
-    double* tr_chi, int64_t slot, int64_t B, const double* __restrict__ beta, DevModel md) {
+    double* tr_chi, int64_t slot, int64_t B, const double* __restrict__ beta, double* hist_rate,
+    double* hist_step, int64_t hist_slot, DevModel md) {
   __shared__ double scratch[EW_NT / 64 + 1];
   __shared__ int accepted;
   __shared__ double chi_new;
@@ -132,7 +133,11 @@ __global__ __launch_bounds__(EW_NT) void k_metropolis(
     const double H1 = bt * chi1 + k1;
     const double a = exp(H0 - H1);
     if (autotune) {
-      // Samplers.py:1502-1522
+      // Samplers.py:1496-1522: record the rate and the step before the update
+      if (hist_slot >= 0) {
+        hist_rate[hist_slot * B + c] = a;
+        hist_step[hist_slot * B + c] = step[c];
+      }
       const double wgt = pow((double)(prop_index + 1), -lr);
       const double ar = isnan(a) ? 0.0 : a;
       double s = step[c] - wgt * (target - fmin(ar, 1.0));
@@ -302,25 +307,28 @@ void elem_drift(Chains& c, double drift) {
 }
 
 void elem_metropolis(Chains& c, bool have_partials, int autotune, double target,
-                     double learning_rate, int64_t prop_index, int64_t trace_slot) {
+                     double learning_rate, int64_t prop_index, int64_t trace_slot, int64_t hist_slot) {
   Problem& p = *c.p;
   const int64_t nmb = have_partials ? endpoint_blocks(c) : 0;
   const double mconst = have_partials ? endpoint_const(p) : p.misfit_const;
   hipEvent_t ev;
   time_begin(c, 2, ev);
   if (trace_slot >= 0) HMC_REQUIRE(trace_slot < c.tr_cap, HMC_ERR_STATE, "trace capacity exceeded");
+  if (hist_slot >= 0) HMC_REQUIRE(hist_slot < c.hist_cap, HMC_ERR_STATE, "tuning history capacity exceeded");
   if (p.state_dtype == HMC_F64)
     hipLaunchKernelGGL(k_metropolis<double>, dim3(c.B), dim3(EW_NT), 0, p.stream, (double*)c.x0,
                        c.chi0, (const double*)c.xt, (const double*)c.pt, c.ldx, p.d, c.partial, nmb,
                        c.B_pad, mconst, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
                        autotune, target, learning_rate, prop_index, (double*)c.tr_x, c.tr_chi,
-                       trace_slot, c.B, (const double*)c.beta, p.model());
+                       trace_slot, c.B, (const double*)c.beta, c.hist_rate, c.hist_step, hist_slot,
+                       p.model());
   else
     hipLaunchKernelGGL(k_metropolis<float>, dim3(c.B), dim3(EW_NT), 0, p.stream, (float*)c.x0,
                        c.chi0, (const float*)c.xt, (const float*)c.pt, c.ldx, p.d, c.partial, nmb,
                        c.B_pad, mconst, c.k0, c.u, c.h0, c.h1, c.acc, c.acc_count, c.step,
                        autotune, target, learning_rate, prop_index, (float*)c.tr_x, c.tr_chi,
-                       trace_slot, c.B, (const double*)c.beta, p.model());
+                       trace_slot, c.B, (const double*)c.beta, c.hist_rate, c.hist_step, hist_slot,
+                       p.model());
   HMC_HIP(hipGetLastError());
   time_end(c, 2, ev);
 }

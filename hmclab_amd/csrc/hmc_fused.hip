@@ -182,6 +182,9 @@ struct FusedArgs {
   void* tr_x;
   double* tr_chi;
   int64_t slot;
+  double* hist_rate;   // autotuning history [hist][B] (Samplers.py:1496-1497) or null
+  double* hist_step;
+  int64_t hist_slot;
   int64_t ldx, B;
   const double* dobs;
   const double* var;
@@ -417,6 +420,10 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
       const double H1 = s_beta[c] * s_chi1[c] + s_k1[c];
       const double av = exp(H0 - H1);
       if (a.autotune) {
+        if (a.hist_slot >= 0) {
+          a.hist_rate[a.hist_slot * a.B + ch] = av;
+          a.hist_step[a.hist_slot * a.B + ch] = a.step[ch];
+        }
         const double wgt = pow((double)(a.prop + 1), -a.lr);
         const double ar = isnan(av) ? 0.0 : av;
         double sv = a.step[ch] - wgt * (a.target - fmin(ar, 1.0));
@@ -710,7 +717,7 @@ static void dispatch(const Problem& p, int ct, int kmax, F&& f) {
 
 void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed, int64_t prop,
                    int randomize, int autotune, double target, double learning_rate,
-                   int64_t trace_slot) {
+                   int64_t trace_slot, int64_t hist_slot) {
   Problem& p = *c.p;
   size_t lds = 0;
   const int ct = fused_ct(p, c.B, lds);
@@ -720,6 +727,7 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
   integrator_schedule(integrator, L, first, st);
   const int P = (int)st.size() / L;
   if (trace_slot >= 0) HMC_REQUIRE(trace_slot < c.tr_cap, HMC_ERR_STATE, "trace capacity exceeded");
+  if (hist_slot >= 0) HMC_REQUIRE(hist_slot < c.hist_cap, HMC_ERR_STATE, "tuning history capacity exceeded");
   fused_premult_build(c);
 
   FusedArgs a{};
@@ -739,6 +747,9 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
   a.tr_x = c.tr_x;
   a.tr_chi = c.tr_chi;
   a.slot = trace_slot;
+  a.hist_rate = c.hist_rate;
+  a.hist_step = c.hist_step;
+  a.hist_slot = hist_slot;
   a.ldx = c.ldx;
   a.B = c.B;
   a.dobs = p.dobs;

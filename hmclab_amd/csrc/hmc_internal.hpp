@@ -147,6 +147,13 @@ struct Chains {
   int64_t* acc_count = nullptr;
   int32_t* oob = nullptr;   // per-chain out-of-bounds flag (gradient_batch)
 
+  // autotuning history of device runs (Samplers.py:1419-1426,1494-1522): per
+  // proposal and chain the acceptance value exp(H0 - H1) and the step before
+  // the update, [hist_cap][B]
+  double* hist_rate = nullptr;
+  double* hist_step = nullptr;
+  int64_t hist_cap = 0, hist_n = 0;
+
   // trace
   void* tr_x = nullptr;
   double* tr_chi = nullptr;
@@ -260,7 +267,8 @@ void elem_refresh(Chains& c, bool host_p, double drift0, uint64_t seed, int64_t 
 void elem_drift(Chains& c, double drift);
 // endpoint Hamiltonian + Metropolis + state update (+ autotune, trace record)
 void elem_metropolis(Chains& c, bool have_partials, int autotune, double target,
-                     double learning_rate, int64_t prop_index, int64_t trace_slot);
+                     double learning_rate, int64_t prop_index, int64_t trace_slot,
+                     int64_t hist_slot = -1);
 // misfit of X (device, state dtype) -> out[B] using c.partial (or none)
 void elem_finish_misfit(Chains& c, const void* X, bool have_partials, double* out);
 // convert f64 host-layout rows into state dtype rows (+bf16 planes when given)
@@ -279,7 +287,7 @@ bool fused_supported(const Problem& p);
 // Whole proposal (refresh, L steps, endpoint, Metropolis) in one launch.
 void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed, int64_t prop,
                    int randomize, int autotune, double target, double learning_rate,
-                   int64_t trace_slot);
+                   int64_t trace_slot, int64_t hist_slot = -1);
 void fused_misfit(Chains& c, const void* X, double* out);
 void fused_gradient(Chains& c, const void* X, double* out);
 

@@ -27,6 +27,27 @@ def bf16_to_f32(bits: np.ndarray) -> np.ndarray:
     return (bits.astype(np.uint32) << 16).view(np.float32)
 
 
+def integrator_schedule(integrator: str, L: int):
+    """(first drift, stages (S, 3) of kick, drift, drift2) of an engine integrator,
+    in units of the local step (host only)."""
+    first = C.c_double(0.0)
+    n = C.c_int64(0)
+    call("hmc_integrator_schedule", _lib.INTEGRATORS[integrator], int(L), C.byref(first), None, 0, C.byref(n))
+    st = np.empty((n.value, 3))
+    call("hmc_integrator_schedule", _lib.INTEGRATORS[integrator], int(L), C.byref(first), dptr(st), n.value,
+         C.byref(n))
+    return first.value, st
+
+
+def philox4x32_10(counter, key):
+    """The device RNG's Philox4x32-10 block, evaluated by the library on the host."""
+    c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in counter])
+    k = (C.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (C.c_uint32 * 4)()
+    call("hmc_philox4x32_10", c, k, o)
+    return [int(v) for v in o]
+
+
 class Problem:
     """One BayesRule target on the device (likelihood, priors, bounds, mass)."""
 
@@ -192,6 +213,20 @@ class Chains:
              dptr(step), int(bool(randomize)), int(bool(autotune)), float(target), float(learning_rate),
              C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), int(first_proposal), int(thinning), lptr(accepted))
         return step, accepted
+
+    def set_tuning_history(self, capacity: int):
+        """Record exp(H0 - H1) and the pre-update step of every autotuned
+        device proposal (up to `capacity` proposals); 0 frees."""
+        call("hmc_chains_set_tuning_history", self._h, int(capacity))
+
+    def tuning_history(self):
+        """(rates, steps), each (proposals recorded, B)."""
+        n = C.c_int64(0)
+        call("hmc_chains_tuning_history_count", self._h, C.byref(n))
+        rates = np.empty((n.value, self.B))
+        steps = np.empty((n.value, self.B))
+        call("hmc_chains_read_tuning_history", self._h, 0, n.value, dptr(rates), dptr(steps))
+        return rates, steps
 
     def set_trace(self, capacity: int):
         call("hmc_chains_set_trace", self._h, int(capacity), self.d)

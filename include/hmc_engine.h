@@ -252,6 +252,34 @@ int hmc_chains_run(hmc_chains* c, int64_t proposals, int amount_of_steps, int in
                    double learning_rate, uint64_t seed, int64_t first_proposal,
                    int64_t thinning, int64_t* accepted_total);
 
+/* Autotuning history of hmc_chains_run calls with autotune != 0: for every
+ * proposal (up to `capacity`, then HMC_ERR_STATE) and chain, the acceptance
+ * value exp(H0 - H1) (may be inf or NaN) and the step size before its update.
+ * Setting a capacity (0 frees) discards the recorded history.  read copies
+ * proposals [first, first+count) as (count x B) arrays; either may be NULL.
+ * Replaces: the acceptance_rates / stepsizes arrays of HMC.autotune
+ * (Samplers.py:1494-1500) written by _close_sampler_specific (:1419-1426). */
+int hmc_chains_set_tuning_history(hmc_chains* c, int64_t capacity);
+int hmc_chains_tuning_history_count(hmc_chains* c, int64_t* count);
+int hmc_chains_read_tuning_history(hmc_chains* c, int64_t first, int64_t count, double* rates, double* steps);
+
+/* Stage coefficients of an integrator (host only, no device work), in units of
+ * the local step: the first drift, then per stage (kick, drift, drift2) as
+ * the engines apply them; L steps give L (leapfrog), 3L (3-stage) or 4L
+ * (4-stage) stages; the closing drift of a multi-stage step and the opening
+ * drift of the next share a stage (drift, drift2).  `count` receives the
+ * number of stages; `stages` (3 * capacity values) may be NULL.
+ * Replaces: the coefficient tables and stage order of _propagate_leapfrog,
+ * _propagate_3_stage_simplified and _propagate_4_stage_simplified
+ * (Samplers.py:1524-1726). */
+int hmc_integrator_schedule(int integrator, int L, double* first_drift, double* stages, int64_t capacity,
+                            int64_t* count);
+
+/* Philox4x32-10 block of the device RNG (host evaluation of the same code):
+ * out = philox(counter[4], key[2]).  No counterpart in the reference, whose
+ * draws come from numpy's default_rng; for known-answer tests. */
+int hmc_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* out);
+
 /* Device trace of recorded samples: capacity in slots; each slot holds
  * B x dimensions coordinates (stored in the trajectory precision) and B
  * misfits.  Setting a capacity discards recorded samples. */

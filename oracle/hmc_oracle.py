@@ -248,6 +248,7 @@ class LinearMatrixSimple(_Bounded):
         self.premultiplication = (
             premultiplication if premultiplication is not None else self.G.shape[0] > self.G.shape[1]
         )
+        self._G_in = G_in
         if self.premultiplication:
             if sparse:
                 if type(self.data_variance) == float:
@@ -263,7 +264,9 @@ class LinearMatrixSimple(_Bounded):
             self.Gtd0 = G_in.T @ invcov @ self.d      # un-cast G, LinearMatrix.py:176,353
             self.dtd = (self.d.T @ invcov @ self.d).item()
         else:
-            self.Gt = self.G.T
+            # the dense backend keeps the caller's (uncast) G for the gradient
+            # (LinearMatrix.py:206), the sparse one the cast G (:359)
+            self.Gt = self.G.T if sparse else np.asarray(G_in).T
 
     def misfit(self, x):
         if self.premultiplication:
@@ -276,6 +279,76 @@ class LinearMatrixSimple(_Bounded):
         if self.premultiplication:
             return self.GtG @ x - self.Gtd0
         return self.Gt @ ((self.G @ x - self.d) / self.data_variance)
+
+
+class LinearMatrixDenseCovariance(_Bounded):
+    """Dense G with a full (n, n) data covariance C.
+
+    Restates ``_LinearMatrix_dense_forward_dense_covariance``
+    (LinearMatrix.py:226-305): ``invcov = inv(C)``; premultiplied
+    ``chi = 0.5*(m^T(G^T invcov G m - 2 G_in^T invcov d) + d^T invcov d)``,
+    ``grad = G^T invcov G m - G_in^T invcov d``; residual
+    ``chi = 0.5*||U (G m - d)||^2`` with ``U`` the upper Cholesky factor of
+    invcov (``cholesky(invcov).T``), ``grad = G^T invcov (G m - d)``.
+    """
+
+    def __init__(self, G, d, data_covariance, dtype=np.float64, premultiplication=None):
+        self.dimensions = G.shape[1]
+        self.G = np.asarray(G).astype(dtype)
+        self.d = np.asarray(d).astype(dtype)
+        self.data_covariance = np.asarray(data_covariance).astype(dtype)
+        self.premultiplication = (
+            premultiplication if premultiplication is not None else self.G.shape[0] > self.G.shape[1]
+        )
+        self.invcov = np.linalg.inv(self.data_covariance)                     # :250
+        if self.premultiplication:
+            self.GtG = self.G.T @ self.invcov @ self.G                         # :254-256
+            self.Gtd0 = np.asarray(G).T @ self.invcov @ self.d
+            self.dtd = (self.d.T @ self.invcov @ self.d).item()
+        else:
+            self.Gt = self.G.T
+            self.cholesky_upper_inv_covariance = np.linalg.cholesky(self.invcov).T   # :262-264
+
+    def misfit(self, x):
+        if self.premultiplication:
+            return self.misfit_bounds(x) + (0.5 * (x.T @ (self.GtG @ x - 2 * self.Gtd0) + self.dtd)).item()
+        r = self.cholesky_upper_inv_covariance @ (self.G @ x - self.d)
+        return self.misfit_bounds(x) + (0.5 * np.linalg.norm(r) ** 2).item()
+
+    def gradient(self, x):
+        if self.premultiplication:
+            return self.GtG @ x - self.Gtd0
+        return self.Gt @ self.invcov @ (self.G @ x - self.d)                  # :291
+
+
+class LinearMatrixSparseCovariance(_Bounded):
+    """Sparse G with a sparse full data covariance C.
+
+    Restates ``_LinearMatrix_sparse_forward_sparse_covariance``
+    (LinearMatrix.py:444-481): solves with C through a sparse LU
+    factorisation (``scipy.sparse.linalg.factorized``); ``chi = 0.5*(m^T G^T
+    - d^T) C^-1 (G m - d)``, ``grad = G^T C^-1 (G m - d)``.
+    """
+
+    def __init__(self, G, d, data_covariance, dtype=np.float64):
+        from scipy.sparse import linalg as spla
+
+        self.dimensions = G.shape[1]
+        self.G = sp.csr_matrix(G).astype(dtype)
+        self.d = np.asarray(d).astype(dtype)
+        C = data_covariance if sp.issparse(data_covariance) else sp.csr_matrix(data_covariance, dtype=dtype)
+        self.data_covariance = C
+        self.Gt = self.G.T.tocsr()                                             # :462
+        self.dt = np.asarray(d).T.astype(dtype)
+        self.factorized_covariance = spla.factorized(C.tocsc())               # :464-466
+
+    def misfit(self, x):
+        r = (self.G @ x - self.d).astype(self.data_covariance.dtype)
+        return self.misfit_bounds(x) + (0.5 * ((x.T @ self.Gt - self.dt) @ self.factorized_covariance(r))).item()
+
+    def gradient(self, x):
+        r = (self.G @ x - self.d).astype(self.data_covariance.dtype)
+        return self.Gt @ self.factorized_covariance(r)
 
 
 # --------------------------------------------------------------------------- #

@@ -76,7 +76,7 @@ def _tomo_targets(t, precision):
     Gp = t["G"].astype(dtype)
     dd = t["d"].astype(dtype) if dtype == np.float32 else t["d"]
     post = D.BayesRule([D.Normal(np.full((d, 1), t["m0"]), t["sm2"]),
-                        D.LinearMatrix(Gp, dd, t["sd2"], dtype=dtype if dtype == np.float32 else None)])
+                        D.LinearMatrix(Gp, dd, t["sd2"], dtype=dtype)])
     orc = O.BayesRule([O.Normal(np.full((d, 1), t["m0"]), t["sm2"]),
                        O.LinearMatrixSimple(Gp, dd, t["sd2"], dtype=dtype, premultiplication=False)])
     return post, orc

@@ -51,7 +51,7 @@ def _dense_case(seed, n, d, prior="normal", bounds=False, sparse=False, density=
         lo, hi = -3.0 * np.ones(d), 3.0 * np.ones(d)
         comps_gpu.append(D.Uniform(lo, hi))
         comps_or.append(O.Uniform(lo, hi))
-    comps_gpu.append(D.LinearMatrix(Gp, dd, var, dtype=None if g_dtype == np.float64 else g_dtype))
+    comps_gpu.append(D.LinearMatrix(Gp, dd, var, dtype=g_dtype))
     comps_or.append(O.LinearMatrixSimple(Gp, dd, var, dtype=g_dtype, premultiplication=False))
     return D.BayesRule(comps_gpu), O.BayesRule(comps_or), rng
 
@@ -310,7 +310,8 @@ def test_premultiplied_endpoint_matches_residual_form(precision, g_dtype, d, sym
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
     var = rng.uniform(0.005, 0.02, size=(n, 1))
-    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, var, dtype=g_dtype)])
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0),
+                        D.LinearMatrix(G, dd, var, dtype=g_dtype if g_dtype is not None else np.float64)])
     orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
                        O.LinearMatrixSimple(G, dd, var, dtype=np.float64, premultiplication=False)])
     X = (rng.normal(size=(B, d)) * 0.5).astype(np.float32).astype(np.float64)
@@ -483,7 +484,7 @@ def _c1(seed=0):
     G = rng.normal(size=(100, 50))
     m_true = rng.normal(size=(50, 1))
     dd = G @ m_true + rng.normal(0, 0.1, size=(100, 1))
-    post = D.BayesRule([D.Normal(np.zeros((50, 1)), 1.0), D.LinearMatrix(G, dd, 0.01)])
+    post = D.BayesRule([D.Normal(np.zeros((50, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, dtype=np.float64)])
     orc = O.BayesRule([O.Normal(np.zeros((50, 1)), 1.0),
                        O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
     return G, dd, post, orc
@@ -503,8 +504,8 @@ def test_seeded_chain_matches_oracle(engine, stepsize):
     ref, rec, accepted = O.sample_chain(orc, O.Unit(50), seed=7, stepsize=stepsize, amount_of_steps=10,
                                         proposals=60, initial_model=mu, record=True)
     assert got.shape == ref.shape
-    assert _rel(got[:-1], ref[:-1]) < 1e-9            # positions
-    assert _rel(got[-1], ref[-1]) < 1e-9              # misfits
+    assert _rel(got[:-1], ref[:-1]) < 1e-10           # positions
+    assert _rel(got[-1], ref[-1]) < 1e-10             # misfits
     assert sam.accepted_proposals == accepted
     if stepsize < 0.01:
         assert accepted > 20                          # the chain moves (stable step)
@@ -518,7 +519,7 @@ def test_device_rng_posterior_matches_analytic(engine):
     n, d, B = 40, 6, 256
     G = rng.normal(size=(n, d))
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.3, size=(n, 1))
-    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.09)])
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.09, dtype=np.float64)])
     mu, Sigma = O.analytic_linear_gaussian(G, dd, 0.09, np.zeros(d), 1.0)
     prob = build_problem(post, precision="f64", engine=engine)
     ch = Chains(prob, B)
@@ -589,7 +590,7 @@ def test_premultiplication_keyword_selects_the_algebra():
         out = []
         for kw, opts in ((dict(premultiplication=flag), None),
                          ({}, {"endpoint_premult": int(flag), "premult_trajectory": int(flag)})):
-            post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, **kw)])
+            post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, dtype=np.float64, **kw)])
             prob = build_problem(post, precision="f64", engine="gemm", options=opts)
             chi = prob.misfit_batch(X)
             ch = Chains(prob, B)
@@ -601,3 +602,123 @@ def test_premultiplication_keyword_selects_the_algebra():
         assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
         for b in range(B):
             assert _rel(out[0][0][b], orc.misfit(X[b].reshape(d, 1))) < 1e-11
+
+
+# --------------------------------------------------------------------------- #
+def test_reference_default_float32_storage():
+    """LinearMatrix(G, d, var) without dtype stores G and d in float32, as the
+    reference's backends do (LinearMatrix.py:148-155,318-329): misfits equal
+    the oracle with float32 storage to 1e-10 (FP64 arithmetic on the same
+    float32 values), gradients to 1e-6 (the reference's dense gradient uses the
+    caller's uncast G, :206-208), and differ from float64 storage by more than
+    1e-9; a dense and a sparse G, scalar and vector variance."""
+    rng = np.random.default_rng(41)
+    n, d = 90, 40
+    G = rng.normal(size=(n, d)) / np.sqrt(d)
+    dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+    X = rng.normal(size=(5, d))
+    for Gp, var in ((G, 0.01), (sp.csr_matrix(np.where(np.abs(G) > 0.1, G, 0.0)), rng.uniform(0.005, 0.02, (n, 1)))):
+        post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(Gp, dd, var)])
+        orc32 = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
+                             O.LinearMatrixSimple(Gp, dd, var, dtype=np.float32, premultiplication=False)])
+        orc64 = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
+                             O.LinearMatrixSimple(Gp, dd, var, dtype=np.float64, premultiplication=False)])
+        prob = build_problem(post, precision="f64")
+        chi = prob.misfit_batch(X)
+        g = prob.gradient_batch(X)
+        for b in range(5):
+            x = X[b].reshape(d, 1)
+            # vector variances: the reference takes sigma = sqrt(float32 var) in float32
+            tol = 1e-10 if np.ndim(var) == 0 else 1e-6
+            assert _rel(chi[b], orc32.misfit(x)) < tol
+            assert _rel(g[b], orc32.gradient(x)[:, 0]) < 1e-6
+            assert abs(chi[b] - orc64.misfit(x)) > 1e-9 * abs(chi[b])
+        prob.close()
+
+
+def _full_cov_case(seed, n=120, d=40, sparse=False):
+    rng = np.random.default_rng(seed)
+    G = rng.normal(size=(n, d)) / np.sqrt(d)
+    if sparse:
+        G = sp.csr_matrix(np.where(np.abs(G) > 0.12, G, 0.0))
+    dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+    A = rng.normal(size=(n, n)) / np.sqrt(n)
+    C = 0.01 * (A @ A.T + 0.5 * np.eye(n))
+    return rng, G, dd, C
+
+
+@pytest.mark.parametrize("kind", ["dense", "sparse"])
+def test_full_data_covariance_matches_oracle(kind):
+    """A full data covariance, pre-whitened on the host (G' = L^-1 G,
+    d' = L^-1 d), against the oracle's dense-covariance backend
+    (LinearMatrix.py:226-305) or sparse-covariance backend (:444-481) in
+    FP64: misfits and gradients 1e-10, a teacher-forced 10-step proposal 1e-10."""
+    rng, G, dd, C = _full_cov_case(43, sparse=(kind == "sparse"))
+    d = G.shape[1]
+    Cp = sp.csr_matrix(C) if kind == "sparse" else C
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, Cp, dtype=np.float64)])
+    lik_o = (O.LinearMatrixSparseCovariance(G, dd, Cp, dtype=np.float64) if kind == "sparse"
+             else O.LinearMatrixDenseCovariance(G, dd, C, dtype=np.float64, premultiplication=False))
+    orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0), lik_o])
+    prob = build_problem(post, precision="f64")
+    X = 0.5 * rng.normal(size=(4, d))
+    chi = prob.misfit_batch(X)
+    g = prob.gradient_batch(X)
+    for b in range(4):
+        x = X[b].reshape(d, 1)
+        assert _rel(chi[b], orc.misfit(x)) < 1e-10
+        assert _rel(g[b], orc.gradient(x)[:, 0]) < 1e-10
+    ch = Chains(prob, 4)
+    ch.set_state(X)
+    P0 = rng.normal(size=(4, d))
+    e = 0.002 * rng.uniform(0.5, 1.5, size=4)
+    u = rng.uniform(size=4)
+    res = ch.propose(P0, e, u, 10, "lf", trajectory=True)
+    unit = O.Unit(d, np.random.default_rng(0))
+    for c in range(4):
+        ref = O.propose(orc, unit, X[c].reshape(d, 1), None, P0[c].reshape(d, 1), e[c], 10, u[c])
+        assert _rel(res["X1"][c], ref["x1"][:, 0]) < 1e-10, c
+        assert _rel(res["H1"][c], ref["h1"]) < 1e-10, c
+        assert bool(res["accepted"][c]) == ref["accepted"], c
+
+
+@pytest.mark.parametrize("with_likelihood", [True, False])
+def test_full_covariance_normal_prior_matches_oracle(with_likelihood):
+    """A full-covariance Normal prior (base.py:531,553-574), normalised, alone or
+    with a dense likelihood, against the oracle: misfits and gradients 1e-10,
+    a teacher-forced 10-step proposal 1e-10."""
+    rng = np.random.default_rng(44)
+    n, d = 70, 30
+    A = rng.normal(size=(d, d)) / np.sqrt(d)
+    Cm = A @ A.T + 0.3 * np.eye(d)
+    mu = rng.normal(size=(d, 1)) * 0.2
+    prior = D.Normal(mu.copy(), Cm.copy())
+    prior.normalize()
+    prior_o = O.Normal(mu, Cm)
+    prior_o.normalize()
+    comps, comps_o = [prior], [prior_o]
+    if with_likelihood:
+        G = rng.normal(size=(n, d)) / np.sqrt(d)
+        dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+        comps.append(D.LinearMatrix(G, dd, 0.01, dtype=np.float64))
+        comps_o.append(O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False))
+    post, orc = D.BayesRule(comps), O.BayesRule(comps_o)
+    prob = build_problem(post, precision="f64")
+    X = mu[:, 0][None, :] + 0.3 * rng.normal(size=(4, d))
+    chi = prob.misfit_batch(X)
+    g = prob.gradient_batch(X)
+    for b in range(4):
+        x = X[b].reshape(d, 1)
+        assert _rel(chi[b], orc.misfit(x)) < 1e-10
+        assert _rel(g[b], orc.gradient(x)[:, 0]) < 1e-10
+    ch = Chains(prob, 4)
+    ch.set_state(X)
+    P0 = rng.normal(size=(4, d))
+    e = 0.01 * rng.uniform(0.5, 1.5, size=4)
+    u = rng.uniform(size=4)
+    res = ch.propose(P0, e, u, 10, "lf", trajectory=True)
+    unit = O.Unit(d, np.random.default_rng(0))
+    for c in range(4):
+        ref = O.propose(orc, unit, X[c].reshape(d, 1), None, P0[c].reshape(d, 1), e[c], 10, u[c])
+        assert _rel(res["X1"][c], ref["x1"][:, 0]) < 1e-10, c
+        assert _rel(res["H1"][c], ref["h1"]) < 1e-10, c

@@ -84,6 +84,34 @@ def test_in_memory_results_and_diagnostics(capsys):
     assert d["element-wise / Metropolis"][1] == 0
 
 
+@pytest.mark.parametrize("rng_mode", ["host", "device"])
+def test_keyboard_interrupt_leaves_a_valid_file(tmp_path, rng_mode):
+    """A SIGINT during sampling (tests/test_break.py:11-61 of the reference:
+    _thread.interrupt_main from a timer thread) ends sample() normally, drops
+    the interrupted proposal (Samplers.py:688-690) and leaves a readable file
+    whose columns are the completed samples, the last one not all zeros."""
+    import _thread
+    import threading
+    import time
+
+    fn = str(tmp_path / f"brk_{rng_mode}.npy")
+    post = _post(d=8, bounded=True)
+    timer = threading.Thread(target=lambda: (time.sleep(0.5), _thread.interrupt_main()))
+    timer.start()
+    s = hmc.HMC(seed=1).sample(fn, post, proposals=2_000_000, amount_of_steps=2, stepsize=0.03,
+                               chains=1 if rng_mode == "host" else 2, rng=rng_mode, chunk=64,
+                               disable_progressbar=True)
+    timer.join()
+    assert 0 < s.current_proposal < 2_000_000 - 1
+    name = fn if rng_mode == "host" else str(tmp_path / f"brk_{rng_mode}_0.npy")
+    with Samples(name) as smp:
+        arr = np.array(smp.numpy)
+        assert arr.shape[0] == 9 and arr.shape[1] == smp.read_attribute("write_index")
+        assert arr.shape[1] >= s.current_proposal
+        assert not np.all(arr[:, -1] == 0.0)
+        assert np.all(np.isfinite(arr))
+
+
 def test_gpu_misfit_gradient_methods():
     post = _post(d=4)
     x = np.ones((4, 1)) * 0.3

@@ -141,7 +141,7 @@ def test_surrogate_trajectory_matches_oracle_with_exact_metropolis():
     G = rng.normal(size=(n, d)) / np.sqrt(d)
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
     mu, b = rng.normal(size=(d, 1)) * 0.1, rng.uniform(0.3, 1.0, size=(d, 1))
-    post = D.BayesRule([D.Laplace(mu.copy(), b.copy()), D.LinearMatrix(G, dd, 0.01)])
+    post = D.BayesRule([D.Laplace(mu.copy(), b.copy()), D.LinearMatrix(G, dd, 0.01, dtype=np.float64)])
     prior_o = O.Laplace(mu, b)
     exact_o = O.BayesRule([prior_o, O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
     prob = build_problem(post, precision="f64", engine="gemm")
@@ -210,7 +210,7 @@ def test_trained_surrogate_gains_over_plain_hmc():
     d, n, H, B, L, sigma = 32, 2000, 64, 256, 20, 0.05
     G = rng.normal(size=(n, d)) / np.sqrt(d)
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, sigma, size=(n, 1))
-    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, sigma ** 2)])
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, sigma ** 2, dtype=np.float64)])
     Hs = G.T @ G / sigma ** 2 + np.eye(d)
     Sigma = np.linalg.inv(Hs)
     mu = Sigma @ (G.T @ dd[:, 0] / sigma ** 2)

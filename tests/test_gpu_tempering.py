@@ -25,7 +25,7 @@ def _case(seed, n=120, d=40):
     G = rng.normal(size=(n, d)) / np.sqrt(d)
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
     mu, b = rng.normal(size=(d, 1)) * 0.1, rng.uniform(0.3, 1.0, size=(d, 1))
-    post = D.BayesRule([D.Laplace(mu.copy(), b.copy()), D.LinearMatrix(G, dd, 0.01)])
+    post = D.BayesRule([D.Laplace(mu.copy(), b.copy()), D.LinearMatrix(G, dd, 0.01, dtype=np.float64)])
     orc = O.BayesRule([O.Laplace(mu, b), O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
     return post, orc, rng
 
@@ -111,7 +111,7 @@ def test_device_parallel_tempering_targets_tempered_posteriors():
     n, d = 60, 10
     G = rng.normal(size=(n, d))
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.5, size=(n, 1))
-    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.25)])
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.25, dtype=np.float64)])
     mu, Sigma = O.analytic_linear_gaussian(G, dd, 0.25, np.zeros(d), 1.0)
     levels = np.array([1.0, 2.0, 4.0, 8.0])
     per = 64

@@ -86,6 +86,35 @@ def test_samples_file_roundtrip(tmp_path):
         SamplesWriter(str(tmp_path / "x.h5"), 3)
 
 
+def test_samples_file_header_is_npy_2_0_with_spare_room(tmp_path):
+    """The samples file is NPY 2.0 and its header is as long as a fresh 2.0
+    header of the current shape plus 64 spare bytes, the two conditions under
+    which the reference's appender reopens a file (AppendNPY.py:65-85); when
+    the shape outgrows the spare bytes the header keeps its length."""
+    from hmclab_amd.Samples import _AppendableNPY, npy_header
+
+    fn = str(tmp_path / "v2.npy")
+    w = SamplesWriter(fn, 4)
+    cols = np.arange(15.0).reshape(5, 3)
+    w.append(cols)
+    w.append(cols + 100)
+    w.close()
+    with open(fn, "rb") as f:
+        assert np.lib.format.read_magic(f) == (2, 0)
+        shape, fortran, dtype = np.lib.format.read_array_header_2_0(f)
+        assert (shape, fortran, dtype) == ((6, 5), False, np.dtype("<f8"))
+        assert f.tell() == len(npy_header(6, 5))
+        assert f.tell() % 64 == 0
+    arr = np.load(fn)
+    assert np.array_equal(arr.T, np.hstack([cols, cols + 100]))
+    # growth: find a row count whose shape no longer fits the spare bytes
+    a = _AppendableNPY(str(tmp_path / "g.npy"), 5)
+    n0 = len(a.header)
+    grew = next(r for r in (10 ** k for k in range(1, 200)) if len(npy_header(r, 5)) != n0)
+    assert len(a._header_for(grew)) == n0
+    a.close()
+
+
 def test_distribution_api_validation():
     n = D.Normal(np.zeros((3, 1)), 1.0)
     with pytest.raises(ValueError, match="Bounds vectors are incompatible."):

@@ -142,6 +142,75 @@ def test_linear_identity_closed_form_and_float32_storage():
     assert np.isclose(l32.misfit(x), 0.5 * np.sum((G32 @ x - d32) ** 2) / 0.5, rtol=1e-12)
 
 
+def test_float32_storage_gradient_keeps_the_callers_dense_g():
+    """Dense residual form: the misfit uses the float32 G, the gradient the
+    caller's G (LinearMatrix.py:148,206-208); sparse: both use the cast G
+    (:323,359)."""
+    rng = np.random.default_rng(16)
+    G = rng.normal(size=(30, 12))
+    dd = rng.normal(size=(30, 1))
+    x = rng.normal(size=(12, 1))
+    l32 = O.LinearMatrixSimple(G, dd, 0.5, dtype=np.float32, premultiplication=False)
+    G32 = G.astype(np.float32).astype(np.float64)
+    r = G32 @ x - dd.astype(np.float32).astype(np.float64)
+    assert np.allclose(l32.gradient(x), G.T @ r / 0.5, rtol=1e-12)
+    s32 = O.LinearMatrixSimple(sp.csr_matrix(G), dd, 0.5, dtype=np.float32, premultiplication=False)
+    assert np.allclose(s32.gradient(x), G32.T @ r / 0.5, rtol=1e-6)
+    assert not np.allclose(s32.gradient(x), G.T @ r / 0.5, rtol=1e-12, atol=0)
+
+
+def _full_cov_case(seed=17, n=25, d=8):
+    rng = np.random.default_rng(seed)
+    G = rng.normal(size=(n, d))
+    dd = rng.normal(size=(n, 1))
+    A = rng.normal(size=(n, n)) / np.sqrt(n)
+    C = A @ A.T + 0.5 * np.eye(n)
+    return rng, G, dd, C
+
+
+@pytest.mark.parametrize("premult", [False, True])
+def test_full_covariance_misfit_is_the_gaussian_log_density(premult):
+    """chi = -log N(d; G m, C) - 0.5 log|2 pi C| (scipy.stats), the residual and
+    premultiplied forms agree, and the gradient matches central differences."""
+    from scipy.stats import multivariate_normal
+
+    rng, G, dd, C = _full_cov_case()
+    lm = O.LinearMatrixDenseCovariance(G, dd, C, dtype=np.float64, premultiplication=premult)
+    const = 0.5 * np.linalg.slogdet(2 * np.pi * C)[1]
+    for _ in range(3):
+        x = rng.normal(size=(G.shape[1], 1))
+        want = -multivariate_normal(mean=(G @ x)[:, 0], cov=C).logpdf(dd[:, 0]) - const
+        assert np.isclose(lm.misfit(x), want, rtol=1e-10)
+        h = 1e-6
+        fd = np.array([(lm.misfit(x + h * e[:, None]) - lm.misfit(x - h * e[:, None])) / (2 * h)
+                       for e in np.eye(G.shape[1])])
+        assert np.allclose(lm.gradient(x)[:, 0], fd, rtol=1e-6, atol=1e-6)
+
+
+def test_full_covariance_forms_sparse_whitened_and_diagonal_agree():
+    """The sparse-covariance backend, the pre-whitened simple form
+    (L^-1 G, L^-1 d, unit variance with C = L L^T) and, for a diagonal C, the
+    vector-variance backend all give the dense-covariance misfit and gradient."""
+    import scipy.linalg as sla
+
+    rng, G, dd, C = _full_cov_case(18)
+    dense = O.LinearMatrixDenseCovariance(G, dd, C, dtype=np.float64, premultiplication=False)
+    sparse = O.LinearMatrixSparseCovariance(sp.csr_matrix(G), dd, sp.csr_matrix(C), dtype=np.float64)
+    L = np.linalg.cholesky(C)
+    white = O.LinearMatrixSimple(sla.solve_triangular(L, G, lower=True), sla.solve_triangular(L, dd, lower=True),
+                                 1.0, dtype=np.float64, premultiplication=False)
+    v = rng.uniform(0.5, 2.0, size=(G.shape[0], 1))
+    diag = O.LinearMatrixDenseCovariance(G, dd, np.diag(v[:, 0]), dtype=np.float64, premultiplication=False)
+    simple = O.LinearMatrixSimple(G, dd, v, dtype=np.float64, premultiplication=False)
+    for _ in range(3):
+        x = rng.normal(size=(G.shape[1], 1))
+        for other in (sparse, white):
+            assert np.isclose(other.misfit(x), dense.misfit(x), rtol=1e-10)
+            assert np.allclose(other.gradient(x), dense.gradient(x), rtol=1e-9, atol=1e-12)
+        assert np.isclose(diag.misfit(x), simple.misfit(x), rtol=1e-12)
+        assert np.allclose(diag.gradient(x), simple.gradient(x), rtol=1e-12)
+
+
 def test_mass_matrices_closed_form_and_shape_errors():
     rng = np.random.default_rng(7)
     d = 6
