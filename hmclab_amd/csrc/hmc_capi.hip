@@ -218,7 +218,7 @@ static void gemm_proposal(Chains& c, bool host_p, int L, int integ, uint64_t see
   elem_refresh(c, host_p, first, seed, prop, !host_p, randomize);
   for (size_t k = 0; k < st.size(); ++k) {
     const Stage& s = st[k];
-    if (c.surrogate) surrogate_kick_drift(c, s.kick, s.drift, s.drift2);
+    if (c.surrogate) surrogate_kick_drift(c, s.kick, s.drift, s.drift2, k == 0);
     else gemm_kick_drift(c, s.kick, s.drift, s.drift2, k == 0);
   }
   gemm_endpoint(c, c.xt, c.p->state_dtype);

@@ -84,10 +84,12 @@ struct SplitPlan {
   bool split() const { return SPL > 1; }
 };
 
-static SplitPlan plan_split(int BM, int BN, int M, int N, int K, int n_cu, bool allow, int BK = 64) {
+// min_piece: fewest 64-wide k-tiles per piece; max_spl: most pieces per tile
+static SplitPlan plan_split(int BM, int BN, int M, int N, int K, int n_cu, bool allow, int BK = 64,
+                            int min_piece = 8, int max_spl = 8) {
   SplitPlan s;
   const int gm = (M + BM - 1) / BM, gn = (N + BN - 1) / BN, T = gm * gn;
-  const int nk = (K + BK - 1) / BK * BK / 64;   // pieces of at least 8 x 64 k
+  const int nk = (K + BK - 1) / BK * BK / 64;
   s.F = T;
   s.grid = grid_for(gm, gn);
   if (!allow || n_cu <= 0) return s;
@@ -97,8 +99,8 @@ static SplitPlan plan_split(int BM, int BN, int M, int N, int K, int n_cu, bool 
   const int tail = T - F;
   if (tail <= 0) return s;
   int spl = n_cu / tail;
-  spl = spl > 8 ? 8 : spl;
-  spl = spl > nk / 8 ? nk / 8 : spl;           // at least 8 k-tiles per piece
+  spl = spl > max_spl ? max_spl : spl;
+  spl = spl > nk / min_piece ? nk / min_piece : spl;
   if (spl < 2) return s;
   s.F = F;
   s.SPL = spl;
@@ -637,6 +639,8 @@ struct EpiLeap {
   const float* xa;      // anchor: planes hold x - xa (or null)
   const float* yref;    // premultiplied bf16: g = yref + acc (or null)
   const double* bias;   // premultiplied FP64 trajectories: g = acc - bias[j] (or null)
+  const float* gscale;  // surrogate: g = gscale[j] * acc (or null)
+  const float* xsinv;   // planes hold (x - xa) * xsinv (surrogate's normalised input) (or null)
   __device__ __forceinline__ void step(int64_t j, T& xv, T& pv, T g, T kk, T dd, T dd2) const {
     g += prior_grad<T>(md, j, xv);
     if (out_of_bounds(md, j, (double)xv)) g = (T)INFINITY;
@@ -661,9 +665,12 @@ struct EpiLeap {
         floatx4 x4 = *reinterpret_cast<const floatx4*>(&x[o]);
         floatx4 p4 = *reinterpret_cast<const floatx4*>(&p[o]);
         floatx4 g4 = {(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+        if (gscale) g4 *= *reinterpret_cast<const floatx4*>(&gscale[m0]);
         if (yref) g4 += *reinterpret_cast<const floatx4*>(&yref[m0]);
         floatx4 a4 = {0.f, 0.f, 0.f, 0.f};
+        floatx4 s4 = {1.f, 1.f, 1.f, 1.f};
         if (xa) a4 = *reinterpret_cast<const floatx4*>(&xa[m0]);
+        if (xsinv) s4 = *reinterpret_cast<const floatx4*>(&xsinv[m0]);
         uint32_t h[4], l[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -673,7 +680,7 @@ struct EpiLeap {
           step(m0 + k, xv, pv, g, kk, dd, dd2);
           x4[k] = xv;
           p4[k] = pv;
-          const float f = (float)xv - a4[k];
+          const float f = ((float)xv - a4[k]) * s4[k];
           h[k] = f32_to_bf16_bits(f);
           l[k] = f32_to_bf16_bits(f - bf16_bits_to_f32((uint16_t)h[k]));
         }
@@ -693,17 +700,72 @@ struct EpiLeap {
       const int64_t o = n * ld + j;
       T xv = x[o], pv = p[o];
       T g = (T)v[k];
+      if (gscale) g = (T)((float)v[k] * gscale[j]);
       if (yref) g = (T)((float)v[k] + yref[j]);
       if (bias) g -= (T)bias[j];
       step(j, xv, pv, g, kk, dd, dd2);
       x[o] = xv;
       p[o] = pv;
       if (xh) {
-        const float f = xa ? (float)xv - xa[j] : (float)xv;
+        float f = xa ? (float)xv - xa[j] : (float)xv;
+        if (xsinv) f *= xsinv[j];
         const uint16_t h = f32_to_bf16_bits(f);
         xh[o] = h;
         if (xl) xl[o] = f32_to_bf16_bits(f - bf16_bits_to_f32(h));
       }
+    }
+  }
+};
+
+// surrogate hidden layer of chain n, unit m: z = acc + bias_m, s = sigmoid(z);
+// mode 0: bf16 plane of silu(z) = z s and silu'(z) = s (1 + z (1 - s)) in
+// float; mode 1: bf16 plane of silu'(z) w3_m
+struct EpiSurAct {
+  static constexpr bool kReduce = false;
+  const float* bias;
+  const float* w3;
+  uint16_t* plane;
+  float* sd;
+  int64_t ld;
+  int mode;
+  int M, N;
+  template <typename V>
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t m = m0 + (int64_t)e * ms;
+      if (m >= M) return;
+      const int64_t o = n * ld + m;
+      const float z = (float)v[e] + bias[m];
+      const float sg = 1.f / (1.f + __expf(-z));
+      const float d1 = sg * (1.f + z * (1.f - sg));
+      if (mode == 0) {
+        plane[o] = f32_to_bf16_bits(z * sg);
+        sd[o] = d1;
+      } else {
+        plane[o] = f32_to_bf16_bits(d1 * w3[m]);
+      }
+    }
+  }
+};
+
+// surrogate backward product: bf16 plane of sd[n][m] * acc
+struct EpiSurMul {
+  static constexpr bool kReduce = false;
+  const float* sd;
+  uint16_t* plane;
+  int64_t ld;
+  int M, N;
+  template <typename V>
+  __device__ __forceinline__ void apply(int64_t m0, int ms, int64_t n, const V v[4]) const {
+    if (n >= N) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t m = m0 + (int64_t)e * ms;
+      if (m >= M) return;
+      const int64_t o = n * ld + m;
+      plane[o] = f32_to_bf16_bits(sd[o] * (float)v[e]);
     }
   }
 };
@@ -985,11 +1047,13 @@ static size_t split_elems_f64(int M, int N, int K, int n_cu) {
   return pl.split() ? (size_t)pl.SPL * N * pl.TR : 0;
 }
 
-// Workspace for the split tail: float32 partial planes, or null (no split).
+// Workspace for the split tail: float32 partial planes, or null (no split);
+// min_piece / max_spl as in plan_split.
 struct SplitWs {
   float* ptr = nullptr;
   size_t elems = 0;
   int n_cu = 0;
+  int min_piece = 8, max_spl = 8;
 };
 
 template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
@@ -997,7 +1061,7 @@ static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const
                             const uint16_t* B1, int64_t ldb, int M, int N, int K, int order, const SplitWs& ws,
                             const Epi& epi) {
   constexpr int NT = (BM / WM) * (BN / WN) * 64;
-  SplitPlan pl = plan_split(BM, BN, M, N, K, ws.n_cu, ws.ptr != nullptr);
+  SplitPlan pl = plan_split(BM, BN, M, N, K, ws.n_cu, ws.ptr != nullptr, 64, ws.min_piece, ws.max_spl);
   if (pl.split() && (size_t)pl.SPL * N * pl.TR > ws.elems) pl = plan_split(BM, BN, M, N, K, 0, false);
   hipLaunchKernelGGL((k_gemm_bf16<BM, BN, WM, WN, NSPLIT, S, Epi>), dim3(pl.grid), dim3(NT), 0, s, A, lda, B0, B1,
                      ldb, M, N, K, order, pl.F, pl.SPL, ws.ptr, pl.TR, epi);
@@ -1012,8 +1076,8 @@ static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const
 }
 
 template <int BM, int BN>
-static size_t split_elems(int M, int N, int K, int n_cu) {
-  const SplitPlan pl = plan_split(BM, BN, M, N, K, n_cu, true);
+static size_t split_elems(int M, int N, int K, int n_cu, int min_piece = 8, int max_spl = 8) {
+  const SplitPlan pl = plan_split(BM, BN, M, N, K, n_cu, true, 64, min_piece, max_spl);
   return pl.split() ? (size_t)pl.SPL * N * pl.TR : 0;
 }
 
@@ -1402,6 +1466,51 @@ void gemm_gradient(Chains& c, const void* X, double* out) {
     launch_f64(c, (const double*)p.Gt, p.ldgt, (const double*)c.r, c.ldr, d, B, (int)p.ldgt, e2);
   } else {
     throw Error{HMC_ERR_UNSUPPORTED, "GEMM engine supports float64 and bfloat16 trajectories"};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Learned surrogate stage (bf16 MFMA, direct-to-LDS pipeline): 128 x 128 tiles
+// (config 1); the two K = Dp / K = H products with few output tiles split K
+// into float32 partials when that fills more compute units.
+// ---------------------------------------------------------------------------
+// the surrogate's GEMMs have few output tiles (H x B): their K splits into up
+// to 16 pieces of at least one k-tile
+constexpr int SUR_MIN_PIECE = 1, SUR_MAX_SPL = 16;
+
+size_t surrogate_split_workspace(int64_t D, int64_t H, int64_t B, int n_cu) {
+  auto el = [&](int64_t M, int64_t K) {
+    return split_elems<BF_BM, BF_BN>((int)M, (int)B, (int)K, n_cu, SUR_MIN_PIECE, SUR_MAX_SPL);
+  };
+  return std::max({el(H, round_up(D, 64)), el(H, H), el(D, H)});
+}
+
+void surrogate_stage_bf16(Chains& c, const SurGemm& g, double kick, double drift, double drift2) {
+  Problem& p = *c.p;
+  HMC_REQUIRE(c.ldx == g.Dp, HMC_ERR_STATE, "surrogate plane stride differs from the chains' rows");
+  const int B = (int)c.B, H = (int)g.H, D = (int)g.D, Dp = (int)g.Dp;
+  const SplitWs ws{g.ws, g.ws_elems, p.n_cu, SUR_MIN_PIECE, SUR_MAX_SPL};
+  const hipStream_t st = p.stream;
+  // Z1 = W1 xn + b1 -> A1 = silu(Z1) (bf16), S1 = silu'(Z1)
+  launch_bf16_split<1>(1, st, g.w1h, Dp, g.xh, nullptr, Dp, H, B, Dp, 0, ws,
+                       EpiSurAct{g.b1, nullptr, g.a1h, g.s1, H, 0, H, B});
+  // U2 = silu'(W2 A1 + b2) w3 (bf16)
+  launch_bf16_split<1>(1, st, g.w2h, H, g.a1h, nullptr, H, H, B, H, 0, ws,
+                       EpiSurAct{g.b2, g.w3, g.u2h, nullptr, H, 1, H, B});
+  // U1 = S1 (W2^T U2) (bf16)
+  launch_bf16_split<1>(1, st, g.w2t, H, g.u2h, nullptr, H, H, B, H, 0, ws, EpiSurMul{g.s1, g.u1h, H, H, B});
+  // g = gscale (W1^T U1) + prior'(x) -> kick / drift; the epilogue also writes
+  // the next stage's normalised plane (x - x_mean) / x_scale (row stride ldx = Dp)
+  const DevModel md = p.model();
+  uint16_t* xo = const_cast<uint16_t*>(g.xh);
+  if (p.state_dtype == HMC_F64) {
+    EpiLeap<double> e{(double*)c.xt, (double*)c.pt, xo, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
+                      md, D, B, g.xm, nullptr, nullptr, g.gscale, g.xsinv};
+    launch_bf16_split<1>(1, st, g.w1t, H, g.u1h, nullptr, H, D, B, H, 0, ws, e);
+  } else {
+    EpiLeap<float> e{(float*)c.xt, (float*)c.pt, xo, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
+                     md, D, B, g.xm, nullptr, nullptr, g.gscale, g.xsinv};
+    launch_bf16_split<1>(1, st, g.w1t, H, g.u1h, nullptr, H, D, B, H, 0, ws, e);
   }
 }
 

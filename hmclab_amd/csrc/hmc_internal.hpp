@@ -291,10 +291,31 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
 void fused_misfit(Chains& c, const void* X, double* out);
 void fused_gradient(Chains& c, const void* X, double* out);
 
+// ---- learned surrogate stage on the bf16 GEMM pipeline (hmc_gemm.hip) -------
+// Operands of one surrogate-driven trajectory stage for the chains' rows:
+// xh = bf16 normalised positions [rows][Dp]; weight planes w1h [H][Dp],
+// w1t [Dp][H], w2h [H][H], w2t [H][H] (row padding to 256 zero filled);
+// per-stage buffers a1h, u2h, u1h (bf16 [rows][H]) and s1 (float [rows][H]).
+struct SurGemm {
+  int64_t D = 0, Dp = 0, H = 0;
+  const uint16_t *xh = nullptr, *w1h = nullptr, *w1t = nullptr, *w2h = nullptr, *w2t = nullptr;
+  const float *b1 = nullptr, *b2 = nullptr, *w3 = nullptr, *gscale = nullptr;
+  const float *xm = nullptr, *xsinv = nullptr;   // input normalisation (x - xm) * xsinv
+  uint16_t *a1h = nullptr, *u2h = nullptr, *u1h = nullptr;
+  float* s1 = nullptr;
+  float* ws = nullptr;        // split-tail partials
+  size_t ws_elems = 0;
+};
+size_t surrogate_split_workspace(int64_t D, int64_t H, int64_t B, int n_cu);
+// Z1 = W1 xn + b1, A1 = silu(Z1); U2 = silu'(W2 A1 + b2) w3; V1 = W2^T U2;
+// U1 = silu'(Z1) V1; g = gscale (W1^T U1) + prior'(x) -> kick/drift of the
+// chains (four GEMMs; the last one's epilogue is the leapfrog stage)
+void surrogate_stage_bf16(Chains& c, const SurGemm& g, double kick, double drift, double drift2);
+
 // ---- learned surrogate (hmc_surrogate.hip) ---------------------------------
 // Same contract as gemm_kick_drift with the likelihood gradient taken from
 // the surrogate network attached to the chains.
-void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2);
+void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2, bool first);
 
 // integrator schedules: coefficients of the first drift and of each
 // (kick, drift, drift2) stage; returned in units of the local step.

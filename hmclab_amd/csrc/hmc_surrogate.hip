@@ -308,6 +308,7 @@ struct Surrogate {
   float* xm = nullptr;
   float* xs = nullptr;
   float* gscale = nullptr;      // f_scale / x_scale
+  float* xsinv = nullptr;       // 1 / x_scale
   double f_mean = 0.0, f_scale = 1.0;
   // work buffers for up to Bmax rows
   int64_t Bmax = 0;
@@ -317,6 +318,16 @@ struct Surrogate {
         *dV1 = nullptr, *dU2 = nullptr, *dZ2 = nullptr, *dA1 = nullptr, *dZ1 = nullptr, *lrow = nullptr,
         *scal = nullptr;
   int32_t* idx = nullptr;
+  // bf16 MFMA trajectory stage: weight planes (rebuilt when the parameters
+  // change) and per-stage operands for up to Bh rows
+  uint16_t *w1h = nullptr, *w1t = nullptr, *w2h = nullptr, *w2t = nullptr;
+  bool planes_ok = false;
+  int64_t Bh = 0;
+  uint16_t *xh = nullptr, *a1h = nullptr, *u2h = nullptr, *u1h = nullptr;
+  float* s1h = nullptr;
+  const void* planes_for = nullptr;   // chain set whose positions xh holds
+  float* ws = nullptr;
+  size_t ws_elems = 0;
 
   float* W1() { return P + oW1; }
   float* b1() { return P + ob1; }
@@ -355,6 +366,74 @@ static void gemm(Surrogate& s, const float* A, int64_t lda, const float* B, int6
 }
 
 static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// dst[r][c] = bf16(src[r][c]) (trans 0) or dst[c][r] = bf16(src[r][c]) (trans 1)
+__global__ void k_weight_plane(const float* __restrict__ src, int64_t lds, int64_t rows, int64_t cols,
+                               uint16_t* __restrict__ dst, int64_t ldd, int trans) {
+  const int64_t n = rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols, c = i % cols;
+    const uint16_t v = f32_to_bf16_bits(src[r * lds + c]);
+    if (trans) dst[c * ldd + r] = v;
+    else dst[r * ldd + c] = v;
+  }
+}
+
+// bf16 normalised rows Xh[b][j] = bf16((x - xm_j) / xs_j) for b < B, zero in the
+// padding (rows up to `rows`, columns up to Dp)
+// one workgroup row per chain row (blockIdx.y), 4 columns per thread
+template <typename T>
+__global__ void k_norm_bf16(const T* __restrict__ X, int64_t ldx, uint16_t* __restrict__ Xh, int Dp, int D,
+                            const float* __restrict__ xm, const float* __restrict__ xsinv, int B) {
+  const int b = blockIdx.y;
+  for (int j0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x); j0 < Dp; j0 += 4 * gridDim.x * blockDim.x) {
+    uint32_t h[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int j = j0 + e;
+      h[e] = (b < B && j < D) ? f32_to_bf16_bits(((float)X[(int64_t)b * ldx + j] - xm[j]) * xsinv[j]) : 0u;
+    }
+    *reinterpret_cast<uint2*>(&Xh[(int64_t)b * Dp + j0]) = uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+  }
+}
+
+// (launched on `st`, the stream of the GEMMs that read the planes)
+static void ensure_planes(Surrogate& s, hipStream_t st) {
+  if (s.planes_ok) return;
+  // parameters written on the surrogate's own stream (training) are complete
+  HMC_HIP(hipStreamSynchronize(s.stream));
+  const int64_t H = s.H, Dp = s.Dp, Hp = round_up(H, 256), Dpp = round_up(Dp, 256);
+  if (!s.w1h) {
+    s.w1h = (uint16_t*)dev_alloc(s.owned, (size_t)Hp * Dp * 2);
+    s.w1t = (uint16_t*)dev_alloc(s.owned, (size_t)Dpp * H * 2);
+    s.w2h = (uint16_t*)dev_alloc(s.owned, (size_t)Hp * H * 2);
+    s.w2t = (uint16_t*)dev_alloc(s.owned, (size_t)Hp * H * 2);
+  }
+  const dim3 grid(4096);
+  hipLaunchKernelGGL(k_weight_plane, grid, dim3(256), 0, st, s.W1(), Dp, H, Dp, s.w1h, Dp, 0);
+  hipLaunchKernelGGL(k_weight_plane, grid, dim3(256), 0, st, s.W1(), Dp, H, Dp, s.w1t, H, 1);
+  hipLaunchKernelGGL(k_weight_plane, grid, dim3(256), 0, st, s.W2(), H, H, H, s.w2h, H, 0);
+  hipLaunchKernelGGL(k_weight_plane, grid, dim3(256), 0, st, s.W2(), H, H, H, s.w2t, H, 1);
+  HMC_HIP(hipGetLastError());
+  s.planes_ok = true;
+}
+
+static void ensure_bf16_rows(Surrogate& s, int64_t B, int n_cu) {
+  const int64_t rows = round_up(B, 256);
+  if (rows > s.Bh) {
+    s.xh = (uint16_t*)dev_alloc(s.owned, (size_t)rows * s.Dp * 2);
+    s.a1h = (uint16_t*)dev_alloc(s.owned, (size_t)rows * s.H * 2);
+    s.u2h = (uint16_t*)dev_alloc(s.owned, (size_t)rows * s.H * 2);
+    s.u1h = (uint16_t*)dev_alloc(s.owned, (size_t)rows * s.H * 2);
+    s.s1h = (float*)dev_alloc(s.owned, (size_t)rows * s.H * 4);
+    s.Bh = rows;
+  }
+  const size_t e = surrogate_split_workspace(s.D, s.H, B, n_cu);
+  if (e > s.ws_elems) {
+    s.ws = (float*)dev_alloc(s.owned, e * 4);
+    s.ws_elems = e;
+  }
+}
 
 // forward + input gradient for `rows` normalised rows in s.Xn (rows multiple of 64)
 static void forward_grad(Surrogate& s, int64_t rows) {
@@ -406,6 +485,7 @@ static void backward(Surrogate& s, int64_t rows, float alpha, float beta) {
 
 static void adam(Surrogate& s, float lr) {
   s.adam_t += 1;
+  s.planes_ok = false;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - std::pow(b1, (float)s.adam_t), c2 = 1.f - std::pow(b2, (float)s.adam_t);
   hipLaunchKernelGGL(k_adam, g1(s.nP), dim3(256), 0, s.stream, s.P, s.dP, s.Pm, s.Pv, s.nP, lr, b1, b2, c1, c2,
@@ -414,9 +494,57 @@ static void adam(Surrogate& s, float lr) {
 }
 
 // gradient of the surrogate at the chains' trajectory positions + kick/drift
-void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2) {
+void surrogate_kick_drift(Chains& c, double kick, double drift, double drift2, bool first) {
   Surrogate& s = *reinterpret_cast<Surrogate*>(c.surrogate);
   Problem& p = *c.p;
+  if (s.mode == 0) {
+    // bf16 MFMA: normalised rows, then four pipelined GEMMs whose last
+    // epilogue applies the kick and drift
+    ensure_planes(s, p.stream);
+    ensure_bf16_rows(s, c.B, p.n_cu);
+    hipEvent_t ev;
+    time_begin(c, 5, ev);
+    // the first stage of a trajectory normalises the rows (up to the GEMM's
+    // 128-row chain tile: padding rows become zeros); later stages read the
+    // plane the previous stage's kick/drift epilogue wrote
+    if (first || s.planes_for != &c) {
+      HMC_REQUIRE(s.Dp % 4 == 0 && s.Bh < 65536, HMC_ERR_STATE, "surrogate row layout");
+      const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((s.Dp / 4 + 255) / 256, 8)),
+                      (unsigned)round_up(c.B, 128));
+      if (p.state_dtype == HMC_F64)
+        hipLaunchKernelGGL(k_norm_bf16<double>, grid, dim3(256), 0, p.stream, (const double*)c.xt, c.ldx, s.xh,
+                           (int)s.Dp, (int)s.D, s.xm, s.xsinv, (int)c.B);
+      else
+        hipLaunchKernelGGL(k_norm_bf16<float>, grid, dim3(256), 0, p.stream, (const float*)c.xt, c.ldx, s.xh,
+                           (int)s.Dp, (int)s.D, s.xm, s.xsinv, (int)c.B);
+      HMC_HIP(hipGetLastError());
+      s.planes_for = &c;
+    }
+    SurGemm g;
+    g.D = s.D;
+    g.Dp = s.Dp;
+    g.H = s.H;
+    g.xh = s.xh;
+    g.w1h = s.w1h;
+    g.w1t = s.w1t;
+    g.w2h = s.w2h;
+    g.w2t = s.w2t;
+    g.b1 = s.b1();
+    g.b2 = s.b2();
+    g.w3 = s.w3();
+    g.gscale = s.gscale;
+    g.xm = s.xm;
+    g.xsinv = s.xsinv;
+    g.a1h = s.a1h;
+    g.u2h = s.u2h;
+    g.u1h = s.u1h;
+    g.s1 = s.s1h;
+    g.ws = s.ws;
+    g.ws_elems = s.ws_elems;
+    surrogate_stage_bf16(c, g, kick, drift, drift2);
+    time_end(c, 5, ev);
+    return;
+  }
   const int64_t rows = round_up(c.B, 64);
   ensure_rows(s, rows);
   // launches go to the problem's stream for the duration of the call
@@ -504,8 +632,10 @@ int hmc_surrogate_create(int64_t dimensions, int64_t hidden, uint64_t seed, hmc_
       s.xm = (float*)dev_alloc(s.owned, s.Dp * 4);
       s.xs = (float*)dev_alloc(s.owned, s.Dp * 4);
       s.gscale = (float*)dev_alloc(s.owned, s.Dp * 4);
+      s.xsinv = (float*)dev_alloc(s.owned, s.Dp * 4);
       h2d(s.xs, ones.data(), s.Dp * 4);
       h2d(s.gscale, ones.data(), s.Dp * 4);
+      h2d(s.xsinv, ones.data(), s.Dp * 4);
     } catch (...) {
       for (void* p : s.owned) (void)hipFree(p);
       if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -538,16 +668,19 @@ int hmc_surrogate_set_normalization(hmc_surrogate* h, const double* x_mean, cons
   return guarded("hmc_surrogate_set_normalization", [&] {
     HMC_REQUIRE(h && x_mean && x_scale && f_scale > 0.0, HMC_ERR_ARG, "invalid normalisation");
     Surrogate& s = h->s;
-    std::vector<float> m(s.Dp, 0.f), sc(s.Dp, 1.f), gs(s.Dp, 0.f);
+    std::vector<float> m(s.Dp, 0.f), sc(s.Dp, 1.f), gs(s.Dp, 0.f), si(s.Dp, 0.f);
     for (int64_t j = 0; j < s.D; ++j) {
       HMC_REQUIRE(x_scale[j] > 0.0, HMC_ERR_ARG, "x_scale must be positive");
       m[j] = (float)x_mean[j];
       sc[j] = (float)x_scale[j];
       gs[j] = (float)(f_scale / x_scale[j]);
+      si[j] = (float)(1.0 / x_scale[j]);
     }
     h2d(s.xm, m.data(), s.Dp * 4);
     h2d(s.xs, sc.data(), s.Dp * 4);
     h2d(s.gscale, gs.data(), s.Dp * 4);
+    h2d(s.xsinv, si.data(), s.Dp * 4);
+    s.planes_for = nullptr;
     s.f_mean = f_mean;
     s.f_scale = f_scale;
   });
@@ -593,6 +726,7 @@ int hmc_surrogate_set_params(hmc_surrogate* h, const float* params) {
     HMC_REQUIRE(h && params, HMC_ERR_ARG, "NULL argument");
     HMC_HIP(hipStreamSynchronize(h->s.stream));
     pack_unpack(h->s, const_cast<float*>(params), true, h->s.P);
+    h->s.planes_ok = false;
     h->s.adam_t = 0;
     dzero(h->s.Pm, h->s.nP * 4);
     dzero(h->s.Pv, h->s.nP * 4);

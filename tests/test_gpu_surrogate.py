@@ -135,7 +135,12 @@ class _SurrogateTarget:
         self.exact.corrector(x, p)
 
 
-def test_surrogate_trajectory_matches_oracle_with_exact_metropolis():
+@pytest.mark.parametrize("mode", ["f32", "bf16"])
+def test_surrogate_trajectory_matches_oracle_with_exact_metropolis(mode):
+    """f32: positions and momenta 1e-4 of the oracle leapfrog driven by the
+    oracle MLP; bf16 (MFMA pipeline with fused epilogues, the last one the
+    kick/drift): the displacement from the start within 3e-2.  Energies are
+    exact FP64 in both."""
     rng = np.random.default_rng(8)
     n, d, B, L, H = 90, 40, 8, 12, 64
     G = rng.normal(size=(n, d)) / np.sqrt(d)
@@ -146,7 +151,7 @@ def test_surrogate_trajectory_matches_oracle_with_exact_metropolis():
     exact_o = O.BayesRule([prior_o, O.LinearMatrixSimple(G, dd, 0.01, dtype=np.float64, premultiplication=False)])
     prob = build_problem(post, precision="f64", engine="gemm")
     ch = Chains(prob, B)
-    s = Surrogate(d, H, seed=5, mode="f32")
+    s = Surrogate(d, H, seed=5, mode=mode)
     P = _params(rng, d, H)
     s.set_params(P)
     xm, xs = rng.normal(size=d) * 0.05, rng.uniform(0.2, 0.5, size=d)
@@ -170,13 +175,19 @@ def test_surrogate_trajectory_matches_oracle_with_exact_metropolis():
         k0 = unit.kinetic_energy(P0[k].reshape(d, 1))
         h0 = exact_o.misfit(x0) + k0
         h1 = exact_o.misfit(x1) + unit.kinetic_energy(p1)
-        assert _rel(res["X1"][k], x1[:, 0]) < 1e-4
-        assert _rel(res["P1"][k], p1[:, 0]) < 1e-4
+        if mode == "f32":
+            assert _rel(res["X1"][k], x1[:, 0]) < 1e-4
+            assert _rel(res["P1"][k], p1[:, 0]) < 1e-4
+        else:
+            assert _rel(res["X1"][k] - X0s[k], x1[:, 0] - X0s[k]) < 3e-2
         assert abs(res["H0"][k] - h0) <= 1e-10 * abs(h0), (k, res["H0"][k], h0, chi0[k], k0, res["H1"][k], h1,
                                                             chi0_after[k])
-        assert abs(res["H1"][k] - h1) <= 1e-6 * abs(h1)          # end: exact misfit at a 1e-4 position
-        if abs(np.log(u[k]) - (h0 - h1)) > 1e-3:
-            assert bool(res["accepted"][k]) == bool(np.exp(h0 - h1) > u[k])
+        # end: the exact FP64 Hamiltonian of the returned end point
+        x1g, p1g = res["X1"][k].reshape(d, 1), res["P1"][k].reshape(d, 1)
+        h1g = exact_o.misfit(x1g) + unit.kinetic_energy(p1g)
+        assert abs(res["H1"][k] - h1g) <= 1e-10 * abs(h1g)
+        if abs(np.log(u[k]) - (res["H0"][k] - res["H1"][k])) > 1e-9:
+            assert bool(res["accepted"][k]) == bool(np.exp(res["H0"][k] - res["H1"][k]) > u[k])
     # detaching restores the exact-gradient trajectory
     ch.set_surrogate(None)
     ch.set_state(X0s)
@@ -199,26 +210,35 @@ def test_surrogate_requires_gemm_engine_and_matching_dimension():
 
 
 def test_trained_surrogate_gains_over_plain_hmc():
-    """What the trained network buys: on a problem where the exact gradient is
-    expensive (n = 2000 data, d = 32) a 64-wide surrogate trained on warm-up
-    pairs keeps most of plain HMC's acceptance at a tenth of the trajectory
-    FLOPs, the untrained network is rejected, and the chain still targets the
-    exact posterior."""
+    """What the trained network buys, in wall-clock time: on a target whose
+    exact trajectory gradient is expensive (residual form, n = 100 000 data,
+    d = 32; each stage reads G twice and writes the n x B residual) a 64-wide
+    surrogate trained on warm-up pairs keeps most of plain HMC's acceptance,
+    accepts more than twice as many proposals per second, the untrained
+    network is rejected, and the chain still targets the exact posterior."""
+    import time
+
     from hmclab_amd.surrogate import training_pairs
 
     rng = np.random.default_rng(21)
-    d, n, H, B, L, sigma = 32, 2000, 64, 256, 20, 0.05
+    d, n, H, B, L, sigma = 32, 100_000, 64, 256, 20, 0.5
     G = rng.normal(size=(n, d)) / np.sqrt(d)
     dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, sigma, size=(n, 1))
-    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, sigma ** 2, dtype=np.float64)])
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0),
+                        D.LinearMatrix(G, dd, sigma ** 2, dtype=np.float64, premultiplication=False)])
     Hs = G.T @ G / sigma ** 2 + np.eye(d)
     Sigma = np.linalg.inv(Hs)
     mu = Sigma @ (G.T @ dd[:, 0] / sigma ** 2)
     eps = 0.5 / np.sqrt(np.linalg.eigvalsh(Hs).max())
     ch = Chains(build_problem(post, precision="f64", engine="gemm"), B)
     ch.set_state(rng.multivariate_normal(mu, Sigma, size=B))
+    ch.run(2, L, np.full(B, eps), seed=9)                 # first-use set-up out of the timing
     ch.set_trace(16)
-    _, acc_plain = ch.run(16, L, np.full(B, eps), seed=1)
+    ch.sync()
+    t0 = time.perf_counter()
+    _, acc_plain = ch.run(16, L, np.full(B, eps), seed=1, first_proposal=2)
+    ch.sync()
+    t_plain = (time.perf_counter() - t0) / 16
     X, _ = ch.read_trace(0, 16)
     X = X.reshape(-1, d)
     ch.set_trace(0)
@@ -226,22 +246,26 @@ def test_trained_surrogate_gains_over_plain_hmc():
     untrained = Surrogate(d, H, seed=0, mode="f32")
     untrained.fit_normalization(X, chi)
     ch.set_surrogate(untrained)
-    _, acc_untrained = ch.run(4, L, np.full(B, eps), seed=2, first_proposal=16)
+    _, acc_untrained = ch.run(4, L, np.full(B, eps), seed=2, first_proposal=18)
     sur = Surrogate(d, H, seed=0, mode="f32")
     hist = sur.train(X, chi, grad, epochs=40, batch=256, learning_rate=3e-3)
     assert hist[-1] < 0.05 * hist[0]
     ch.set_surrogate(sur)
+    ch.run(2, L, np.full(B, eps), seed=8, first_proposal=22)
     ch.set_trace(60)
-    _, acc_sur = ch.run(60, L, np.full(B, eps), seed=3, first_proposal=20)
+    ch.sync()
+    t0 = time.perf_counter()
+    _, acc_sur = ch.run(60, L, np.full(B, eps), seed=3, first_proposal=24)
+    ch.sync()
+    t_sur = (time.perf_counter() - t0) / 60
     a_plain, a_untrained, a_sur = acc_plain.mean() / 16, acc_untrained.mean() / 4, acc_sur.mean() / 60
-    flops_plain = L * 4 * n * d + 2 * n * d          # trajectory gradients + exact endpoint misfit
-    flops_sur = L * 4 * (d * H + H * H) + 2 * n * d
-    gain = (a_sur / flops_sur) / (a_plain / flops_plain)
-    print(f"acceptance plain {a_plain:.3f}, untrained surrogate {a_untrained:.3f}, trained {a_sur:.3f}; "
-          f"accepted proposals per FLOP gain {gain:.2f}")
+    gain = (a_sur / t_sur) / (a_plain / t_plain)
+    print(f"acceptance plain {a_plain:.3f} ({t_plain * 1e3:.2f} ms/proposal), untrained surrogate "
+          f"{a_untrained:.3f}, trained {a_sur:.3f} ({t_sur * 1e3:.2f} ms/proposal); accepted proposals per "
+          f"second gain {gain:.2f}")
     assert a_untrained < 0.2
     assert a_sur > 0.5
-    assert gain > 3.0
+    assert gain > 2.0
     Xs, _ = ch.read_trace(0, 60)
     flat = Xs.reshape(-1, d)
     sd = np.sqrt(np.diag(Sigma))
