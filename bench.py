@@ -29,10 +29,15 @@ trajectories stay reversible).  --premult-trajectory 0 runs both legs in the
 residual form (g = G^T (G x - d) / var).
 
 A "step" is one proposal for all chains.  W warm-up proposals run untimed,
-K proposals are timed; min-ESS is the rank-normalised split bulk ESS
-minimised over all 8192 parameters, pooled over chains.  Posterior mean/std
-errors of both legs are against the analytic FP64 posterior of the
-bf16-rounded G ("matched posterior error").
+K proposals are timed.  value = (min-ESS per proposal) x (timed proposals per
+second): min-ESS is the rank-normalised split bulk ESS minimised over all
+8192 parameters, pooled over chains, estimated from E = max(K, --ess-draws)
+draws per chain (the K timed draws continued untimed by the same chains; the
+split-chain estimator overstates ESS on short chains: 1.32x at 20 draws,
+1.05x at 200, against 2000-draw runs of both legs), and min-ESS of the K
+timed draws alone is reported beside it.  Posterior mean/std errors of both
+legs are against the analytic FP64 posterior of the bf16-rounded G
+("matched posterior error").
 
 The roofline line is measured live with HIP events on the engine's stream
 around every trajectory stage GEMM launch and its split-tail epilogue
@@ -175,14 +180,19 @@ def trace_to_torch(torch, ch, K):
     return out
 
 
-def _posterior_errors(torch, ch, K, mu_np, sd_np, d):
+def _posterior_errors(torch, ch, K, mu_np, sd_np, d, K_timed=None):
     """min-ESS and the RMS errors of the pooled mean (in units of the reference
     sd) and std (relative) against the reference moments (mu_np, sd_np; None:
-    no reference).  Also returns the pooled mean and std."""
+    no reference), from the K recorded draws per chain.  Also returns the
+    pooled mean and std, and the min-ESS of the first K_timed draws alone."""
     from hmclab_amd import diagnostics
 
     draws = trace_to_torch(torch, ch, K)
     ess = diagnostics.ess_bulk(draws) if K >= 4 else np.full(d, np.nan)   # short profiling runs
+    if K_timed is not None and K_timed < K:
+        ess_timed = diagnostics.ess_bulk(draws[:, :K_timed].contiguous()) if K_timed >= 4 else np.full(d, np.nan)
+    else:
+        ess_timed = ess
     pooled = draws.to(torch.float64).reshape(-1, d)
     mean = pooled.mean(dim=0).cpu().numpy()
     std = pooled.std(dim=0).cpu().numpy()
@@ -191,7 +201,7 @@ def _posterior_errors(torch, ch, K, mu_np, sd_np, d):
     if mu_np is not None:
         err_mean = float(np.sqrt(np.mean(((mean - mu_np) / sd_np) ** 2)))
         err_std = float(np.sqrt(np.mean((std / sd_np - 1.0) ** 2)))
-    return float(np.min(ess)), err_mean, err_std, mean, std
+    return float(np.min(ess)), err_mean, err_std, mean, std, float(np.min(ess_timed))
 
 
 def _kernel_times(ch):
@@ -231,16 +241,22 @@ def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, premult):
     sub = argparse.Namespace(**vars(args))
     sub.warmup = min(args.warmup, 5)
     eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), sub, args.seed + 7, "fp64 ")
-    elapsed, acc = _timed(ch, B, L, eps, args.plain_steps, args.seed + 7, done, torch)
-    min_ess, em, es, mean, std = _posterior_errors(torch, ch, args.plain_steps, mu_np, sd_np, d)
+    K = args.plain_steps
+    E = max(K, args.ess_draws)
+    elapsed, acc = _timed(ch, B, L, eps, K, args.seed + 7, done, torch, trace=E)
     kt = _kernel_times(ch)
+    if E > K:   # untimed continuation of the same chains for the ESS estimate
+        ch.run(E - K, L, np.full(B, eps), seed=args.seed + 7, first_proposal=done + K)
+    min_ess, em, es, mean, std, min_ess_timed = _posterior_errors(torch, ch, E, mu_np, sd_np, d, K)
     ch.close()
     prob.close()
-    log(f"plain fp64 ({'premultiplied' if premult else 'residual form'}): {args.plain_steps} proposals in "
-        f"{elapsed:.2f}s, acceptance {acc.mean() / args.plain_steps:.3f}, "
-        f"min ESS {min_ess:.0f} -> {min_ess / elapsed:.1f} ESS/s")
+    value = min_ess / E * K / elapsed
+    log(f"plain fp64 ({'premultiplied' if premult else 'residual form'}): {K} proposals in "
+        f"{elapsed:.2f}s, acceptance {acc.mean() / K:.3f}, min ESS {min_ess:.0f} over {E} draws "
+        f"-> {value:.1f} ESS/s")
     return dict(trajectory_form="premultiplied" if premult else "residual",
-                value=min_ess / elapsed, min_ess=min_ess, seconds=elapsed, proposals=args.plain_steps,
+                value=value, min_ess=min_ess, ess_draws=E, min_ess_timed_draws=min_ess_timed,
+                seconds=elapsed, proposals=args.plain_steps,
                 acceptance_rate=float(acc.mean() / args.plain_steps), stepsize=eps,
                 posterior_rms_mean_err_over_sd=em, posterior_rms_std_rel_err=es,
                 ms_per_proposal=elapsed / args.plain_steps * 1e3, _mean=mean, _std=std,
@@ -259,6 +275,7 @@ def _train_surrogate(args, torch, ch, post, B, L, eps, done):
     ch.set_trace(K)
     ch.run(K, L, np.full(B, eps), seed=args.seed + 3, first_proposal=done)
     X, _ = ch.read_trace(0, K)
+    ch.set_trace(0)            # the probe and the timed phase set their own traces
     X = X.reshape(K * B, d)
     chi, grad = training_pairs(post, X)
     t_pairs = time.perf_counter() - t0
@@ -271,6 +288,9 @@ def _train_surrogate(args, torch, ch, post, B, L, eps, done):
     chi_c, g_c = training_pairs(post, Xc)
     f_s, g_s = sur.evaluate(Xc)
     gc = g_c - g_c.mean(axis=0)
+    log(f"surrogate held-out set: {Xc.shape[0]} chain states, position spread {float(np.std(Xc, axis=0).mean()):.4g} "
+        f"(training {float(np.std(X, axis=0).mean()):.4g}), misfit sd {float(np.std(chi_c)):.4g} "
+        f"(training {float(np.std(chi)):.4g}), gradient spread {float(np.linalg.norm(gc)):.4g}")
     grad_err = float(np.linalg.norm(g_s - g_c) / np.linalg.norm(gc))
     misfit_err = float(np.sqrt(np.mean((f_s - chi_c) ** 2)) / max(np.std(chi_c), 1e-300))
     del X, chi, grad
@@ -368,7 +388,8 @@ def run_dense(args):
 
     # timed sampling phase: chunks of proposals; an attached surrogate is
     # detached when a chunk's acceptance falls below the fallback threshold
-    ch.set_trace(args.steps)
+    E = max(args.steps, args.ess_draws)
+    ch.set_trace(E)
     ch.enable_timing(True)
     ch.sync()
     torch.cuda.synchronize()
@@ -391,18 +412,22 @@ def run_dense(args):
     ch.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    kt = _kernel_times(ch)
+    ch.enable_timing(False)
+    if E > args.steps:   # untimed continuation of the same chains for the ESS estimate
+        ch.run(E - args.steps, L, np.full(B, eps), seed=args.seed, first_proposal=done + args.steps)
     if fallback_at is not None:
         log(f"surrogate fallback after {fallback_at} timed proposals (chunk acceptance {sur_acc[-1]:.3f} < "
             f"{args.sur_fallback})")
-    kt = _kernel_times(ch)
     names = {0: "residual gemm", 1: "endpoint", 2: "elementwise", 3: "gradient gemm", 4: "premultiplied stage gemm",
-             5: "surrogate gemms", 6: "surrogate kick/drift"}
+             5: "surrogate gemms", 6: "surrogate kick/drift", 7: "premultiplied stage split-tail epilogue"}
     log(f"timed {args.steps} proposals in {elapsed:.3f}s; " +
         ", ".join(f"{names[k_]} {kt[k_][0]:.1f} ms/{kt[k_][1]}" for k_ in names if kt[k_][1]))
     if args.premult_trajectory:
-        slot, gemm_key = 4, "premult"
-        gemm_name = ("k_gemm_bf16<256,128,64,64,1,3,EpiLeap> + k_split_epilogue (premultiplied stage: "
-                     "g = y_ref + bf16(H) bf16(x - x_ref), kick/drift; timing slot 4 holds only these launches)")
+        slot, gemm_key = 4, "premult_gemm"
+        gemm_name = ("k_gemm_bf16<256,128,64,64,1,3,EpiLeap<float>> alone (premultiplied stage GEMM: "
+                     "g = y_ref + bf16(H) bf16(x - x_ref) with the kick/drift epilogue; its split-tail epilogue "
+                     "kernel is timed apart, in 'stage')")
     elif kt[3][0] >= kt[0][0]:
         slot, gemm_key = 3, "gradient"
         gemm_name = ("k_gemm_bf16<...,EpiLeap> + k_split_epilogue (gradient GEMM + kick/drift; "
@@ -412,9 +437,9 @@ def run_dense(args):
         gemm_name = ("k_gemm_bf16<...,EpiResidAnchor> + k_split_epilogue (residual GEMM; the timed region "
                      "holds both kernels)")
     gemm_ms, gemm_n = kt[slot]
-    min_ess, err_mean, err_std, _, _ = _posterior_errors(torch, ch, args.steps, mu_np, sd_np, d)
+    min_ess, err_mean, err_std, _, _, min_ess_timed = _posterior_errors(torch, ch, E, mu_np, sd_np, d, args.steps)
     acc_rate = float(acc.mean() / args.steps)
-    value = min_ess / elapsed
+    value = min_ess / E * args.steps / elapsed
     # the gradient field the timed trajectories integrated, at the chains' final positions
     Xf, _ = ch.get_state()
     grad_err, grad_err_rms = (_trajectory_gradient_error(post, ch, Xf, args.precision)
@@ -428,7 +453,7 @@ def run_dense(args):
     # HBM bytes per launch of the dominant kernel from the committed counter
     # profile (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel); only for
     # the configuration and GEMM options it was measured with
-    traffic = None
+    traffic = stage_traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_dense_gemm.json")
     if os.path.exists(tfile) and args.precision == "bf16" and n == 20000 and d == 8192 and B == 512:
         try:
@@ -441,8 +466,18 @@ def run_dense(args):
             for prof in tj.get("profiles", []):
                 if all(prof.get("options", {}).get(k_, object()) == v for k_, v in ran.items()):
                     traffic = prof["bytes_per_launch"].get(gemm_key)
+                    stage_traffic = prof["bytes_per_launch"].get("premult_stage")
         except Exception:
-            traffic = None
+            traffic = stage_traffic = None
+    # the whole premultiplied stage: GEMM kernel + split-tail epilogue kernel
+    stage = None
+    if args.premult_trajectory and args.precision == "bf16" and gemm_n:
+        st_s = (kt[4][0] + kt[7][0]) / 1e3 / gemm_n
+        stage = {"kernels": "k_gemm_bf16<EpiLeap<float>> + k_split_epilogue<EpiLeap<float>>",
+                 "avg_us": st_s * 1e6, "epilogue_launches": kt[7][1],
+                 "achieved": flop_launch / st_s / 1e12, "frac": flop_launch / st_s / peak,
+                 "traffic": stage_traffic,
+                 "algorithmic_bytes": 2.0 * d * d + 2 * 2.0 * d * B + 4 * 4.0 * d * B}
 
     cfg = {
         "prior": "Laplace(0, 0.3); errors are against the plain fp64 leg" if args.prior == "laplace"
@@ -453,7 +488,9 @@ def run_dense(args):
                     + ("premultiplied (H = G^T C^-1 G, the reference's form for n > d)" if args.premult_trajectory
                        else "in the residual form"),
         "chains": B, "n_data": n, "n_params": d, "leapfrog_steps": L, "stepsize": eps,
-        "min_ess": min_ess, "acceptance_rate": acc_rate,
+        "min_ess": min_ess, "ess_draws_per_chain": E, "min_ess_per_proposal": min_ess / E,
+        "min_ess_timed_draws": min_ess_timed,
+        "value_from_timed_draws_only": min_ess_timed / elapsed, "acceptance_rate": acc_rate,
         "posterior_rms_mean_err_over_sd": err_mean, "posterior_rms_std_rel_err": err_std,
         "proposals_per_s": args.steps / elapsed,
         "surrogate_fraction_of_timed_proposals": sur_props / args.steps,
@@ -474,8 +511,8 @@ def run_dense(args):
         sur_info.update(timed_chunk_acceptance=sur_acc, timed_fallback_after_proposals=fallback_at,
                         fallback_threshold=args.sur_fallback, max_heldout_grad_err=args.sur_max_grad_err)
         if sur_props:
-            sur_info["value_including_training_s"] = min_ess / (elapsed + sur_info["train_s"]
-                                                                + sur_info["warmup_and_pairs_s"])
+            sur_info["value_including_training_s"] = min_ess / E * args.steps / (
+                elapsed + sur_info["train_s"] + sur_info["warmup_and_pairs_s"])
         cfg["surrogate"] = sur_info
     if plain is not None:
         plain.pop("_mean", None)
@@ -506,11 +543,11 @@ def run_dense(args):
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel": gemm_name, "avg_launch_us": avg_s * 1e6, "launches": gemm_n,
                      "algorithmic_flop_per_launch": flop_launch,
-                     "endpoint_fp64_avg_us": kt[1][0] / max(kt[1][1], 1) * 1e3},
+                     "endpoint_fp64_avg_us": kt[1][0] / max(kt[1][1], 1) * 1e3, "stage": stage},
     }
 
     if args.cpu_baseline:
-        ess_per_prop_chain = min_ess / (args.steps * B)
+        ess_per_prop_chain = min_ess / (E * B)
         rates, secs = cpu_baseline(args, eps)
         agg = float(np.sum(rates))
         result["cpu_baseline"] = {
@@ -544,8 +581,8 @@ def _tune(ch, B, L, eps, args, seed, log_prefix=""):
     return eps, done + args.warmup
 
 
-def _timed(ch, B, L, eps, steps, seed, first, torch=None):
-    ch.set_trace(steps)
+def _timed(ch, B, L, eps, steps, seed, first, torch=None, trace=None):
+    ch.set_trace(steps if trace is None else trace)
     ch.enable_timing(True)
     ch.sync()
     if torch is not None:
@@ -831,20 +868,22 @@ def main():
     ap.add_argument("--bf16-split-grad", type=int, default=1, choices=[1, 2],
                     help="bf16 planes of the scaled residual in the gradient GEMM (1: one plane, "
                          "measured to leave acceptance unchanged; 2: hi/lo)")
-    ap.add_argument("--gemm-config", type=int, default=3, choices=[0, 1, 2, 3])
+    ap.add_argument("--gemm-config", type=int, default=3, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--fused-tile-chains", type=int, default=0, choices=[0, 1, 2, 4],
                     help="tomography: largest chains per fused-engine workgroup (0: as LDS allows)")
     ap.add_argument("--split-tail", type=int, default=1, choices=[0, 1],
                     help="bf16 GEMMs: split the last partial round of tiles along K")
     ap.add_argument("--tile-order", type=int, default=0, choices=[0, 1])
-    ap.add_argument("--plain-residual", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--plain-residual", type=int, default=0, choices=[0, 1],
                     help="with premultiplied trajectories, also time the residual-form FP64 comparator")
+    ap.add_argument("--ess-draws", type=int, default=200,
+                    help="draws per chain for the ESS estimate: the timed proposals continued untimed")
     ap.add_argument("--premult-trajectory", type=int, default=1, choices=[0, 1],
                     help="trajectories of both dense legs through H = G^T C^-1 G (the reference's "
                          "premultiplication, its default when n > d); 0: residual form")
     ap.add_argument("--incremental-residual", type=int, default=1, choices=[0, 1],
                     help="bf16: advance a float32 residual by G dx after the first stage")
-    ap.add_argument("--gemm-config-grad", type=int, default=None, choices=[0, 1, 2, 3],
+    ap.add_argument("--gemm-config-grad", type=int, default=None, choices=[0, 1, 2, 3, 4],
                     help="tile configuration of the gradient GEMM (default: --gemm-config)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-proposals", type=int, default=6,

@@ -467,7 +467,7 @@ class HMC:
             total = (self.end_time - self.start_time).total_seconds()
             names = ["residual GEMM / fused proposal", "endpoint misfit", "element-wise / Metropolis",
                      "gradient GEMM + kick/drift", "premultiplied stage GEMM + kick/drift",
-                     "surrogate GEMMs", "surrogate kick/drift"]
+                     "surrogate GEMMs", "surrogate kick/drift", "premultiplied split-tail epilogue"]
             self.diagnostics = {}
             print("Detailed statistics:")
             print(f"Total runtime: {total:.2f} seconds")

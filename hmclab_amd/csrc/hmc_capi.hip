@@ -235,13 +235,21 @@ static void evaluate_misfit(Chains& c, const void* X, double* out_dev) {
   }
 }
 
+// rows of d doubles between host (packed) and the staging buffer (row stride
+// ldx); one contiguous copy when the rows are not padded
 static void copy_rows_to_device(Chains& c, const double* host, int64_t rows) {
   Problem& P = *c.p;
-  HMC_HIP(hipMemcpy2DAsync(c.p_host, c.ldx * 8, host, P.d * 8, P.d * 8, rows, hipMemcpyHostToDevice, P.stream));
+  if (c.ldx == P.d)
+    HMC_HIP(hipMemcpyAsync(c.p_host, host, (size_t)rows * P.d * 8, hipMemcpyHostToDevice, P.stream));
+  else
+    HMC_HIP(hipMemcpy2DAsync(c.p_host, c.ldx * 8, host, P.d * 8, P.d * 8, rows, hipMemcpyHostToDevice, P.stream));
 }
 static void copy_rows_to_host(Chains& c, double* host, int64_t rows) {
   Problem& P = *c.p;
-  HMC_HIP(hipMemcpy2DAsync(host, P.d * 8, c.p_host, c.ldx * 8, P.d * 8, rows, hipMemcpyDeviceToHost, P.stream));
+  if (c.ldx == P.d)
+    HMC_HIP(hipMemcpyAsync(host, c.p_host, (size_t)rows * P.d * 8, hipMemcpyDeviceToHost, P.stream));
+  else
+    HMC_HIP(hipMemcpy2DAsync(host, P.d * 8, c.p_host, c.ldx * 8, P.d * 8, rows, hipMemcpyDeviceToHost, P.stream));
   HMC_HIP(hipStreamSynchronize(P.stream));
 }
 
@@ -474,13 +482,13 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "incremental_residual must be 0 or 1");
       P.resid_update = (int)value;
     } else if (key == "gemm_config") {
-      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0, HMC_ERR_ARG,
-                  "gemm_config must be 0, 1, 2 or 3");
+      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0 || value == 4.0, HMC_ERR_ARG,
+                  "gemm_config must be 0, 1, 2, 3 or 4");
       P.gemm_cfg = (int)value;
       P.gemm_cfg_grad = (int)value;
     } else if (key == "gemm_config_grad") {
-      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0, HMC_ERR_ARG,
-                  "gemm_config_grad must be 0, 1, 2 or 3");
+      HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0 || value == 4.0, HMC_ERR_ARG,
+                  "gemm_config_grad must be 0, 1, 2, 3 or 4");
       P.gemm_cfg_grad = (int)value;
     } else if (key == "gemm_split_tail") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_split_tail must be 0 or 1");

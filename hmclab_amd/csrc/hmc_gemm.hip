@@ -110,8 +110,11 @@ static SplitPlan plan_split(int BM, int BN, int M, int N, int K, int n_cu, bool 
   return s;
 }
 
+// order 1 with SPL dividing 8: XCD x takes only k-range x % SPL of the tail
+// tiles (8 / SPL XCDs per k-range, each a contiguous range of tiles), so an
+// XCD's L2 holds one k-range of the shared B operand instead of all of them.
 __device__ inline bool split_job(int gm, int gn, int nk, int F, int SPL, int& tm, int& tn, int& kt0, int& kt1,
-                                 int& sp) {
+                                 int& sp, int order = 0) {
   const int T = gm * gn;
   const int u = blockIdx.x;
   if (u < F) {
@@ -121,12 +124,21 @@ __device__ inline bool split_job(int gm, int gn, int nk, int F, int SPL, int& tm
     return true;
   }
   const int units = (T - F) * SPL, chunk = (units + 7) / 8, v = u - F;
-  const int t2 = (v % 8) * chunk + v / 8;
-  if (t2 >= units) return false;
-  const int g = t2 / gn;
-  tn = t2 % gn;
-  tm = F / gn + g / SPL;
-  sp = g % SPL;
+  if (order == 1 && 8 % SPL == 0) {
+    const int x = v % 8, nxs = 8 / SPL, per = (T - F + nxs - 1) / nxs;
+    const int t = (x / SPL) * per + v / 8;
+    if (v / 8 >= per || t >= T - F) return false;
+    tm = F / gn + t / gn;
+    tn = t % gn;
+    sp = x % SPL;
+  } else {
+    const int t2 = (v % 8) * chunk + v / 8;
+    if (t2 >= units) return false;
+    const int g = t2 / gn;
+    tn = t2 % gn;
+    tm = F / gn + g / SPL;
+    sp = g % SPL;
+  }
   kt0 = sp * nk / SPL;
   kt1 = (sp + 1) * nk / SPL;
   return true;
@@ -388,7 +400,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
   int tm, tn, kt0 = 0, kt1 = (K + BK - 1) / BK, sp = -1;
   if (F >= gm * gn) {
     if (!map_tile(gm, gn, tm, tn, order)) return;
-  } else if (!split_job(gm, gn, kt1, F, SPL, tm, tn, kt0, kt1, sp)) {
+  } else if (!split_job(gm, gn, kt1, F, SPL, tm, tn, kt0, kt1, sp, order)) {
     return;
   }
   const int64_t m_blk = (int64_t)tm * BM, n_blk = (int64_t)tn * BN;
@@ -1054,6 +1066,10 @@ struct SplitWs {
   size_t elems = 0;
   int n_cu = 0;
   int min_piece = 8, max_spl = 8;
+  // per-kernel event timing (when tc is set and its timing is on): the GEMM
+  // kernel into kind, the split-tail epilogue into tail_kind
+  Chains* tc = nullptr;
+  int kind = -1, tail_kind = -1;
 };
 
 template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
@@ -1063,15 +1079,21 @@ static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const
   constexpr int NT = (BM / WM) * (BN / WN) * 64;
   SplitPlan pl = plan_split(BM, BN, M, N, K, ws.n_cu, ws.ptr != nullptr, 64, ws.min_piece, ws.max_spl);
   if (pl.split() && (size_t)pl.SPL * N * pl.TR > ws.elems) pl = plan_split(BM, BN, M, N, K, 0, false);
+  const bool timed = ws.tc != nullptr && ws.kind >= 0;
+  hipEvent_t ev = nullptr;
+  if (timed) time_begin(*ws.tc, ws.kind, ev);
   hipLaunchKernelGGL((k_gemm_bf16<BM, BN, WM, WN, NSPLIT, S, Epi>), dim3(pl.grid), dim3(NT), 0, s, A, lda, B0, B1,
                      ldb, M, N, K, order, pl.F, pl.SPL, ws.ptr, pl.TR, epi);
   HMC_HIP(hipGetLastError());
+  if (timed) time_end(*ws.tc, ws.kind, ev);
   if (pl.split()) {
     const int64_t work = (int64_t)N * (pl.TR / 4);
     const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+    if (timed) time_begin(*ws.tc, ws.tail_kind, ev);
     hipLaunchKernelGGL((k_split_epilogue<Epi>), dim3(blocks), dim3(256), 0, s, ws.ptr, pl.SPL, pl.TR, N, pl.mt0,
                        epi);
     HMC_HIP(hipGetLastError());
+    if (timed) time_end(*ws.tc, ws.tail_kind, ev);
   }
 }
 
@@ -1087,6 +1109,10 @@ static size_t split_elems(int M, int N, int K, int n_cu, int min_piece = 8, int 
 // 3 = 256x128, 8 waves of 64x64, 3 stages (one-plane operands; two-plane
 //     operands use the config 1 tile): a third less G and chain traffic from
 //     L2 per MFMA than 128x128 tiles.
+// 4 = 256x128, 8 waves of 128x32, 3 stages (one-plane operands; two-plane
+//     operands use the config 1 tile): each wave runs 4 MFMAs per k-step on
+//     four A fragments and one B fragment; at the premultiplied stage shape
+//     (M = K = 8192, N = 512) its main loop takes 0.87x the time of config 3.
 // One-plane operands need 32 KB per stage of a 128x128 tile: a 4-stage ring
 // keeps three k-tiles in flight.
 template <int NSPLIT, class Epi>
@@ -1100,6 +1126,8 @@ static void launch_bf16_split(int cfg, hipStream_t st, const uint16_t* A, int64_
     launch_bf16_cfg<256, 128, 128, 64, NSPLIT, 2>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
   else if (cfg == 3 && NSPLIT == 1)
     launch_bf16_cfg<256, 128, 64, 64, 1, 3>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
+  else if (cfg == 4 && NSPLIT == 1)
+    launch_bf16_cfg<256, 128, 128, 32, 1, 3>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
   else
     launch_bf16_cfg<BF_BM, BF_BN, 64, 32, NSPLIT, S>(st, A, lda, B0, B1, ldb, M, N, K, order, ws, epi);
 }
@@ -1107,16 +1135,24 @@ static void launch_bf16_split(int cfg, hipStream_t st, const uint16_t* A, int64_
 // partial-plane elements a launch of (cfg, planes) may need
 static size_t split_elems_cfg(int cfg, int nsplit, int M, int N, int K, int n_cu) {
   if (cfg == 2) return split_elems<256, 128>(M, N, K, n_cu);
-  if (cfg == 3 && nsplit == 1) return split_elems<256, 128>(M, N, K, n_cu);
+  if ((cfg == 3 || cfg == 4) && nsplit == 1) return split_elems<256, 128>(M, N, K, n_cu);
   return split_elems<BF_BM, BF_BN>(M, N, K, n_cu);
 }
 
+// kind / tail_kind >= 0: time the GEMM kernel and its split-tail epilogue in
+// those timing groups of c
 template <class Epi>
-static void launch_bf16(const Chains& c, int cfg, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
-                        const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi) {
+static void launch_bf16(Chains& c, int cfg, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
+                        const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi, int kind = -1,
+                        int tail_kind = -1) {
   const Problem& p = *c.p;
   SplitWs ws;
   if (p.split_tail && c.split_ws) ws = SplitWs{c.split_ws, c.split_ws_elems, p.n_cu};
+  if (kind >= 0) {
+    ws.tc = &c;
+    ws.kind = kind;
+    ws.tail_kind = tail_kind;
+  }
   if (split == 2) launch_bf16_split<2>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, ws, epi);
   else launch_bf16_split<1>(cfg, p.stream, A, lda, B0, B1, ldb, M, N, K, p.tile_order, ws, epi);
 }
@@ -1243,25 +1279,26 @@ static void premult_kick_drift(Chains& c, double kick, double drift, double drif
   const DevModel md = p.model();
   if (!p.H) build_premult(c);
   if (p.precision == HMC_BF16) ensure_h_plane(p);
-  hipEvent_t ev;
-  time_begin(c, 4, ev);
   if (p.precision == HMC_BF16) {
     HMC_REQUIRE(c.anchor_ready, HMC_ERR_STATE, "anchored trajectory without an anchor");
     // g = (H x_ref - b) + H_hi bf16(x - x_ref): one plane of the offset from
-    // the anchor, written by the previous stage (or the momentum refresh)
+    // the anchor, written by the previous stage (or the momentum refresh).
+    // The GEMM kernel is timed in group 4, its split-tail epilogue in group 7.
     const int cur = c.dx_cur;
     EpiLeap<float> e{(float*)c.xt, (float*)c.pt, c.dxp[1 - cur], nullptr, c.ldx, c.eps, c.beta, kick, drift,
                      drift2, md, d, B, c.xa, c.yref, nullptr};
-    launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, c.dxp[cur], nullptr, c.ldx, d, B, (int)p.ldh, e);
+    launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, c.dxp[cur], nullptr, c.ldx, d, B, (int)p.ldh, e, 4, 7);
     c.dx_cur = 1 - cur;
   } else {
     // FP64: g = H x - b from a copy of the positions (the epilogue updates xt)
+    hipEvent_t ev;
+    time_begin(c, 4, ev);
     HMC_HIP(hipMemcpyAsync(c.xs, c.xt, (size_t)c.B_pad * c.ldx * 8, hipMemcpyDeviceToDevice, p.stream));
     EpiLeap<double> e{(double*)c.xt, (double*)c.pt, nullptr, nullptr, c.ldx, c.eps, c.beta, kick, drift, drift2,
                       md, d, B, nullptr, nullptr, p.hb};
     launch_f64(c, (const double*)p.H, p.ldh, (const double*)c.xs, c.ldx, d, B, (int)p.ldh, e);
+    time_end(c, 4, ev);
   }
-  time_end(c, 4, ev);
 }
 
 // Trajectory: residual GEMM then gradient GEMM with kick/drift epilogue.

@@ -125,11 +125,12 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 residual at the chain set's anchor x_ref (one bf16 plane of
  *                 x - x_ref per stage instead of hi/lo planes of x: half the
  *                 MFMA work of the residual GEMM; default 0)
- *   "gemm_config" 0-3: bf16 GEMM tile of both trajectory GEMMs
+ *   "gemm_config" 0-4: bf16 GEMM tile of both trajectory GEMMs
  *                 (0 = 128x128, 4 waves of 64x64; 1 = 128x128, 8 waves of
  *                 64x32; 2 = 256x128, 4 waves of 128x64; 3 = 256x128, 8 waves
- *                 of 64x64 for one-plane operands, config 1 for two; default 1)
- *   "gemm_config_grad" 0-3: the same for the gradient GEMM only
+ *                 of 64x64; 4 = 256x128, 8 waves of 128x32; 3 and 4 for
+ *                 one-plane operands, config 1 for two; default 1)
+ *   "gemm_config_grad" 0-4: the same for the gradient GEMM only
  *   "gemm_split_tail" 0 or 1: when the tiles of a bf16 GEMM leave a last
  *                 partial round on the compute units, split those tiles along
  *                 K into float32 partials summed in fixed order (default 1)
@@ -162,7 +163,9 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 of the device)
  *   "tile_order"  0 or 1: bf16 GEMM tiles dealt to the 8 XCDs in m-major
  *                 (tiles sharing G rows share an L2; default) or n-major
- *                 (tiles sharing chains share an L2) ranges
+ *                 (tiles sharing chains share an L2) ranges; for tiles split
+ *                 along K into 2, 4 or 8 k-ranges, 1 gives each XCD a single
+ *                 k-range (its L2 holds that k-range of the chain operand)
  * Must be set before hmc_chains_create. */
 int hmc_problem_set_option(hmc_problem* p, const char* name, double value);
 
@@ -294,12 +297,14 @@ int hmc_chains_trace_device(hmc_chains* c, void** x_ptr, double** chi_ptr, int* 
 int hmc_chains_reset_trace(hmc_chains* c);
 
 /* Accumulated kernel times since the last reset, from HIP events recorded on
- * the engine's stream around each launch (a launch and its split-tail
- * epilogue kernel count as one).  kind: 0 = residual GEMM (fused engine: the
- * whole-proposal kernel), 1 = endpoint misfit, 2 = element-wise / Metropolis,
- * 3 = gradient GEMM with the kick/drift epilogue (residual form),
- * 4 = premultiplied stage GEMM with the kick/drift epilogue, 5 = surrogate
- * forward and input-gradient GEMMs, 6 = surrogate kick/drift, 7 = unused.
+ * the engine's stream around each launch (in kinds 0-3 and 5 a launch and
+ * its split-tail epilogue kernel count as one).  kind: 0 = residual GEMM
+ * (fused engine: the whole-proposal kernel), 1 = endpoint misfit,
+ * 2 = element-wise / Metropolis, 3 = gradient GEMM with the kick/drift
+ * epilogue (residual form), 4 = premultiplied stage GEMM kernel with the
+ * kick/drift epilogue (bf16: the GEMM kernel alone; FP64: the whole stage),
+ * 5 = surrogate forward and input-gradient GEMMs, 6 = surrogate kick/drift,
+ * 7 = the bf16 premultiplied stage's split-tail epilogue kernel.
  * total_ms and launches may be NULL.  Timing is off until
  * hmc_chains_enable_timing(c,1). */
 int hmc_chains_enable_timing(hmc_chains* c, int on);

@@ -90,7 +90,7 @@ def test_sparse_misfit_gradient(precision):
         assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("n,d,B", [(520, 384, 3), (300, 700, 130)])
 def test_bf16_gemm_gradient_and_fp64_misfit(cfg, n, d, B):
     rng = np.random.default_rng(3)
@@ -243,9 +243,11 @@ def test_bf16_anchored_trajectory_is_reversible(premult, integ):
     assert err.max() < 5e-4, err
 
 
-@pytest.mark.parametrize("precision,cfg,B,pm", [("bf16", 1, 256, 0), ("bf16", 3, 256, 0), ("bf16", 3, 200, 0),
-                                                ("f64", 1, 256, 0), ("bf16", 3, 256, 1), ("f64", 1, 200, 1)])
-def test_split_tail_matches_unsplit(precision, cfg, B, pm):
+@pytest.mark.parametrize("precision,cfg,B,pm,order", [
+    ("bf16", 1, 256, 0, 0), ("bf16", 3, 256, 0, 0), ("bf16", 3, 200, 0, 0), ("f64", 1, 256, 0, 0),
+    ("bf16", 3, 256, 1, 0), ("f64", 1, 200, 1, 0), ("bf16", 4, 256, 1, 0), ("bf16", 4, 200, 0, 0),
+    ("bf16", 4, 256, 1, 1), ("bf16", 3, 200, 0, 1)])
+def test_split_tail_matches_unsplit(precision, cfg, B, pm, order):
     """Planning for 16 compute units makes both GEMMs of a 2500 x 2200 problem
     with 256 chains leave a partial last round (residual 40 tiles of 128 rows
     or 20 of 256, gradient 36 or 18; 200 chains leave a partly empty chain
@@ -257,7 +259,9 @@ def test_split_tail_matches_unsplit(precision, cfg, B, pm):
     oracle.  The FP64 endpoint GEMM (128-row tiles: 80 of them) splits its
     tail the same way, and so do both GEMMs of FP64 trajectories (to 1e-12).
     pm = 1: premultiplied trajectories, whose stage GEMMs (M = d = 2200)
-    split their tails the same way."""
+    split their tails the same way.  order = 1 deals the tail's k-ranges one
+    per XCD (when the split count divides 8) instead of by tile; cfg = 4 is
+    the 256x128 tile of 8 waves of 128x32."""
     rng = np.random.default_rng(21)
     n, d, L = 2500, 2200, 5
     G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
@@ -271,7 +275,7 @@ def test_split_tail_matches_unsplit(precision, cfg, B, pm):
     u = rng.uniform(size=B)
     res, grad = {}, {}
     for name, opts in (("split", {"gemm_split_cus": 16}), ("plain", {"gemm_split_tail": 0})):
-        base = {"gemm_config": cfg, "incremental_residual": 1} if precision == "bf16" else {}
+        base = {"gemm_config": cfg, "incremental_residual": 1, "tile_order": order} if precision == "bf16" else {}
         base["premult_trajectory"] = pm
         prob = build_problem(post, precision=precision, engine="gemm", options={**base, **opts})
         ch = Chains(prob, B)
