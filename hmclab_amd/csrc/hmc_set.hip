@@ -57,7 +57,61 @@ struct SetDesc {
   int64_t hm_off;                                     // premultiplied G^T C^-1 G [ldg][ldg]
   int32_t prior_kind, L;
   int32_t ptile0, n_ptiles;                           // endpoint tiles of this problem
+  int32_t S, c_off;       // gradient stages of a trajectory; first stage's coefficients
+  int32_t mass, bounded;  // diagonal mass; bounds (reflection corrector)
 };
+
+// Per-set model arrays beside the descriptors: stage coefficients {kick,
+// drift, drift2} (the integrator's schedule for each problem's L, as the
+// chain engine's), inverse / square-root mass diagonals and bounds (a_off
+// layout; present when some problem has them).
+struct SetModel {
+  const double* coef;
+  const double* minv;
+  const double* msqrt;
+  const double* lo;
+  const double* hi;
+};
+
+__device__ inline bool set_oob(const SetModel& m, const SetDesc& D, int64_t aj, double x) {
+  return D.bounded && (x < m.lo[aj] || x > m.hi[aj]);
+}
+
+// dK/dp (MassMatrices.py:133, 218)
+__device__ inline float set_kgrad(const SetModel& m, const SetDesc& D, int64_t aj, float p) {
+  return D.mass ? (float)m.minv[aj] * p : p;
+}
+
+// reflection at the problem's bounds, lower first then upper (base.py:257-270)
+__device__ inline void set_correct(const SetModel& m, const SetDesc& D, int64_t aj, float& x, float& p) {
+  if (!D.bounded) return;
+  const float lo = (float)m.lo[aj];
+  if (x < lo) {
+    x += 2.f * (lo - x);
+    p *= -1.f;
+  }
+  const float hi = (float)m.hi[aj];
+  if (x > hi) {
+    x += 2.f * (hi - x);
+    p *= -1.f;
+  }
+}
+
+// Stage `step` of one coordinate: gradient g (likelihood + prior), kick,
+// drift(s) with the corrector after each (Samplers.py:1524-1726).
+__device__ inline void set_leap(const SetModel& m, const SetDesc& D, int64_t aj, int step, double e, float g,
+                                float& xv, float& pv) {
+  const double* cf = m.coef + 3 * ((int64_t)D.c_off + step);
+  const float kk = (float)(cf[0] * e), dd = (float)(cf[1] * e), dd2 = (float)(cf[2] * e);
+  if (set_oob(m, D, aj, (double)xv)) g = INFINITY;
+  pv -= kk * g;
+  xv += dd * set_kgrad(m, D, aj, pv);
+  set_correct(m, D, aj, xv, pv);
+  if (cf[2] != 0.0) {
+    xv += dd2 * set_kgrad(m, D, aj, pv);
+    set_correct(m, D, aj, xv, pv);
+  }
+}
 
 struct SetRun {
   uint64_t seed;
@@ -193,7 +247,7 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
                                                        const double* __restrict__ pa,
                                                        const double* __restrict__ pb,
                                                        const double* __restrict__ hb,
-                                                       const double* __restrict__ eps, int step) {
+                                                       const double* __restrict__ eps, int step, SetModel md) {
   constexpr int ROWS = 64 * RW, LD = KC + 4;
   constexpr int LPR = KC / 4, RPP = 256 / LPR;   // loader lanes per row, rows per pass
   constexpr int GPT = ROWS / RPP;                // float4 loads of G per thread per chunk
@@ -209,7 +263,7 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
   const int nr = KICK ? D.d : D.n;               // rows of the operator
   const int nrow = (nr + 15) & ~15;              // rows present in memory (zero padded)
   const float* Gp = G + (KICK ? D.hm_off : D.g_off);
-  const bool flip = KICK && ((D.L - step) & 1);
+  const bool flip = KICK && ((D.S - step) & 1);
   const float* Xin = flip ? X2 : X;
   const float* Xp = Xin + D.x_off;
   // loader mapping: thread tid covers column 4 * (tid % LPR) of rows tid / LPR + RPP j
@@ -263,9 +317,8 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
   }
   if (KICK) {
     // lane (i, q) holds parameters row0 .. row0 + 3 of chain i
-    if (step >= D.L) return;
-    const float ec = (float)eps[(int64_t)t.x * SC + i];
-    const float drift = (step == D.L - 1) ? 0.5f : 1.f;
+    if (step >= D.S) return;
+    const double ec = eps[(int64_t)t.x * SC + i];
     float* Xout = flip ? X : X2;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
@@ -279,9 +332,9 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
         const float xv0 = Xin[o];
         const float gv = (acc[r][v] - (float)hb[D.a_off + j]) +
                          set_prior_grad(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], xv0);
-        float pv = P[o];
-        pv -= ec * gv;
-        Xout[o] = xv0 + (drift * ec) * pv;
+        float xv = xv0, pv = P[o];
+        set_leap(md, D, D.a_off + j, step, ec, gv, xv, pv);
+        Xout[o] = xv;
         P[o] = pv;
       }
     }
@@ -326,7 +379,8 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
                                                       const double* __restrict__ eps, int step,
                                                       double* __restrict__ gout,
                                                       float* __restrict__ X2,
-                                                      const double* __restrict__ hb) {
+                                                      const double* __restrict__ hb, SetModel md,
+                                                      const int32_t* __restrict__ oob) {
   __shared__ float red[NW][SC][64];
   const int2 t = tiles[blockIdx.x];
   const SetDesc D = desc[t.x];
@@ -337,7 +391,7 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
   floatx4 acc[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const bool flip = MODE == 2 && ((D.L - step) & 1);
+  const bool flip = MODE == 2 && ((D.S - step) & 1);
   const float* Xin = flip ? X2 : X;
   float* Xout = MODE == 2 ? (flip ? X : X2) : X;
   const float* wrow = MODE == 2 ? Xin + D.x_off + (int64_t)i * D.ldg + 4 * q
@@ -393,14 +447,14 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
     if (MODE == 2) gs -= (float)hb[D.a_off + j];
     const float gv = gs + set_prior_grad(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], xv0);
     if (MODE == 1) {
-      gout[D.h_off + (int64_t)c * D.d + j] = (double)gv;
+      // BayesRule gradient: + inf everywhere when the chain is outside the
+      // bounds (base.py:1045-1054)
+      gout[D.h_off + (int64_t)c * D.d + j] = oob[(int64_t)t.x * SC + c] ? INFINITY : (double)gv;
     } else {
-      if (step >= D.L) continue;
-      const float ec = (float)eps[(int64_t)t.x * SC + c];
-      const float drift = (step == D.L - 1) ? 0.5f : 1.f;
-      float pv = P[o];
-      pv -= ec * gv;
-      Xout[o] = xv0 + (drift * ec) * pv;
+      if (step >= D.S) continue;
+      float xv = xv0, pv = P[o];
+      set_leap(md, D, D.a_off + j, step, eps[(int64_t)t.x * SC + c], gv, xv, pv);
+      Xout[o] = xv;
       P[o] = pv;
     }
   }
@@ -558,7 +612,8 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
                                                     double* __restrict__ k0, double* __restrict__ eps,
                                                     double* __restrict__ u,
                                                     const double* __restrict__ stepv,
-                                                    const SetRun* __restrict__ run) {
+                                                    const SetRun* __restrict__ run, double drift0,
+                                                    SetModel md) {
   __shared__ double scratch[SW / 64 + 1];
   const int64_t gc = blockIdx.x;
   const SetDesc D = desc[gc / SC];
@@ -576,11 +631,11 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
   } else {
     e = eps[gc];
   }
-  const float half = (float)(0.5 * e);
+  const float dd = (float)(drift0 * e);
   double ks = 0.0;
   const int64_t base = D.x_off + (int64_t)c * D.ldg;
-  // premultiplied trajectories start in buffer L mod 2 (their last step writes XT)
-  float* xs = (XT2 && (D.L & 1)) ? XT2 : XT;
+  // premultiplied trajectories start in buffer S mod 2 (their last stage writes XT)
+  float* xs = (XT2 && (D.S & 1)) ? XT2 : XT;
   const int npairs = (D.d + 1) / 2;
   for (int qp = threadIdx.x; qp < npairs; qp += SW) {
     double z[2] = {0.0, 0.0};
@@ -589,9 +644,14 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
     for (int h = 0; h < 2; ++h) {
       const int j = 2 * qp + h;
       if (j >= D.d) break;
-      const float pv = R.host_p ? (float)p_host[D.h_off + (int64_t)c * D.d + j] : (float)z[h];
-      ks += (double)pv * (double)pv;
-      xs[base + j] = X0[base + j] + half * pv;
+      const int64_t aj = D.a_off + j;
+      float pv = R.host_p ? (float)p_host[D.h_off + (int64_t)c * D.d + j]
+                          : (float)(D.mass ? md.msqrt[aj] * z[h] : z[h]);
+      ks += D.mass ? (double)pv * (md.minv[aj] * (double)pv) : (double)pv * (double)pv;
+      float xv = X0[base + j];
+      xv += dd * set_kgrad(md, D, aj, pv);
+      set_correct(md, D, aj, xv, pv);
+      xs[base + j] = xv;
       PT[base + j] = pv;
     }
   }
@@ -599,20 +659,25 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
   if (threadIdx.x == 0) k0[gc] = 0.5 * s;
 }
 
+// misfit of chain c (prior + likelihood partials; inf outside the bounds,
+// base.py:361-374) and, with p, its kinetic energy (MassMatrices.py:113, 198)
 __device__ inline double set_chain_misfit(const SetDesc& D, const float* x, const double* partial,
                                           int c, const double* pa, const double* pb, double* scratch,
-                                          const float* p, double* kin) {
-  double pm = 0.0, kk = 0.0;
+                                          const float* p, double* kin, const SetModel& md) {
+  double pm = 0.0, kk = 0.0, ob = 0.0;
   for (int j = threadIdx.x; j < D.d; j += SW) {
-    pm += set_prior_misfit(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], (double)x[j]);
-    if (p) kk += (double)p[j] * (double)p[j];
+    const int64_t aj = D.a_off + j;
+    pm += set_prior_misfit(D.prior_kind, pa[aj], pb[aj], (double)x[j]);
+    if (set_oob(md, D, aj, (double)x[j])) ob = 1.0;
+    if (p) kk += D.mass ? (double)p[j] * (md.minv[aj] * (double)p[j]) : (double)p[j] * (double)p[j];
   }
   pm = block_sum<SW>(pm, scratch);
   if (p) kk = block_sum<SW>(kk, scratch);
+  if (D.bounded) ob = block_sum<SW>(ob, scratch);
   double lik = 0.0;
   for (int k = 0; k < D.n_ptiles; ++k) lik += partial[(int64_t)(D.ptile0 + k) * SC + c];
   if (kin) *kin = 0.5 * kk;
-  return 0.5 * lik + pm;
+  return ob > 0.0 ? INFINITY : 0.5 * lik + pm;
 }
 
 __global__ __launch_bounds__(SW) void k_set_metropolis(
@@ -621,7 +686,7 @@ __global__ __launch_bounds__(SW) void k_set_metropolis(
     const double* __restrict__ pb, double* __restrict__ chi0, const double* __restrict__ k0,
     const double* __restrict__ u, double* __restrict__ h0o, double* __restrict__ h1o,
     int32_t* __restrict__ acc, int64_t* __restrict__ acc_count, const SetRun* __restrict__ run,
-    float* __restrict__ tr_x, double* __restrict__ tr_chi, int64_t tr_row, int64_t nchains) {
+    float* __restrict__ tr_x, double* __restrict__ tr_chi, int64_t tr_row, int64_t nchains, SetModel md) {
   __shared__ double scratch[SW / 64 + 1];
   __shared__ int accepted;
   __shared__ double chi_new;
@@ -630,7 +695,7 @@ __global__ __launch_bounds__(SW) void k_set_metropolis(
   const int c = (int)(gc % SC);
   const int64_t base = D.x_off + (int64_t)c * D.ldg;
   double k1;
-  const double chi1 = set_chain_misfit(D, XT + base, partial, c, pa, pb, scratch, PT + base, &k1);
+  const double chi1 = set_chain_misfit(D, XT + base, partial, c, pa, pb, scratch, PT + base, &k1, md);
   if (threadIdx.x == 0) {
     const double H0 = chi0[gc] + k0[gc];
     const double H1 = chi1 + k1;
@@ -666,14 +731,29 @@ __global__ __launch_bounds__(SW) void k_set_finish(const SetDesc* __restrict__ d
                                                    const double* __restrict__ partial,
                                                    const double* __restrict__ pa,
                                                    const double* __restrict__ pb,
-                                                   double* __restrict__ out) {
+                                                   double* __restrict__ out, SetModel md) {
   __shared__ double scratch[SW / 64 + 1];
   const int64_t gc = blockIdx.x;
   const SetDesc D = desc[gc / SC];
   const int c = (int)(gc % SC);
   const double chi = set_chain_misfit(D, X + D.x_off + (int64_t)c * D.ldg, partial, c, pa, pb, scratch,
-                                      nullptr, nullptr);
+                                      nullptr, nullptr, md);
   if (threadIdx.x == 0) out[gc] = chi;
+}
+
+// per chain: 1 when its state lies outside its problem's bounds
+__global__ __launch_bounds__(SW) void k_set_oob(const SetDesc* __restrict__ desc, const float* __restrict__ X,
+                                                int32_t* __restrict__ oob, SetModel md) {
+  __shared__ double scratch[SW / 64 + 1];
+  const int64_t gc = blockIdx.x;
+  const SetDesc D = desc[gc / SC];
+  const int c = (int)(gc % SC);
+  double ob = 0.0;
+  if (D.bounded)
+    for (int j = threadIdx.x; j < D.d; j += SW)
+      if (set_oob(md, D, D.a_off + j, (double)X[D.x_off + (int64_t)c * D.ldg + j])) ob = 1.0;
+  ob = block_sum<SW>(ob, scratch);
+  if (threadIdx.x == 0) oob[gc] = ob > 0.0 ? 1 : 0;
 }
 
 // host-layout f64 rows <-> padded device fp32 states
@@ -702,6 +782,7 @@ struct SetProblemHost {
   int64_t d = 0, n = 0;
   bool has_data = false;
   int prior_kind = 0;
+  bool mass = false, bounded = false;
 };
 
 struct ProblemSet {
@@ -738,8 +819,15 @@ struct ProblemSet {
   // per step s: tiles of the problems with L > s at [off[s], off[s] + count[s])
   std::vector<int64_t> roff, rprefix, goff, gprefix, hoff, hprefix;
   int64_t n_ptiles = 0;
-  int Lmax = 0;
-  std::vector<int32_t> Lcur;
+  // integrator (HMC_INT_*), its first drift, and per problem the stage count
+  // of a trajectory (L, 3L or 4L); Smax the largest
+  int integ = HMC_INT_LEAPFROG;
+  double fd = 0.5;
+  int Smax = 0;
+  std::vector<int32_t> Lcur, Scur;
+  double* coef = nullptr;           // stage coefficients (in tile_mem)
+  double *minv = nullptr, *msqrt = nullptr, *lo = nullptr, *hi = nullptr;
+  SetModel model() const { return SetModel{coef, minv, msqrt, lo, hi}; }
   bool finalized = false, state_set = false;
   // graph
   hipGraph_t graph = nullptr;
@@ -791,6 +879,7 @@ static void free_tiles(ProblemSet& s) {
   for (void* p : s.tile_mem) (void)hipFree(p);
   s.tile_mem.clear();
   s.rtiles = s.gtiles = s.htiles = nullptr;
+  s.coef = nullptr;
 }
 
 static void drop_graph(ProblemSet& s) {
@@ -806,8 +895,26 @@ static void drop_graph(ProblemSet& s) {
 // and short ones fill the end of the launch.  Step 0 covers every problem.
 static void build_tiles(ProblemSet& s) {
   free_tiles(s);
-  s.Lmax = 0;
-  for (int64_t p = 0; p < s.P; ++p) s.Lmax = std::max(s.Lmax, (int)s.Lcur[p]);
+  // each problem's stage schedule for its L (the chain engine's integrator
+  // tables, Samplers.py:1524-1726)
+  std::vector<double> coef;
+  std::vector<Stage> sched;
+  s.Scur.assign(s.P, 0);
+  s.Smax = 0;
+  for (int64_t p = 0; p < s.P; ++p) {
+    integrator_schedule(s.integ, s.Lcur[p], s.fd, sched);
+    s.hd[p].c_off = (int32_t)(coef.size() / 3);
+    s.hd[p].S = (int32_t)sched.size();
+    s.Scur[p] = (int32_t)sched.size();
+    s.Smax = std::max(s.Smax, (int)sched.size());
+    for (const Stage& g : sched) {
+      coef.push_back(g.kick);
+      coef.push_back(g.drift);
+      coef.push_back(g.drift2);
+    }
+  }
+  s.coef = (double*)dev_alloc(s.tile_mem, std::max<size_t>(coef.size(), 3) * 8);
+  h2d(s.coef, coef.data(), coef.size() * 8);
   std::vector<int2> rt, gt, ht;
   const int rows = 64 * s.rw;
   auto build = [&](std::vector<int2>& out, std::vector<int64_t>& off, std::vector<int64_t>& cnt,
@@ -815,12 +922,12 @@ static void build_tiles(ProblemSet& s) {
     std::vector<int32_t> ord(s.P);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return cost(a) > cost(b); });
-    off.assign(s.Lmax, 0);
-    cnt.assign(s.Lmax, 0);
-    for (int st = 0; st < s.Lmax; ++st) {
+    off.assign(s.Smax, 0);
+    cnt.assign(s.Smax, 0);
+    for (int st = 0; st < s.Smax; ++st) {
       off[st] = (int64_t)out.size();
       for (int32_t p : ord) {
-        if (s.Lcur[p] <= st) continue;
+        if (s.Scur[p] <= st) continue;
         const int64_t nb = ntiles(p);
         for (int64_t b = 0; b < nb; ++b) out.push_back(make_int2(p, (int)b));
       }
@@ -841,7 +948,11 @@ static void build_tiles(ProblemSet& s) {
   h2d(s.rtiles, rt.data(), rt.size() * sizeof(int2));
   h2d(s.gtiles, gt.data(), gt.size() * sizeof(int2));
   h2d(s.htiles, ht.data(), ht.size() * sizeof(int2));
-  for (int64_t p = 0; p < s.P; ++p) s.hd[p].L = s.Lcur[p];
+  for (int64_t p = 0; p < s.P; ++p) {
+    s.hd[p].L = s.Lcur[p];
+    s.hd[p].mass = s.hp[p].mass ? 1 : 0;
+    s.hd[p].bounded = s.hp[p].bounded ? 1 : 0;
+  }
   h2d(s.desc, s.hd.data(), s.P * sizeof(SetDesc));
   drop_graph(s);
 }
@@ -853,7 +964,7 @@ static void launch_rows_kc(ProblemSet& s, int64_t ntiles, const int2* tiles, con
                            int step) {
   hipLaunchKernelGGL((k_set_resid_lds<RW, KICK, KC>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc,
                      tiles, Gop, X, s.dobs, s.ivar, s.w, KICK ? s.xt2 : nullptr, KICK ? s.pt : nullptr, s.pa,
-                     s.pb, KICK ? s.hb : nullptr, s.eps, step);
+                     s.pb, KICK ? s.hb : nullptr, s.eps, step, s.model());
 }
 
 // chunk width s.kc (64 or 128 columns); 128 only for RW <= 2 (LDS size)
@@ -922,21 +1033,21 @@ static void launch_grad(ProblemSet& s, float* X, int step, int mode) {
   if (mode == 0) {
     if (s.nw == 4)
       hipLaunchKernelGGL((k_set_grad<4, 0>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
-                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr);
+                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr, s.model(), nullptr);
     else
       hipLaunchKernelGGL((k_set_grad<8, 0>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, tiles,
-                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr);
+                         s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, nullptr, nullptr, s.model(), nullptr);
   } else if (mode == 2) {
     // X is xt; the premultiplied step alternates between xt and xt2
     if (s.nw == 4)
       hipLaunchKernelGGL((k_set_grad<4, 2>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
-                         s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb);
+                         s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb, s.model(), nullptr);
     else
       hipLaunchKernelGGL((k_set_grad<8, 2>), dim3((unsigned)ntiles), dim3(512), 0, s.stream, s.desc, tiles,
-                         s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb);
+                         s.H, nullptr, X, s.pt, s.pa, s.pb, s.eps, step, nullptr, s.xt2, s.hb, s.model(), nullptr);
   } else {
     hipLaunchKernelGGL((k_set_grad<4, 1>), dim3((unsigned)ntiles), dim3(256), 0, s.stream, s.desc, tiles,
-                       s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, s.gout, nullptr, nullptr);
+                       s.G, s.w, X, s.pt, s.pa, s.pb, s.eps, step, s.gout, nullptr, nullptr, s.model(), s.acc);
   }
   HMC_HIP(hipGetLastError());
   set_time(s, 1, false, e);
@@ -957,10 +1068,10 @@ static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
   hipEvent_t e;
   set_time(s, 3, true, e);
   hipLaunchKernelGGL(k_set_refresh, dim3((unsigned)B), dim3(SW), 0, s.stream, s.desc, s.x0, s.xt,
-                     s.pm ? s.xt2 : nullptr, s.pt, s.p_host, s.k0, s.eps, s.u, s.step, s.run);
+                     s.pm ? s.xt2 : nullptr, s.pt, s.p_host, s.k0, s.eps, s.u, s.step, s.run, s.fd, s.model());
   HMC_HIP(hipGetLastError());
   set_time(s, 3, false, e);
-  for (int st = 0; st < s.Lmax; ++st) {
+  for (int st = 0; st < s.Smax; ++st) {
     if (s.pm) {
       launch_premult_step(s, st);
     } else {
@@ -973,7 +1084,7 @@ static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
   hipLaunchKernelGGL(k_set_metropolis, dim3((unsigned)B), dim3(SW), 0, s.stream, s.desc, s.x0, s.xt, s.pt,
                      s.partial, s.pa, s.pb, s.chi0, s.k0, s.u, want_energies ? s.h0 : nullptr,
                      want_energies ? s.h1 : nullptr, s.acc, s.acc_count, s.run, trace ? s.tr_x : nullptr,
-                     s.tr_chi, s.h_total, B);
+                     s.tr_chi, s.h_total, B, s.model());
   hipLaunchKernelGGL(k_set_advance, dim3(1), dim3(64), 0, s.stream, s.run);
   HMC_HIP(hipGetLastError());
   set_time(s, 3, false, e);
@@ -982,7 +1093,7 @@ static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
 static void set_misfit_of(ProblemSet& s, const float* X, double* out) {
   launch_endpoint(s, X);
   hipLaunchKernelGGL(k_set_finish, dim3((unsigned)s.chains()), dim3(SW), 0, s.stream, s.desc, X, s.partial,
-                     s.pa, s.pb, out);
+                     s.pa, s.pb, out, s.model());
   HMC_HIP(hipGetLastError());
 }
 
@@ -998,6 +1109,8 @@ static void finalize(ProblemSet& s) {
     for (int64_t b = 0; b < nr; ++b) pt.push_back(make_int2((int)p, (int)b));
     s.hd[p].n_ptiles = (int32_t)nr;
     s.hd[p].prior_kind = s.hp[p].prior_kind;
+    s.hd[p].mass = s.hp[p].mass ? 1 : 0;
+    s.hd[p].bounded = s.hp[p].bounded ? 1 : 0;
   }
   s.n_ptiles = (int64_t)pt.size();
   s.ptiles = (int2*)dev_alloc(s.owned, pt.size() * sizeof(int2));
@@ -1286,6 +1399,70 @@ int hmc_set_problem_prior(hmc_set* h, int64_t i, int kind, const double* a, cons
   });
 }
 
+// Diagonal mass of problem i (MassMatrices.py:159-238): kind HMC_MASS_DIAGONAL
+// with diag[d] > 0, or HMC_MASS_UNIT (diag ignored) to return to Unit mass.
+int hmc_set_problem_mass(hmc_set* h, int64_t i, int kind, const double* diag) {
+  return guarded("hmc_set_problem_mass", [&] {
+    HMC_REQUIRE(h, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(i >= 0 && i < s.P, HMC_ERR_ARG, "problem index out of range");
+    HMC_REQUIRE(kind == HMC_MASS_UNIT || kind == HMC_MASS_DIAGONAL, HMC_ERR_ARG, "unknown mass kind");
+    HMC_REQUIRE(!s.finalized, HMC_ERR_STATE, "the set is already in use; masses cannot change");
+    const int64_t d = s.hp[i].d;
+    if (kind == HMC_MASS_UNIT) {
+      s.hp[i].mass = false;
+      return;
+    }
+    HMC_REQUIRE(diag, HMC_ERR_ARG, "NULL mass diagonal");
+    std::vector<double> mi(d), ms(d);
+    for (int64_t j = 0; j < d; ++j) {
+      HMC_REQUIRE(diag[j] > 0.0 && std::isfinite(diag[j]), HMC_ERR_ARG, "mass diagonal must be positive");
+      mi[j] = 1.0 / diag[j];
+      ms[j] = std::sqrt(diag[j]);
+    }
+    if (!s.minv) {
+      s.minv = (double*)dev_alloc(s.owned, s.a_total * 8);
+      s.msqrt = (double*)dev_alloc(s.owned, s.a_total * 8);
+    }
+    h2d(s.minv + s.hd[i].a_off, mi.data(), d * 8);
+    h2d(s.msqrt + s.hd[i].a_off, ms.data(), d * 8);
+    s.hp[i].mass = true;
+  });
+}
+
+// Bounds of problem i (the BayesRule's collapsed bounds, base.py:1061-1101):
+// lower / upper [d] or NULL (no bound on that side); lower < upper
+// elementwise (base.py:349-359).  Both NULL removes the bounds.
+int hmc_set_problem_bounds(hmc_set* h, int64_t i, const double* lower, const double* upper) {
+  return guarded("hmc_set_problem_bounds", [&] {
+    HMC_REQUIRE(h, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(i >= 0 && i < s.P, HMC_ERR_ARG, "problem index out of range");
+    HMC_REQUIRE(!s.finalized, HMC_ERR_STATE, "the set is already in use; bounds cannot change");
+    const int64_t d = s.hp[i].d;
+    if (!lower && !upper) {
+      s.hp[i].bounded = false;
+      return;
+    }
+    std::vector<double> lo(d, -INFINITY), hi(d, INFINITY);
+    for (int64_t j = 0; j < d; ++j) {
+      if (lower) lo[j] = lower[j];
+      if (upper) hi[j] = upper[j];
+      HMC_REQUIRE(!(std::isnan(lo[j]) || std::isnan(hi[j])) && lo[j] < hi[j], HMC_ERR_ARG,
+                  "Bounds vectors are incompatible.");
+    }
+    if (!s.lo) {
+      s.lo = (double*)dev_alloc(s.owned, s.a_total * 8);
+      s.hi = (double*)dev_alloc(s.owned, s.a_total * 8);
+    }
+    h2d(s.lo + s.hd[i].a_off, lo.data(), d * 8);
+    h2d(s.hi + s.hd[i].a_off, hi.data(), d * 8);
+    s.hp[i].bounded = true;
+  });
+}
+
+// "integrator" HMC_INT_LEAPFROG | HMC_INT_3STAGE | HMC_INT_4STAGE (every
+// problem; a trajectory of L steps makes L, 3L or 4L gradient stages),
 // "autotune" (1: time the tile shapes on first use, 0: keep the set ones),
 // "resid_rows" (64, 128, 256 rows per residual tile), "grad_waves" (4 or 8),
 // "resid_lds" (1: residual tiles staged through LDS, 0: read into registers),
@@ -1298,7 +1475,11 @@ int hmc_set_set_option(hmc_set* h, const char* name, double value) {
     HMC_REQUIRE(h && name, HMC_ERR_ARG, "NULL argument");
     ProblemSet& s = h->s;
     const std::string k(name);
-    if (k == "autotune") {
+    if (k == "integrator") {
+      HMC_REQUIRE(value == HMC_INT_LEAPFROG || value == HMC_INT_3STAGE || value == HMC_INT_4STAGE, HMC_ERR_ARG,
+                  "integrator must be 0 (lf), 1 (3s) or 2 (4s)");
+      s.integ = (int)value;
+    } else if (k == "autotune") {
       s.autotune = value != 0.0;
     } else if (k == "resid_rows") {
       HMC_REQUIRE(value == 64 || value == 128 || value == 256, HMC_ERR_ARG, "resid_rows must be 64, 128 or 256");
@@ -1344,6 +1525,7 @@ int hmc_set_get_option(hmc_set* h, const char* name, double* value) {
     else if (k == "lds_chunk") *value = s.kc;
     else if (k == "autotune") *value = s.autotune;
     else if (k == "premult") *value = s.pm ? 1.0 : 0.0;
+    else if (k == "integrator") *value = s.integ;
     else throw Error{HMC_ERR_ARG, "unknown option '" + k + "'"};
   });
 }
@@ -1420,6 +1602,10 @@ int hmc_set_gradient(hmc_set* h, const double* X, double* out) {
     HMC_HIP(hipMemcpyAsync(s.xin, X, s.h_total * 8, hipMemcpyHostToDevice, s.stream));
     hipLaunchKernelGGL(k_set_load, dim3(8, (unsigned)s.chains()), dim3(256), 0, s.stream, s.desc, s.xin, s.xt,
                        s.h_total);
+    // out-of-bounds flags of the chains (in the accept-flag buffer)
+    hipLaunchKernelGGL(k_set_oob, dim3((unsigned)s.chains()), dim3(SW), 0, s.stream, s.desc, s.xt, s.acc,
+                       s.model());
+    HMC_HIP(hipGetLastError());
     launch_resid(s, 0, s.xt);
     launch_grad(s, s.xt, 0, 1);
     HMC_HIP(hipMemcpyAsync(out, s.gout, s.h_total * 8, hipMemcpyDeviceToHost, s.stream));

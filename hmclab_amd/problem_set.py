@@ -4,11 +4,13 @@ hmclab samples one target per ``HMC.sample`` call and runs several chains as
 separate processes (``ParallelSampleSMP``, hmclab/Samplers.py:1757-1977).
 :class:`ProblemSet` holds P problems of uneven size — each a
 ``BayesRule([Normal|Laplace prior, LinearMatrix(G float32, d, variance)])``
-with Unit mass — and advances 16 chains per problem in shared launches: the
-leapfrog trajectory (Samplers.py:1524-1584) in float32 on MFMA, the endpoint
-misfit and Metropolis test (Samplers.py:1471-1492) in FP64.  Each problem has
-its own number of leapfrog steps; a whole proposal round is one replay of a
-captured hipGraph.
+with optional bounds (reflection corrector, base.py:239-374) and Unit or
+Diagonal mass (MassMatrices.py:82-238) — and advances 16 chains per problem
+in shared launches: the leapfrog, 3-stage or 4-stage trajectory
+(Samplers.py:1524-1726) in float32 on MFMA, the endpoint misfit and
+Metropolis test (Samplers.py:1471-1492) in FP64.  Each problem has its own
+number of steps; a whole proposal round is one replay of a captured
+hipGraph.
 
 States, momenta and gradients use a flat host layout: problem after problem,
 16 rows of ``d_p`` values.  :meth:`split` and :meth:`join` convert between
@@ -46,10 +48,13 @@ class ProblemSet:
         return self.P * CHAINS
 
     # ---- problem data ----------------------------------------------------------
-    def set_problem(self, i: int, G, d_obs, variance, prior: str = "normal", means=None, scale=1.0):
+    def set_problem(self, i: int, G, d_obs, variance, prior: str = "normal", means=None, scale=1.0,
+                    mass=None, lower=None, upper=None):
         """Problem i: likelihood N(G m, variance) with G (n, d) float32, and a
         Normal (``scale`` = standard deviation) or Laplace (``scale`` =
-        dispersion) prior with the given means (default 0)."""
+        dispersion) prior with the given means (default 0).  ``mass``: the
+        diagonal of a Diagonal mass matrix (default Unit); ``lower`` /
+        ``upper``: bounds of the target (default none)."""
         G = np.ascontiguousarray(np.asarray(G, dtype=np.float32))
         n, d = int(self.n[i]), int(self.d[i])
         if G.shape != (n, d):
@@ -70,6 +75,15 @@ class ProblemSet:
             raise ValueError("prior must be 'normal' or 'laplace'")
         call("hmc_set_problem_prior", self._h, int(i), kind, dptr(np.ascontiguousarray(a)),
              dptr(np.ascontiguousarray(b, dtype=np.float64)))
+        if mass is not None:
+            m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, dtype=np.float64).reshape(-1), (d,)))
+            call("hmc_set_problem_mass", self._h, int(i), _lib.MASS_DIAGONAL, dptr(m))
+        if lower is not None or upper is not None:
+            lo = None if lower is None else np.ascontiguousarray(
+                np.broadcast_to(np.asarray(lower, dtype=np.float64).reshape(-1), (d,)))
+            hi = None if upper is None else np.ascontiguousarray(
+                np.broadcast_to(np.asarray(upper, dtype=np.float64).reshape(-1), (d,)))
+            call("hmc_set_problem_bounds", self._h, int(i), dptr(lo), dptr(hi))
 
     def set_option(self, name: str, value: float):
         call("hmc_set_set_option", self._h, name.encode(), float(value))
@@ -126,7 +140,15 @@ class ProblemSet:
     def _L(self, L):
         return np.ascontiguousarray(np.broadcast_to(np.asarray(L, dtype=np.int32), (self.P,)))
 
-    def propose(self, P0, eps, u, L):
+    def set_integrator(self, integrator: str):
+        """"lf", "3s" or "4s" (Samplers.py:1524-1726), for every problem."""
+        if integrator not in _lib.INTEGRATORS:
+            raise ValueError("integrator must be 'lf', '3s' or '4s'")
+        self.set_option("integrator", _lib.INTEGRATORS[integrator])
+
+    def propose(self, P0, eps, u, L, integrator=None):
+        if integrator is not None:
+            self.set_integrator(integrator)
         eps = np.ascontiguousarray(np.asarray(eps, dtype=np.float64).reshape(self.chains))
         u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(self.chains))
         acc = np.empty(self.chains, dtype=np.int32)
@@ -135,7 +157,10 @@ class ProblemSet:
              iptr(acc), dptr(h0), dptr(h1), dptr(X1))
         return dict(accepted=acc.astype(bool), H0=h0, H1=h1, X1=X1)
 
-    def run(self, proposals: int, L, stepsize, randomize=True, seed=0, first_proposal=0, graph=True):
+    def run(self, proposals: int, L, stepsize, randomize=True, seed=0, first_proposal=0, graph=True,
+            integrator=None):
+        if integrator is not None:
+            self.set_integrator(integrator)
         step = np.ascontiguousarray(np.broadcast_to(np.asarray(stepsize, dtype=np.float64), (self.chains,)))
         acc = np.zeros(self.chains, dtype=np.int64)
         call("hmc_set_run", self._h, int(proposals), iptr(self._L(L)), dptr(step), int(bool(randomize)),

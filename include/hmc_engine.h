@@ -378,13 +378,15 @@ int hmc_chains_set_surrogate(hmc_chains* c, hmc_surrogate* s);
 
 /* ---- sets of independent problems -----------------------------------------
  * Replaces running one HMC.sample per problem and per chain in separate
- * processes (ParallelSampleSMP, Samplers.py:1757-1977, around
- * HMC._propagate_leapfrog :1524-1584 and HMC._evaluate_acceptance
- * :1471-1492): P dense linear problems of uneven size (G float32, n_p x d_p),
- * each with one Normal or Laplace prior (base.py:451-574, 653-710), Unit
- * mass (MassMatrices.py:82-156), leapfrog with per-problem step counts, and
- * 16 chains per problem, advance together.  Trajectories run in float32 on
- * MFMA; the endpoint misfit and Metropolis test are FP64.
+ * processes (ParallelSampleSMP, Samplers.py:1757-1977, around the
+ * integrators :1524-1726 and HMC._evaluate_acceptance :1471-1492): P dense
+ * linear problems of uneven size (G float32, n_p x d_p), each with one Normal
+ * or Laplace prior (base.py:451-574, 653-710), Unit or Diagonal mass
+ * (MassMatrices.py:82-238), optional bounds with the reflection corrector
+ * (base.py:239-374), the leapfrog, 3-stage or 4-stage integrator with
+ * per-problem step counts, and 16 chains per problem, advance together.
+ * Trajectories run in float32 on MFMA; the endpoint misfit and Metropolis
+ * test are FP64.
  * Host layout of states, momenta and gradients: problem after problem, 16
  * rows (chains) of d_p doubles each.  Chain index gc = p * 16 + c. */
 typedef struct hmc_set hmc_set;
@@ -397,7 +399,17 @@ int hmc_set_problem_dense(hmc_set* s, int64_t problem, const float* G, const dou
 /* HMC_PRIOR_NORMAL (a = means, b = inverse variances) or HMC_PRIOR_LAPLACE
  * (a = means, b = inverse dispersions), [d] each. */
 int hmc_set_problem_prior(hmc_set* s, int64_t problem, int kind, const double* a, const double* b);
-/* "autotune" 1|0 (time the tile shapes on this set at first run; default 1),
+/* Diagonal mass (MassMatrices.py:159-238): kind HMC_MASS_DIAGONAL with
+ * diagonal[d] > 0, or HMC_MASS_UNIT (diagonal ignored; the default). */
+int hmc_set_problem_mass(hmc_set* s, int64_t problem, int kind, const double* diagonal);
+/* Collapsed bounds of the problem's target (base.py:1061-1101): lower and
+ * upper [d] or NULL for no bound on that side; lower < upper elementwise
+ * ("Bounds vectors are incompatible.", base.py:359).  Trajectories reflect at
+ * the bounds (base.py:257-270); the misfit is inf outside them (:361-374). */
+int hmc_set_problem_bounds(hmc_set* s, int64_t problem, const double* lower, const double* upper);
+/* "integrator" HMC_INT_LEAPFROG | HMC_INT_3STAGE | HMC_INT_4STAGE (all
+ * problems; Samplers.py:1524-1726; default leapfrog),
+ * "autotune" 1|0 (time the tile shapes on this set at first run; default 1),
  * "resid_rows" 64|128|256, "grad_waves" 4|8, "resid_lds" 0|1 (residual tiles
  * read into registers or staged through LDS; same results) (fix a shape, no
  * autotune); "premult" -1|0|1: trajectories from H = G^T C^-1 G and
@@ -410,7 +422,7 @@ int hmc_set_problem_prior(hmc_set* s, int64_t problem, int kind, const double* a
  * staged row tiles (128 for 64- and 128-row tiles; same results) (fix them,
  * no autotune). */
 int hmc_set_set_option(hmc_set* s, const char* name, double value);
-/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "lds_chunk", "autotune", "premult"
+/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "lds_chunk", "autotune", "integrator", "premult"
  * (the last is decided at the first run or proposal).  No reference
  * counterpart (engine configuration). */
 int hmc_set_get_option(hmc_set* s, const char* name, double* value);

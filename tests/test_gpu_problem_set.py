@@ -114,6 +114,149 @@ def test_teacher_forced_proposal_matches_oracle(Ls, form):
     assert near <= 4
 
 
+def _build_ext(seed, dims, ratios, mass_problems, bound_problems, noise=0.05, bound=0.3):
+    """Problems as in _build, with a Diagonal mass (diagonal uniform in
+    [0.5, 2]) on mass_problems and bounds [-bound, bound] on bound_problems;
+    the oracle holds the same mass matrices and bounds (BayesRule's collapsed
+    bounds, whose corrector reflects)."""
+    rng = np.random.default_rng(seed)
+    n = np.maximum(1, np.round(np.asarray(ratios) * np.asarray(dims))).astype(int)
+    ps = ProblemSet(dims, n)
+    orc, masses = [], []
+    for p, (d, nn) in enumerate(zip(dims, n)):
+        G = (rng.standard_normal((nn, d)) / np.sqrt(d)).astype(np.float32)
+        dd = G.astype(np.float64) @ rng.normal(size=d) * 0.1 + rng.normal(0, noise, size=nn)
+        mu = rng.normal(size=d) * 0.05
+        diag = rng.uniform(0.5, 2.0, size=d) if p in mass_problems else None
+        lo, hi = (np.full(d, -bound), np.full(d, bound)) if p in bound_problems else (None, None)
+        kind, scale = ("normal", 1.0) if p % 2 == 0 else ("laplace", 0.5)
+        ps.set_problem(p, G, dd, noise ** 2, prior=kind, means=mu, scale=scale, mass=diag, lower=lo, upper=hi)
+        prior = (O.Normal(mu.reshape(d, 1), 1.0) if kind == "normal"
+                 else O.Laplace(mu.reshape(d, 1), np.full((d, 1), 0.5)))
+        lik = O.LinearMatrixSimple(G.astype(np.float64), dd.reshape(nn, 1), noise ** 2, dtype=np.float64,
+                                   premultiplication=False)
+        orc.append(O.BayesRule([prior, lik], lower_bounds=None if lo is None else lo.reshape(d, 1),
+                               upper_bounds=None if hi is None else hi.reshape(d, 1)))
+        masses.append(O.Diagonal(diag) if diag is not None else O.Unit(d))
+    return ps, orc, masses, rng
+
+
+@pytest.mark.parametrize("form", ["residual", "premult_rows"])
+@pytest.mark.parametrize("integ", ["lf", "3s", "4s"])
+def test_integrators_mass_and_bounds_match_oracle(integ, form):
+    """The 3-stage and 4-stage schedules (Samplers.py:1586-1726), Diagonal mass
+    (MassMatrices.py:159-238: p = sqrt(M) z, K = p^T M^-1 p / 2, drift M^-1 p)
+    on problems 1, 3, 4 and bounds [-0.3, 0.3] with the reflection corrector
+    (base.py:239-270) on problems 1, 2, 5 against the oracle's proposal with
+    the same float32 momenta, local steps and uniforms.  Starts within 0.27 of
+    0 and trajectories of length ~0.05 make some coordinates reflect."""
+    ps, orc, masses, rng = _build_ext(7, DIMS, RATIOS, mass_problems={1, 3, 4}, bound_problems={1, 2, 5})
+    ps.set_option("premult", int(form != "residual"))
+    ps.set_option("premult_rows", 1)
+    Ls = [3, 7, 1, 5, 2, 4]
+    ps.set_state(ps.join([np.clip(rng.normal(size=(CHAINS, d)) * 0.15, -0.27, 0.27) for d in DIMS]))
+    X0, chi0 = ps.get_state()
+    P0 = ps.join([(rng.normal(size=(CHAINS, d)) * (np.sqrt(m.diagonal[:, 0]) if hasattr(m, "diagonal") else 1.0))
+                  .astype(np.float32).astype(np.float64) for d, m in zip(DIMS, masses)])
+    eps = 0.01 * rng.uniform(0.5, 1.5, size=ps.chains)
+    u = rng.uniform(size=ps.chains)
+    res = ps.propose(P0, eps, u, Ls, integrator=integ)
+    assert ps.get_option("integrator") == {"lf": 0, "3s": 1, "4s": 2}[integ]
+    xs, pz, x1 = ps.split(X0), ps.split(P0), ps.split(res["X1"])
+    near, reflected = 0, [0]
+
+    def counting(corrector):
+        def corr(x, p):
+            before = x.copy()
+            corrector(x, p)
+            reflected[0] += int(np.sum(before != x))
+        return corr
+
+    for p, o in enumerate(orc):
+        d = DIMS[p]
+        o.corrector = counting(o.corrector)
+        for c in range(CHAINS):
+            gc = p * CHAINS + c
+            ref = O.propose(o, masses[p], xs[p][c].reshape(d, 1), None, pz[p][c].reshape(d, 1), eps[gc], Ls[p],
+                            u[gc], integrator=integ)
+            assert _rel(chi0[gc], ref["chi0"]) < 1e-10, (p, c)
+            assert _rel(res["H0"][gc], ref["h0"]) < 1e-10, (p, c)
+            assert _rel(x1[p][c], ref["x1"][:, 0]) < 1e-4, (p, c)
+            assert _rel(res["H1"][gc], ref["h1"]) < 1e-4, (p, c)
+            if abs(np.log(u[gc]) - (ref["h0"] - ref["h1"])) < 1e-3:
+                near += 1
+            else:
+                assert bool(res["accepted"][gc]) == ref["accepted"], (p, c)
+    assert near <= 4
+    assert reflected[0] > 0          # the bounds were hit
+
+
+def test_bounds_give_infinite_misfit_and_gradient_outside():
+    """Outside its bounds a chain's misfit is inf (base.py:361-374) and so is
+    every component of its gradient (BayesRule adds the scalar bounds misfit,
+    base.py:1045-1054); chains inside keep the oracle's values."""
+    ps, orc, masses, rng = _build_ext(9, [40, 33], [1.5, 1.5], mass_problems=set(), bound_problems={0})
+    X = [np.clip(rng.normal(size=(CHAINS, d)) * 0.1, -0.25, 0.25) for d in (40, 33)]
+    X[0][3, 7] = 0.31        # chain 3 of problem 0 outside
+    X[0][5, 0] = -0.4        # chain 5 too
+    flat = ps.join(X)
+    chi = ps.misfit(flat)
+    g = ps.split(ps.gradient(flat))
+    for p in range(2):
+        for c in range(CHAINS):
+            x = X[p][c].astype(np.float32).astype(np.float64).reshape(-1, 1)
+            if p == 0 and c in (3, 5):
+                assert np.isinf(chi[c]) and np.all(np.isinf(g[0][c]))
+                assert np.isinf(orc[0].misfit(x))
+            else:
+                assert _rel(chi[p * CHAINS + c], orc[p].misfit(x)) < 1e-10
+                assert _rel(g[p][c], orc[p].gradient(x)[:, 0]) < 1e-4
+    with pytest.raises(ValueError, match="incompatible"):
+        ps2 = ProblemSet([4], [6])
+        ps2.set_problem(0, np.ones((6, 4), np.float32), np.zeros(6), 1.0, lower=np.ones(4), upper=np.zeros(4))
+
+
+@pytest.mark.parametrize("integ,mass", [("lf", False), ("4s", True)])
+def test_device_rng_multistage_diagonal_mass_posterior(integ, mass):
+    """Device-RNG chains of a Normal-prior problem with the 4-stage integrator
+    and a Diagonal mass (momenta sqrt(M) z from the Philox draws) keep the
+    analytic posterior (6400 correlated draws: mean error and sd ratio within
+    0.15 of the posterior sd)."""
+    dims = [20, 12]
+    rng = np.random.default_rng(13)
+    n = [30, 18]
+    ps = ProblemSet(dims, n)
+    Gs, ds = [], []
+    for p, (d, nn) in enumerate(zip(dims, n)):
+        G = (rng.standard_normal((nn, d)) / np.sqrt(d)).astype(np.float32)
+        dd = G.astype(np.float64) @ rng.normal(size=d) + rng.normal(0, 0.3, size=nn)
+        H = G.astype(np.float64).T @ G.astype(np.float64) / 0.09 + np.eye(d)
+        diag = np.diag(H).copy() if mass else None
+        ps.set_problem(p, G, dd, 0.09, prior="normal", scale=1.0, mass=diag)
+        Gs.append(G.astype(np.float64))
+        ds.append(dd)
+    G, dd = Gs[0], ds[0]
+    H = G.T @ G / 0.09 + np.eye(dims[0])
+    Sigma = np.linalg.inv(H)
+    mu = Sigma @ (G.T @ dd / 0.09)
+    ps.set_state(ps.join([np.zeros((CHAINS, d)) for d in dims]))
+    # the diagonal mass rescales the curvature: steps from the preconditioned spectrum
+    Mi = 1.0 / np.diag(H) if mass else np.ones(dims[0])
+    lam = np.linalg.eigvalsh(np.sqrt(Mi)[:, None] * H * np.sqrt(Mi)[None, :]).max()
+    eps = (1.5 if integ == "4s" else 0.5) / np.sqrt(lam)
+    step = np.full(ps.chains, eps)
+    ps.run(50, [10, 10], step, seed=3, integrator=integ)
+    ps.set_trace(400)
+    acc = ps.run(400, [10, 10], step, seed=3, first_proposal=50, integrator=integ)
+    assert acc[:CHAINS].mean() / 400 > 0.5
+    X, _ = ps.read_trace(0, 400)
+    flat = np.stack([ps.split(x)[0] for x in X]).reshape(-1, dims[0])
+    sd = np.sqrt(np.diag(Sigma))
+    err = np.abs(flat.mean(axis=0) - mu) / sd
+    assert np.sqrt(np.mean(err ** 2)) < 0.15
+    assert np.all(np.abs(flat.std(axis=0) / sd - 1) < 0.15)
+
+
 @pytest.mark.parametrize("premult", [0, 1])
 def test_graph_replays_match_direct_launches_and_tile_shapes(premult):
     """Graph replays against direct launches (bitwise), then other tile
