@@ -670,6 +670,76 @@ def run_tomography(args):
     return result
 
 
+def _jump_pilot(ps, args, L, d, step, done, K):
+    """Per problem, the smallest over coordinates of the mean squared jump
+    between consecutive draws over the draws' variance (1 - lag-1
+    autocorrelation, times 2), from K proposals of 16 chains, and the time
+    per proposal round."""
+    import torch
+
+    from hmclab_amd.problem_set import CHAINS
+
+    ps.set_trace(K)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ps.run(K, L, step, seed=args.seed + 17, first_proposal=done)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / K * 1e3
+    X, _ = ps.read_trace(0, K)
+    ps.set_trace(0)
+    J = np.zeros(ps.P)
+    for p in range(ps.P):
+        x = X[:, ps.offsets[p]:ps.offsets[p + 1]].reshape(K, CHAINS, int(d[p]))
+        jump = np.mean((x[1:] - x[:-1]) ** 2, axis=(0, 1))
+        var = np.var(x.reshape(-1, int(d[p])), axis=0) + 1e-300
+        J[p] = float(np.min(jump / var))
+    return J, ms, done + K
+
+
+def _adapt_uneven_integrators(ps, args, L, d, step, done, tune):
+    """Warm-up adaptation of each problem's integrator (each problem is its own
+    HMC.sample, so each may use its own, Samplers.py:1728-1738): the problems
+    whose leapfrog draws move least (pilot jump statistic) switch to the
+    4-stage integrator, whose step at the same acceptance is about twice as
+    long, so their fixed L steps cover a longer trajectory.  Fractions 0..0.4
+    of the problems are tried, each after re-tuning the steps, and the one
+    with the largest predicted min over problems of ESS per second (ESS per
+    draw ~ J / (4 - J) for lag-1 jump J, over the pilot's time per round) is
+    kept.  Everything here is warm-up, before the timed phase."""
+    from hmclab_amd.problem_set import CHAINS
+
+    J0, ms0, done = _jump_pilot(ps, args, L, d, step, done, args.adapt_pilot)
+    order = np.argsort(J0)
+    lf_step = step.copy()
+    best = None
+    scores = {}
+    for frac in (0.0, 0.1, 0.2, 0.3, 0.4):
+        k = int(round(frac * ps.P))
+        sel = order[:k]
+        ps.set_integrator("lf")
+        if k:
+            ps.set_integrator("4s", problems=sel)
+        mask = np.zeros(ps.P, bool)
+        mask[sel] = True
+        st = lf_step * np.repeat(np.where(mask, 2.0, 1.0), CHAINS)
+        if k:
+            st, done, _ = tune(st, args.adapt_tune_rounds, done)
+            J, ms, done = _jump_pilot(ps, args, L, d, st, done, args.adapt_pilot)
+        else:
+            J, ms = J0, ms0
+        score = float(np.min(J / (4.0 - np.minimum(J, 3.99)))) / (ms / 1e3)
+        scores[f"{frac:.1f}"] = round(score, 2)
+        if best is None or score > best[0]:
+            best = (score, k, sel.copy(), st.copy())
+    _, k, sel, st = best
+    ps.set_integrator("lf")
+    if k:
+        ps.set_integrator("4s", problems=sel)
+    return st, done, dict(problems_4s=int(k), scores=scores, rule="largest predicted min-ESS/s over fractions "
+                          "0..0.4 of the least-moving problems on the 4-stage integrator",
+                          L_of_4s_problems=sorted(int(x) for x in L[sel]), problems=sorted(int(x) for x in sel))
+
+
 def run_uneven(args):
     """256 independent dense problems of uneven size (d log-uniform in
     [50, 2000], n = 1.5 d, G float32), 16 chains each, leapfrog steps per
@@ -693,15 +763,24 @@ def run_uneven(args):
     log(f"{args.problems} problems (sum n*d = {int(np.sum(n * d)):,}) ready in {time.perf_counter() - t0:.1f}s")
     # tune each problem's step towards acceptance 0.65 (multiplicative, per problem)
     done = 0
-    for r in range(args.tune_rounds):
-        acc = ps.run(args.tune_props, L, step, seed=args.seed, first_proposal=done)
-        done += args.tune_props
-        rate = acc.reshape(-1, CHAINS).sum(axis=1) / (CHAINS * args.tune_props)
-        gain = 1.5 / (1.0 + r / 5.0)           # shrinking steps average out the acceptance noise
-        step *= np.repeat(np.exp(gain * (rate - 0.65)), CHAINS)
+
+    def tune(step, rounds, done):
+        for r in range(rounds):
+            acc = ps.run(args.tune_props, L, step, seed=args.seed, first_proposal=done)
+            done += args.tune_props
+            rate = acc.reshape(-1, CHAINS).sum(axis=1) / (CHAINS * args.tune_props)
+            gain = 1.5 / (1.0 + r / 5.0)           # shrinking steps average out the acceptance noise
+            step = step * np.repeat(np.exp(gain * (rate - 0.65)), CHAINS)
+        return step, done, rate
+
+    step, done, rate = tune(step, args.tune_rounds, done)
     log(f"tuned: acceptance per problem {rate.min():.2f}..{rate.max():.2f}; tiles {ps.tile_config()}")
     ps.run(args.warmup, L, step, seed=args.seed, first_proposal=done)
     done += args.warmup
+    adapt = None
+    if args.adapt_integrators:
+        step, done, adapt = _adapt_uneven_integrators(ps, args, L, d, step, done, tune)
+        log(f"adapted: {adapt['problems_4s']} problems on the 4-stage integrator (scores {adapt['scores']})")
     # kernel times from one directly launched round with events
     ps.enable_timing(True)
     ps.run(2, L, step, seed=args.seed, first_proposal=done, graph=False)
@@ -722,8 +801,10 @@ def run_uneven(args):
         ess_min.append(float(np.min(diagnostics.ess_bulk(draws))) if args.steps >= 4 else float("nan"))
     ess_min = np.array(ess_min)
     rates = acc.reshape(-1, CHAINS).sum(axis=1) / (CHAINS * args.steps)
-    # algorithmic bytes per round (one pass of G per step, x and p read+write)
-    bytes_round = float(np.sum(L * (4.0 * n * d + 256.0 * d)) + np.sum(4.0 * n * d))
+    # gradient stages of a round per problem: L, or 4L on the 4-stage integrator
+    S = L.astype(np.int64).copy()
+    if adapt:
+        S[adapt["problems"]] *= 4
     r_ms, r_n = kt[0]
     g_ms, g_n = kt[1]
     e_ms, e_n = kt[2]
@@ -732,15 +813,15 @@ def run_uneven(args):
     tiles = ps.tile_config()
     if tiles["premult"]:
         # one pass over H (d x d fp32) per step; x and p read and written
-        traj_bytes = float(np.sum(L * (4.0 * d * d + 256.0 * d)))
+        traj_bytes = float(np.sum(S * (4.0 * d * d + 256.0 * d)))
         actual = traj_bytes / (step_ms / 1e3)
         kernel = ((f"k_set_resid_lds<{tiles['resid_rows'] // 64},1,{tiles['lds_chunk']}> (rows of H)"
                    if tiles["premult_rows"] else f"k_set_grad<{tiles['grad_waves']},2> (column strips of H)")
                   + ": g = H x - b, kick/drift; all trajectory launches of one round")
     else:
         # the algorithm needs one pass of G per step; the residual and gradient launches make two
-        traj_bytes = float(np.sum(L * (4.0 * n * d + 256.0 * d)))
-        actual = float(np.sum(L * (8.0 * n * d + 256.0 * d + 2 * 64.0 * n))) / (step_ms / 1e3)
+        traj_bytes = float(np.sum(S * (4.0 * n * d + 256.0 * d)))
+        actual = float(np.sum(S * (8.0 * n * d + 256.0 * d + 2 * 64.0 * n))) / (step_ms / 1e3)
         kernel = "k_set_resid + k_set_grad (all trajectory launches of one round)"
     achieved = traj_bytes / (step_ms / 1e3)
     result = {
@@ -749,12 +830,19 @@ def run_uneven(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.problems} uneven dense problems (d in [50,2000] log-uniform, n=1.5d, "
-                               f"G fp32), {CHAINS} chains each, L in [5,30]; fp32 MFMA trajectories "
+                               f"G fp32), {CHAINS} chains each, L in [5,30] (leapfrog; the 4-stage integrator "
+                               f"for the least-moving problems when the warm-up adaptation picks it); "
+                               f"fp32 MFMA trajectories "
                                f"({'premultiplied H = G^T C^-1 G' if tiles['premult'] else 'residual form'}), "
                                f"fp64 residual-form endpoint; one hipGraph replay per proposal round",
                    "problems": args.problems, "chains": args.problems * CHAINS,
                    "sum_nd": int(np.sum(n * d)), "value_is": "min over problems of min-ESS, per second",
                    "median_problem_min_ess_per_s": float(np.median(ess_min)) / elapsed,
+                   "integrator_adaptation": adapt,
+                   "worst_problems": [{"problem": int(q), "d": int(d[q]), "n": int(n[q]), "L": int(L[q]),
+                                       "prior": specs[q]["prior"], "acceptance": float(rates[q]),
+                                       "min_ess_per_s": float(ess_min[q]) / elapsed}
+                                      for q in np.argsort(ess_min)[:6]],
                    "acceptance_min": float(rates.min()), "acceptance_median": float(np.median(rates)),
                    "rounds_per_s": args.steps / elapsed, "tile_config": tiles,
                    "kernel_ms_per_round": {"resid": r_ms / 2, "grad": g_ms / 2, "endpoint": e_ms / 2,
@@ -848,6 +936,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small", "uneven", "copy"])
     ap.add_argument("--problems", type=int, default=256)
+    ap.add_argument("--adapt-integrators", type=int, default=1, choices=[0, 1],
+                    help="uneven batch: warm-up choice of the 4-stage integrator for the least-moving problems")
+    ap.add_argument("--adapt-pilot", type=int, default=30, help="proposals per adaptation pilot")
+    ap.add_argument("--adapt-tune-rounds", type=int, default=10)
     ap.add_argument("--prior", default="normal", choices=["normal", "laplace"],
                     help="dense workload prior: Normal(0,1) (analytic posterior) or Laplace(0,0.3)")
     ap.add_argument("--steps", type=int, default=100)

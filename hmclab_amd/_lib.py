@@ -111,6 +111,7 @@ SIGNATURES = {
     "hmc_set_problem_prior": [_p, _i64, _i32, _dp, _dp],
     "hmc_set_problem_mass": [_p, _i64, _i32, _dp],
     "hmc_set_problem_bounds": [_p, _i64, _dp, _dp],
+    "hmc_set_problem_integrator": [_p, _i64, _i32],
     "hmc_set_set_option": [_p, C.c_char_p, C.c_double],
     "hmc_set_tile_config": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "hmc_set_get_option": [_p, C.c_char_p, _dp],

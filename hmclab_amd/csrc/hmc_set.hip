@@ -59,6 +59,7 @@ struct SetDesc {
   int32_t ptile0, n_ptiles;                           // endpoint tiles of this problem
   int32_t S, c_off;       // gradient stages of a trajectory; first stage's coefficients
   int32_t mass, bounded;  // diagonal mass; bounds (reflection corrector)
+  double fd;              // first drift coefficient of the problem's integrator
 };
 
 // Per-set model arrays beside the descriptors: stage coefficients {kick,
@@ -612,8 +613,7 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
                                                     double* __restrict__ k0, double* __restrict__ eps,
                                                     double* __restrict__ u,
                                                     const double* __restrict__ stepv,
-                                                    const SetRun* __restrict__ run, double drift0,
-                                                    SetModel md) {
+                                                    const SetRun* __restrict__ run, SetModel md) {
   __shared__ double scratch[SW / 64 + 1];
   const int64_t gc = blockIdx.x;
   const SetDesc D = desc[gc / SC];
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(SW) void k_set_refresh(const SetDesc* __restrict__ 
   } else {
     e = eps[gc];
   }
-  const float dd = (float)(drift0 * e);
+  const float dd = (float)(D.fd * e);
   double ks = 0.0;
   const int64_t base = D.x_off + (int64_t)c * D.ldg;
   // premultiplied trajectories start in buffer S mod 2 (their last stage writes XT)
@@ -783,6 +783,7 @@ struct SetProblemHost {
   bool has_data = false;
   int prior_kind = 0;
   bool mass = false, bounded = false;
+  int integ = HMC_INT_LEAPFROG;
 };
 
 struct ProblemSet {
@@ -819,10 +820,8 @@ struct ProblemSet {
   // per step s: tiles of the problems with L > s at [off[s], off[s] + count[s])
   std::vector<int64_t> roff, rprefix, goff, gprefix, hoff, hprefix;
   int64_t n_ptiles = 0;
-  // integrator (HMC_INT_*), its first drift, and per problem the stage count
-  // of a trajectory (L, 3L or 4L); Smax the largest
-  int integ = HMC_INT_LEAPFROG;
-  double fd = 0.5;
+  // per problem (in hp) the integrator (HMC_INT_*); per problem the stage
+  // count of a trajectory (L, 3L or 4L); Smax the largest
   int Smax = 0;
   std::vector<int32_t> Lcur, Scur;
   double* coef = nullptr;           // stage coefficients (in tile_mem)
@@ -902,7 +901,7 @@ static void build_tiles(ProblemSet& s) {
   s.Scur.assign(s.P, 0);
   s.Smax = 0;
   for (int64_t p = 0; p < s.P; ++p) {
-    integrator_schedule(s.integ, s.Lcur[p], s.fd, sched);
+    integrator_schedule(s.hp[p].integ, s.Lcur[p], s.hd[p].fd, sched);
     s.hd[p].c_off = (int32_t)(coef.size() / 3);
     s.hd[p].S = (int32_t)sched.size();
     s.Scur[p] = (int32_t)sched.size();
@@ -1068,7 +1067,7 @@ static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
   hipEvent_t e;
   set_time(s, 3, true, e);
   hipLaunchKernelGGL(k_set_refresh, dim3((unsigned)B), dim3(SW), 0, s.stream, s.desc, s.x0, s.xt,
-                     s.pm ? s.xt2 : nullptr, s.pt, s.p_host, s.k0, s.eps, s.u, s.step, s.run, s.fd, s.model());
+                     s.pm ? s.xt2 : nullptr, s.pt, s.p_host, s.k0, s.eps, s.u, s.step, s.run, s.model());
   HMC_HIP(hipGetLastError());
   set_time(s, 3, false, e);
   for (int st = 0; st < s.Smax; ++st) {
@@ -1430,6 +1429,23 @@ int hmc_set_problem_mass(hmc_set* h, int64_t i, int kind, const double* diag) {
   });
 }
 
+// Integrator of problem i (HMC_INT_*; Samplers.py:1524-1726): each problem is
+// its own HMC.sample, so each may have its own; the set option "integrator"
+// sets all of them.
+int hmc_set_problem_integrator(hmc_set* h, int64_t i, int integrator) {
+  return guarded("hmc_set_problem_integrator", [&] {
+    HMC_REQUIRE(h, HMC_ERR_ARG, "NULL argument");
+    ProblemSet& s = h->s;
+    HMC_REQUIRE(i >= 0 && i < s.P, HMC_ERR_ARG, "problem index out of range");
+    HMC_REQUIRE(integrator == HMC_INT_LEAPFROG || integrator == HMC_INT_3STAGE || integrator == HMC_INT_4STAGE,
+                HMC_ERR_ARG, "integrator must be 0 (lf), 1 (3s) or 2 (4s)");
+    if (s.hp[i].integ != integrator) {
+      s.hp[i].integ = integrator;
+      if (s.rtiles) build_tiles(s);
+    }
+  });
+}
+
 // Bounds of problem i (the BayesRule's collapsed bounds, base.py:1061-1101):
 // lower / upper [d] or NULL (no bound on that side); lower < upper
 // elementwise (base.py:349-359).  Both NULL removes the bounds.
@@ -1478,7 +1494,7 @@ int hmc_set_set_option(hmc_set* h, const char* name, double value) {
     if (k == "integrator") {
       HMC_REQUIRE(value == HMC_INT_LEAPFROG || value == HMC_INT_3STAGE || value == HMC_INT_4STAGE, HMC_ERR_ARG,
                   "integrator must be 0 (lf), 1 (3s) or 2 (4s)");
-      s.integ = (int)value;
+      for (auto& q : s.hp) q.integ = (int)value;
     } else if (k == "autotune") {
       s.autotune = value != 0.0;
     } else if (k == "resid_rows") {
@@ -1525,7 +1541,12 @@ int hmc_set_get_option(hmc_set* h, const char* name, double* value) {
     else if (k == "lds_chunk") *value = s.kc;
     else if (k == "autotune") *value = s.autotune;
     else if (k == "premult") *value = s.pm ? 1.0 : 0.0;
-    else if (k == "integrator") *value = s.integ;
+    else if (k == "integrator") {
+      // the common integrator, or -1 when the problems differ
+      *value = s.hp[0].integ;
+      for (const auto& q : s.hp)
+        if (q.integ != s.hp[0].integ) *value = -1.0;
+    }
     else throw Error{HMC_ERR_ARG, "unknown option '" + k + "'"};
   });
 }

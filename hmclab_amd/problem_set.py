@@ -140,11 +140,16 @@ class ProblemSet:
     def _L(self, L):
         return np.ascontiguousarray(np.broadcast_to(np.asarray(L, dtype=np.int32), (self.P,)))
 
-    def set_integrator(self, integrator: str):
-        """"lf", "3s" or "4s" (Samplers.py:1524-1726), for every problem."""
+    def set_integrator(self, integrator: str, problems=None):
+        """"lf", "3s" or "4s" (Samplers.py:1524-1726), for every problem or for
+        the given problem indices."""
         if integrator not in _lib.INTEGRATORS:
             raise ValueError("integrator must be 'lf', '3s' or '4s'")
-        self.set_option("integrator", _lib.INTEGRATORS[integrator])
+        if problems is None:
+            self.set_option("integrator", _lib.INTEGRATORS[integrator])
+        else:
+            for p in np.atleast_1d(problems):
+                call("hmc_set_problem_integrator", self._h, int(p), _lib.INTEGRATORS[integrator])
 
     def propose(self, P0, eps, u, L, integrator=None):
         if integrator is not None:

@@ -402,6 +402,9 @@ int hmc_set_problem_prior(hmc_set* s, int64_t problem, int kind, const double* a
 /* Diagonal mass (MassMatrices.py:159-238): kind HMC_MASS_DIAGONAL with
  * diagonal[d] > 0, or HMC_MASS_UNIT (diagonal ignored; the default). */
 int hmc_set_problem_mass(hmc_set* s, int64_t problem, int kind, const double* diagonal);
+/* Integrator of one problem (HMC_INT_*; Samplers.py:1524-1726; default
+ * leapfrog); the option "integrator" sets every problem's. */
+int hmc_set_problem_integrator(hmc_set* s, int64_t problem, int integrator);
 /* Collapsed bounds of the problem's target (base.py:1061-1101): lower and
  * upper [d] or NULL for no bound on that side; lower < upper elementwise
  * ("Bounds vectors are incompatible.", base.py:359).  Trajectories reflect at
@@ -422,7 +425,8 @@ int hmc_set_problem_bounds(hmc_set* s, int64_t problem, const double* lower, con
  * staged row tiles (128 for 64- and 128-row tiles; same results) (fix them,
  * no autotune). */
 int hmc_set_set_option(hmc_set* s, const char* name, double value);
-/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "lds_chunk", "autotune", "integrator", "premult"
+/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "lds_chunk", "autotune", "integrator" (-1 when
+ * the problems' integrators differ), "premult"
  * (the last is decided at the first run or proposal).  No reference
  * counterpart (engine configuration). */
 int hmc_set_get_option(hmc_set* s, const char* name, double* value);

@@ -142,7 +142,7 @@ def _build_ext(seed, dims, ratios, mass_problems, bound_problems, noise=0.05, bo
 
 
 @pytest.mark.parametrize("form", ["residual", "premult_rows"])
-@pytest.mark.parametrize("integ", ["lf", "3s", "4s"])
+@pytest.mark.parametrize("integ", ["lf", "3s", "4s", "mixed"])
 def test_integrators_mass_and_bounds_match_oracle(integ, form):
     """The 3-stage and 4-stage schedules (Samplers.py:1586-1726), Diagonal mass
     (MassMatrices.py:159-238: p = sqrt(M) z, K = p^T M^-1 p / 2, drift M^-1 p)
@@ -160,8 +160,16 @@ def test_integrators_mass_and_bounds_match_oracle(integ, form):
                   .astype(np.float32).astype(np.float64) for d, m in zip(DIMS, masses)])
     eps = 0.01 * rng.uniform(0.5, 1.5, size=ps.chains)
     u = rng.uniform(size=ps.chains)
-    res = ps.propose(P0, eps, u, Ls, integrator=integ)
-    assert ps.get_option("integrator") == {"lf": 0, "3s": 1, "4s": 2}[integ]
+    if integ == "mixed":        # one integrator per problem (each problem is its own HMC.sample)
+        per = ["lf", "3s", "4s", "4s", "lf", "3s"]
+        for p, k in enumerate(per):
+            ps.set_integrator(k, problems=[p])
+        res = ps.propose(P0, eps, u, Ls)
+        assert ps.get_option("integrator") == -1
+    else:
+        per = [integ] * len(DIMS)
+        res = ps.propose(P0, eps, u, Ls, integrator=integ)
+        assert ps.get_option("integrator") == {"lf": 0, "3s": 1, "4s": 2}[integ]
     xs, pz, x1 = ps.split(X0), ps.split(P0), ps.split(res["X1"])
     near, reflected = 0, [0]
 
@@ -178,7 +186,7 @@ def test_integrators_mass_and_bounds_match_oracle(integ, form):
         for c in range(CHAINS):
             gc = p * CHAINS + c
             ref = O.propose(o, masses[p], xs[p][c].reshape(d, 1), None, pz[p][c].reshape(d, 1), eps[gc], Ls[p],
-                            u[gc], integrator=integ)
+                            u[gc], integrator=per[p])
             assert _rel(chi0[gc], ref["chi0"]) < 1e-10, (p, c)
             assert _rel(res["H0"][gc], ref["h0"]) < 1e-10, (p, c)
             assert _rel(x1[p][c], ref["x1"][:, 0]) < 1e-4, (p, c)
