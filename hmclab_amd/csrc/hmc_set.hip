@@ -60,6 +60,7 @@ struct SetDesc {
   int32_t S, c_off;       // gradient stages of a trajectory; first stage's coefficients
   int32_t mass, bounded;  // diagonal mass; bounds (reflection corrector)
   double fd;              // first drift coefficient of the problem's integrator
+  int32_t so, pad_;       // launch of the round that runs the trajectory's first stage
 };
 
 // Per-set model arrays beside the descriptors: stage coefficients {kick,
@@ -264,7 +265,8 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
   const int nr = KICK ? D.d : D.n;               // rows of the operator
   const int nrow = (nr + 15) & ~15;              // rows present in memory (zero padded)
   const float* Gp = G + (KICK ? D.hm_off : D.g_off);
-  const bool flip = KICK && ((D.S - step) & 1);
+  const int ls = step - D.so;   // the problem's own stage (KICK)
+  const bool flip = KICK && ((D.S - ls) & 1);
   const float* Xin = flip ? X2 : X;
   const float* Xp = Xin + D.x_off;
   // loader mapping: thread tid covers column 4 * (tid % LPR) of rows tid / LPR + RPP j
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
   }
   if (KICK) {
     // lane (i, q) holds parameters row0 .. row0 + 3 of chain i
-    if (step >= D.S) return;
+    if (ls < 0 || ls >= D.S) return;
     const double ec = eps[(int64_t)t.x * SC + i];
     float* Xout = flip ? X : X2;
 #pragma unroll
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(256) void k_set_resid_lds(const SetDesc* __restrict
         const float gv = (acc[r][v] - (float)hb[D.a_off + j]) +
                          set_prior_grad(D.prior_kind, pa[D.a_off + j], pb[D.a_off + j], xv0);
         float xv = xv0, pv = P[o];
-        set_leap(md, D, D.a_off + j, step, ec, gv, xv, pv);
+        set_leap(md, D, D.a_off + j, ls, ec, gv, xv, pv);
         Xout[o] = xv;
         P[o] = pv;
       }
@@ -392,7 +394,8 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
   floatx4 acc[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const bool flip = MODE == 2 && ((D.S - step) & 1);
+  const int ls = step - D.so;   // the problem's own stage
+  const bool flip = MODE == 2 && ((D.S - ls) & 1);
   const float* Xin = flip ? X2 : X;
   float* Xout = MODE == 2 ? (flip ? X : X2) : X;
   const float* wrow = MODE == 2 ? Xin + D.x_off + (int64_t)i * D.ldg + 4 * q
@@ -452,9 +455,9 @@ __global__ __launch_bounds__(NW * 64) void k_set_grad(const SetDesc* __restrict_
       // bounds (base.py:1045-1054)
       gout[D.h_off + (int64_t)c * D.d + j] = oob[(int64_t)t.x * SC + c] ? INFINITY : (double)gv;
     } else {
-      if (step >= D.S) continue;
+      if (ls < 0 || ls >= D.S) continue;
       float xv = xv0, pv = P[o];
-      set_leap(md, D, D.a_off + j, step, eps[(int64_t)t.x * SC + c], gv, xv, pv);
+      set_leap(md, D, D.a_off + j, ls, eps[(int64_t)t.x * SC + c], gv, xv, pv);
       Xout[o] = xv;
       P[o] = pv;
     }
@@ -823,7 +826,11 @@ struct ProblemSet {
   // per problem (in hp) the integrator (HMC_INT_*); per problem the stage
   // count of a trajectory (L, 3L or 4L); Smax the largest
   int Smax = 0;
-  std::vector<int32_t> Lcur, Scur;
+  std::vector<int32_t> Lcur, Scur, Soff;
+  // stage windows of the problems staggered over the round's launches to
+  // balance their bytes (option "stagger"); off while the tile shapes are timed
+  int stagger = 1;
+  bool tuning = false;
   double* coef = nullptr;           // stage coefficients (in tile_mem)
   double *minv = nullptr, *msqrt = nullptr, *lo = nullptr, *hi = nullptr;
   SetModel model() const { return SetModel{coef, minv, msqrt, lo, hi}; }
@@ -914,6 +921,40 @@ static void build_tiles(ProblemSet& s) {
   }
   s.coef = (double*)dev_alloc(s.tile_mem, std::max<size_t>(coef.size(), 3) * 8);
   h2d(s.coef, coef.data(), coef.size() * 8);
+  // stage windows [Soff_p, Soff_p + S_p) within the Smax launches of a round:
+  // problems by decreasing trajectory bytes, each where the heaviest launch
+  // of its window stays lightest (then the least loaded window, then the
+  // earliest); the longest trajectories have no choice
+  s.Soff.assign(s.P, 0);
+  if (s.stagger && !s.tuning) {
+    std::vector<double> w(s.P), load(s.Smax, 0.0);
+    for (int64_t p = 0; p < s.P; ++p)
+      w[p] = s.pm ? (double)s.hd[p].ldg * s.hd[p].ldg : 2.0 * (double)round_up(s.hp[p].n, 16) * s.hd[p].ldg;
+    std::vector<int32_t> ord(s.P);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int32_t a, int32_t b) { return w[a] * s.Scur[a] > w[b] * s.Scur[b]; });
+    for (int32_t p : ord) {
+      const int S = s.Scur[p];
+      int best = 0;
+      double best_max = 1e300, best_sum = 1e300;
+      for (int o = 0; o + S <= s.Smax; ++o) {
+        double mx = 0.0, sm = 0.0;
+        for (int k = o; k < o + S; ++k) {
+          mx = std::max(mx, load[k] + w[p]);
+          sm += load[k];
+        }
+        if (mx < best_max || (mx == best_max && sm < best_sum)) {
+          best_max = mx;
+          best_sum = sm;
+          best = o;
+        }
+      }
+      s.Soff[p] = best;
+      for (int k = best; k < best + S; ++k) load[k] += w[p];
+    }
+  }
+  for (int64_t p = 0; p < s.P; ++p) s.hd[p].so = s.Soff[p];
   std::vector<int2> rt, gt, ht;
   const int rows = 64 * s.rw;
   auto build = [&](std::vector<int2>& out, std::vector<int64_t>& off, std::vector<int64_t>& cnt,
@@ -921,12 +962,13 @@ static void build_tiles(ProblemSet& s) {
     std::vector<int32_t> ord(s.P);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return cost(a) > cost(b); });
-    off.assign(s.Smax, 0);
-    cnt.assign(s.Smax, 0);
-    for (int st = 0; st < s.Smax; ++st) {
+    // list st < Smax: launch st of a round; list Smax: every problem
+    off.assign(s.Smax + 1, 0);
+    cnt.assign(s.Smax + 1, 0);
+    for (int st = 0; st <= s.Smax; ++st) {
       off[st] = (int64_t)out.size();
       for (int32_t p : ord) {
-        if (s.Scur[p] <= st) continue;
+        if (st < s.Smax && (st < s.Soff[p] || st >= s.Soff[p] + s.Scur[p])) continue;
         const int64_t nb = ntiles(p);
         for (int64_t b = 0; b < nb; ++b) out.push_back(make_int2(p, (int)b));
       }
@@ -956,7 +998,7 @@ static void build_tiles(ProblemSet& s) {
   drop_graph(s);
 }
 
-static void launch_grad(ProblemSet& s, float* X, int step, int mode);
+static void launch_grad(ProblemSet& s, float* X, int list, int step, int mode);
 
 template <int RW, int KICK, int KC>
 static void launch_rows_kc(ProblemSet& s, int64_t ntiles, const int2* tiles, const float* Gop, float* X,
@@ -976,9 +1018,10 @@ static void launch_rows_lds(ProblemSet& s, int64_t ntiles, const int2* tiles, co
     launch_rows_kc<RW, KICK, 64>(s, ntiles, tiles, Gop, X, step);
 }
 
-static void launch_resid(ProblemSet& s, int step, float* X) {
-  const int64_t ntiles = s.rprefix[step];
-  const int2* tiles = s.rtiles + s.roff[step];
+// tile list `list` (launch of a round, or s.Smax for every problem)
+static void launch_resid(ProblemSet& s, int list, float* X) {
+  const int64_t ntiles = s.rprefix[list];
+  const int2* tiles = s.rtiles + s.roff[list];
   if (ntiles <= 0) return;
   hipEvent_t e;
   set_time(s, 0, true, e);
@@ -1005,7 +1048,7 @@ static void launch_resid(ProblemSet& s, int step, float* X) {
 // or rows of H (k_set_resid_lds with the kick epilogue), 64 * rw rows a tile.
 static void launch_premult_step(ProblemSet& s, int step) {
   if (!s.hrow) {
-    launch_grad(s, s.xt, step, 2);
+    launch_grad(s, s.xt, step, step, 2);
     return;
   }
   const int64_t ntiles = s.hprefix[step];
@@ -1023,9 +1066,9 @@ static void launch_premult_step(ProblemSet& s, int step) {
   set_time(s, 1, false, e);
 }
 
-static void launch_grad(ProblemSet& s, float* X, int step, int mode) {
-  const int64_t ntiles = s.gprefix[step];
-  const int2* tiles = s.gtiles + s.goff[step];
+static void launch_grad(ProblemSet& s, float* X, int list, int step, int mode) {
+  const int64_t ntiles = s.gprefix[list];
+  const int2* tiles = s.gtiles + s.goff[list];
   if (ntiles <= 0) return;
   hipEvent_t e;
   set_time(s, 1, true, e);
@@ -1075,7 +1118,7 @@ static void enqueue_proposal(ProblemSet& s, bool want_energies, bool trace) {
       launch_premult_step(s, st);
     } else {
       launch_resid(s, st, s.xt);
-      launch_grad(s, s.xt, st, 0);
+      launch_grad(s, s.xt, st, st, 0);
     }
   }
   launch_endpoint(s, s.xt);
@@ -1166,8 +1209,8 @@ static void ensure_premult(ProblemSet& s) {
                        (const int2*)tl, s.G, s.ivar, s.H);
     HMC_HIP(hipGetLastError());
     // every problem's 64-column tiles (all L >= 1, so the step-0 prefix is the whole list)
-    hipLaunchKernelGGL(k_set_build_b, dim3((unsigned)s.gprefix[0]), dim3(256), 0, s.stream, s.desc,
-                       s.gtiles + s.goff[0],
+    hipLaunchKernelGGL(k_set_build_b, dim3((unsigned)s.gprefix[s.Smax]), dim3(256), 0, s.stream, s.desc,
+                       s.gtiles + s.goff[s.Smax],
                        s.G, s.dobs, s.ivar, s.hb);
     HMC_HIP(hipGetLastError());
     HMC_HIP(hipStreamSynchronize(s.stream));
@@ -1184,6 +1227,7 @@ static void autotune_tiles(ProblemSet& s) {
   if (s.tuned || !s.autotune) return;
   const bool saved = s.timing;
   s.timing = false;
+  s.tuning = true;   // unstaggered windows: launch 0 covers every problem
   std::vector<float> xs;   // scratch states: use xt/pt copies of x0
   HMC_HIP(hipMemcpyAsync(s.xt, s.x0, s.x_total * 4, hipMemcpyDeviceToDevice, s.stream));
   HMC_HIP(hipMemsetAsync(s.pt, 0, s.x_total * 4, s.stream));
@@ -1223,7 +1267,7 @@ static void autotune_tiles(ProblemSet& s) {
             launch_premult_step(s, 0);
           } else {
             launch_resid(s, 0, s.xt);
-            launch_grad(s, s.xt, 0, 0);
+            launch_grad(s, s.xt, 0, 0, 0);
           }
         }
         HMC_HIP(hipEventRecord(b, s.stream));
@@ -1247,6 +1291,7 @@ static void autotune_tiles(ProblemSet& s) {
   s.kc = bkc;
   s.rw = brw;
   s.nw = bnw;
+  s.tuning = false;
   build_tiles(s);
   s.tuned = true;
   s.timing = saved;
@@ -1495,6 +1540,9 @@ int hmc_set_set_option(hmc_set* h, const char* name, double value) {
       HMC_REQUIRE(value == HMC_INT_LEAPFROG || value == HMC_INT_3STAGE || value == HMC_INT_4STAGE, HMC_ERR_ARG,
                   "integrator must be 0 (lf), 1 (3s) or 2 (4s)");
       for (auto& q : s.hp) q.integ = (int)value;
+    } else if (k == "stagger") {
+      HMC_REQUIRE(value == 0 || value == 1, HMC_ERR_ARG, "stagger must be 0 or 1");
+      s.stagger = (int)value;
     } else if (k == "autotune") {
       s.autotune = value != 0.0;
     } else if (k == "resid_rows") {
@@ -1540,6 +1588,7 @@ int hmc_set_get_option(hmc_set* h, const char* name, double* value) {
     else if (k == "premult_rows") *value = s.hrow;
     else if (k == "lds_chunk") *value = s.kc;
     else if (k == "autotune") *value = s.autotune;
+    else if (k == "stagger") *value = s.stagger;
     else if (k == "premult") *value = s.pm ? 1.0 : 0.0;
     else if (k == "integrator") {
       // the common integrator, or -1 when the problems differ
@@ -1627,8 +1676,8 @@ int hmc_set_gradient(hmc_set* h, const double* X, double* out) {
     hipLaunchKernelGGL(k_set_oob, dim3((unsigned)s.chains()), dim3(SW), 0, s.stream, s.desc, s.xt, s.acc,
                        s.model());
     HMC_HIP(hipGetLastError());
-    launch_resid(s, 0, s.xt);
-    launch_grad(s, s.xt, 0, 1);
+    launch_resid(s, s.Smax, s.xt);
+    launch_grad(s, s.xt, s.Smax, 0, 1);
     HMC_HIP(hipMemcpyAsync(out, s.gout, s.h_total * 8, hipMemcpyDeviceToHost, s.stream));
     HMC_HIP(hipStreamSynchronize(s.stream));
   });

@@ -412,6 +412,9 @@ int hmc_set_problem_integrator(hmc_set* s, int64_t problem, int integrator);
 int hmc_set_problem_bounds(hmc_set* s, int64_t problem, const double* lower, const double* upper);
 /* "integrator" HMC_INT_LEAPFROG | HMC_INT_3STAGE | HMC_INT_4STAGE (all
  * problems; Samplers.py:1524-1726; default leapfrog),
+ * "stagger" 1|0 (default 1: each problem's stage window starts at the launch of
+ * the round that best balances the launches' bytes; 0: every trajectory starts
+ * at the first launch; same results),
  * "autotune" 1|0 (time the tile shapes on this set at first run; default 1),
  * "resid_rows" 64|128|256, "grad_waves" 4|8, "resid_lds" 0|1 (residual tiles
  * read into registers or staged through LDS; same results) (fix a shape, no
@@ -425,7 +428,7 @@ int hmc_set_problem_bounds(hmc_set* s, int64_t problem, const double* lower, con
  * staged row tiles (128 for 64- and 128-row tiles; same results) (fix them,
  * no autotune). */
 int hmc_set_set_option(hmc_set* s, const char* name, double value);
-/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "lds_chunk", "autotune", "integrator" (-1 when
+/* Resolved "resid_rows", "grad_waves", "resid_lds", "premult_rows", "lds_chunk", "autotune", "stagger", "integrator" (-1 when
  * the problems' integrators differ), "premult"
  * (the last is decided at the first run or proposal).  No reference
  * counterpart (engine configuration). */

@@ -266,6 +266,30 @@ def test_device_rng_multistage_diagonal_mass_posterior(integ, mass):
 
 
 @pytest.mark.parametrize("premult", [0, 1])
+def test_staggered_stage_windows_are_bitwise_identical(premult):
+    """Problems' stage windows staggered over a round's launches (to balance
+    the launches' bytes) against every trajectory starting at launch 0: the
+    same draws bit for bit, with mixed integrators and trajectory lengths."""
+    ps, orc, rng = _build(12, DIMS, RATIOS)
+    ps.set_option("premult", premult)
+    ps.set_integrator("4s", problems=[2])
+    ps.set_integrator("3s", problems=[4])
+    X0 = ps.join([rng.normal(size=(CHAINS, d)) * 0.2 for d in DIMS])
+    Ls = [3, 12, 1, 7, 5, 9]
+    step = np.full(ps.chains, 0.01)
+    out = {}
+    for stagger in (0, 1):
+        ps.set_option("stagger", stagger)
+        assert ps.get_option("stagger") == stagger
+        ps.set_state(X0)
+        ps.set_trace(3)
+        acc = ps.run(3, Ls, step, seed=5)
+        out[stagger] = ps.read_trace(0, 3) + (acc,)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("premult", [0, 1])
 def test_graph_replays_match_direct_launches_and_tile_shapes(premult):
     """Graph replays against direct launches (bitwise), then other tile
     shapes: rows per tile and waves per strip, and for the premultiplied form
