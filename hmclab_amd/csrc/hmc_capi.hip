@@ -208,6 +208,21 @@ void time_collect(Chains& c) {
   c.ev_next = 0;
 }
 
+// After a synchronised call: a tile of the two-k-range fix-up whose partner
+// did not publish its partial within the kernel's bounded wait sets the flag;
+// that call's trajectories are then invalid.
+static void check_fixup(Chains& c) {
+  if (!c.fix_used) return;
+  int flag = 0;
+  d2h(&flag, c.fix, 4);
+  if (flag) {
+    const int zero = 0;
+    h2d(c.fix, &zero, 4);
+    throw Error{HMC_ERR_HIP, "bf16 GEMM split fix-up: a tile's partial was not published within the bounded "
+                             "wait; the results of this call are invalid"};
+  }
+}
+
 // ---- proposal drivers ----------------------------------------------------------
 static void gemm_proposal(Chains& c, bool host_p, int L, int integ, uint64_t seed, int64_t prop,
                           int randomize, int autotune, double target, double lr, int64_t slot,
@@ -490,6 +505,9 @@ int hmc_problem_set_option(hmc_problem* hp, const char* name, double value) {
       HMC_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0 || value == 4.0, HMC_ERR_ARG,
                   "gemm_config_grad must be 0, 1, 2, 3 or 4");
       P.gemm_cfg_grad = (int)value;
+    } else if (key == "split_fixup") {
+      HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "split_fixup must be 0 or 1");
+      P.split_fixup = (int)value;
     } else if (key == "gemm_split_tail") {
       HMC_REQUIRE(value == 0.0 || value == 1.0, HMC_ERR_ARG, "gemm_split_tail must be 0 or 1");
       P.split_tail = (int)value;
@@ -593,6 +611,7 @@ int hmc_chains_create(hmc_problem* hp, int64_t B, hmc_chains** out) {
           c.rh = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
           c.split_ws_elems = gemm_split_workspace(P, c.B);
           if (c.split_ws_elems) c.split_ws = (float*)dev_alloc(c.owned, c.split_ws_elems * 4);
+          if (c.split_ws_elems) c.fix = (int*)dev_alloc(c.owned, (size_t)(1 + kFixTiles) * 4);   // zeroed
           c.rl = (uint16_t*)dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * 2);
         } else if (P.resolved_engine() == HMC_ENGINE_GEMM) {
           c.r = dev_alloc(c.owned, (size_t)c.B_pad * c.ldr * es);
@@ -842,6 +861,7 @@ int hmc_chains_propose(hmc_chains* hc, const double* P0, const double* eps, cons
     }
     HMC_HIP(hipStreamSynchronize(P.stream));
     time_collect(c);
+    check_fixup(c);
   });
 }
 
@@ -885,6 +905,7 @@ int hmc_chains_run(hmc_chains* hc, int64_t proposals, int L, int integrator, dou
       HMC_HIP(hipMemcpyAsync(accepted_total, c.acc_count, c.B * 8, hipMemcpyDeviceToHost, P.stream));
     HMC_HIP(hipStreamSynchronize(P.stream));
     time_collect(c);
+    check_fixup(c);
   });
 }
 

@@ -150,6 +150,7 @@ __device__ inline double philox_exchange(uint64_t seed, uint32_t chain, uint64_t
 // ----------------------------------------------------------------------------
 constexpr int kMaxPriors = 4;
 constexpr int kTimingKinds = 8;   // kernel timing groups of a chain set
+constexpr int kFixTiles = 4096;   // tail tiles the bf16 GEMM's two-k-range fix-up counters cover
 
 struct DevPrior {
   int kind;           // HMC_PRIOR_NORMAL (a = means, b = inverse variances)

@@ -379,7 +379,7 @@ template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
     const uint16_t* __restrict__ A, int64_t lda, const uint16_t* __restrict__ B0,
     const uint16_t* __restrict__ B1, int64_t ldb, int M, int N, int K, int order, int F, int SPL,
-    float* __restrict__ part, int64_t TR, Epi epi) {
+    float* __restrict__ part, int64_t TR, int* __restrict__ fix, Epi epi) {
   constexpr int BK = 64;                 // elements per row per k-tile (128 bytes)
   constexpr int NW = (BM / WM) * (BN / WN);
   constexpr int TM = WM / 32, TN = WN / 32;
@@ -522,12 +522,65 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bf16(
   } else {
     // partial tile of k-range sp: part[sp][chain][row - first tail row]
     const int64_t mt0 = (int64_t)(F / gn) * BM;
+    auto store_partial = [&]() {
+      for (int idx = tid; idx < BN * (BM / 4); idx += NW * 64) {
+        const int nl = idx / (BM / 4), ml = 4 * (idx % (BM / 4));
+        const int64_t n = n_blk + nl;
+        if (n >= N) continue;
+        *reinterpret_cast<floatx4*>(&part[((int64_t)sp * N + n) * TR + (m_blk - mt0) + ml]) =
+            *reinterpret_cast<const floatx4*>(&Cs[nl * CLD + ml]);
+      }
+    };
+    if (fix == nullptr) {
+      store_partial();
+      return;
+    }
+    // Fix-up of a tile cut into two k-ranges (SPL = 2): the two workgroups
+    // take roles from the tile's counter (the first gets 0, the second 1 or
+    // 2).  The first stores its partial, publishes it at agent scope and adds
+    // 1 more, so the counter reaches 3 once both have arrived and the partial
+    // is visible, in either order.  The second waits for 3 (the first is
+    // already running, so the wait ends; a bounded spin sets fix[0] instead
+    // of hanging), resets the counter for the next launch and applies the
+    // epilogue to partial 0 + partial 1 — the split epilogue kernel's sum.
+    __shared__ int s_role;
+    int* cnt = fix + 1 + ((tm - F / gn) * gn + tn);
+    if (tid == 0) s_role = __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    // (Fences: release only in the first, one acquire after the wait in the
+    // second; the spin itself reads the counter relaxed, so it does not
+    // invalidate the caches the other tiles of the XCD are streaming H
+    // through.)
+    if (s_role == 0) {
+      store_partial();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (tid == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    if (tid == 0) {
+      int spins = 0;
+      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 3) {
+        if (++spins > (1 << 22)) {
+          __hip_atomic_fetch_or(fix, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int other = 1 - sp;
     for (int idx = tid; idx < BN * (BM / 4); idx += NW * 64) {
       const int nl = idx / (BM / 4), ml = 4 * (idx % (BM / 4));
       const int64_t n = n_blk + nl;
       if (n >= N) continue;
-      *reinterpret_cast<floatx4*>(&part[((int64_t)sp * N + n) * TR + (m_blk - mt0) + ml]) =
-          *reinterpret_cast<const floatx4*>(&Cs[nl * CLD + ml]);
+      const floatx4 own = *reinterpret_cast<const floatx4*>(&Cs[nl * CLD + ml]);
+      const floatx4 oth = *reinterpret_cast<const floatx4*>(&part[((int64_t)other * N + n) * TR + (m_blk - mt0) + ml]);
+      const floatx4 a = sp == 0 ? own : oth, b = sp == 0 ? oth : own;
+      const float v[4] = {a[0] + b[0], a[1] + b[1], a[2] + b[2], a[3] + b[3]};
+      epi.apply(m_blk + ml, 1, n, v);
     }
   }
 }
@@ -1070,6 +1123,9 @@ struct SplitWs {
   // kernel into kind, the split-tail epilogue into tail_kind
   Chains* tc = nullptr;
   int kind = -1, tail_kind = -1;
+  // tile counters of the two-k-range fix-up ([0] error flag, [1 + tile]), or
+  // null: the split epilogue kernel sums the partials
+  int* fix = nullptr;
 };
 
 template <int BM, int BN, int WM, int WN, int NSPLIT, int S, class Epi>
@@ -1079,14 +1135,16 @@ static void launch_bf16_cfg(hipStream_t s, const uint16_t* A, int64_t lda, const
   constexpr int NT = (BM / WM) * (BN / WN) * 64;
   SplitPlan pl = plan_split(BM, BN, M, N, K, ws.n_cu, ws.ptr != nullptr, 64, ws.min_piece, ws.max_spl);
   if (pl.split() && (size_t)pl.SPL * N * pl.TR > ws.elems) pl = plan_split(BM, BN, M, N, K, 0, false);
+  const int gn = (N + BN - 1) / BN, T = ((M + BM - 1) / BM) * gn;
+  int* fix = (pl.split() && pl.SPL == 2 && ws.fix && T - pl.F <= kFixTiles) ? ws.fix : nullptr;
   const bool timed = ws.tc != nullptr && ws.kind >= 0;
   hipEvent_t ev = nullptr;
   if (timed) time_begin(*ws.tc, ws.kind, ev);
   hipLaunchKernelGGL((k_gemm_bf16<BM, BN, WM, WN, NSPLIT, S, Epi>), dim3(pl.grid), dim3(NT), 0, s, A, lda, B0, B1,
-                     ldb, M, N, K, order, pl.F, pl.SPL, ws.ptr, pl.TR, epi);
+                     ldb, M, N, K, order, pl.F, pl.SPL, ws.ptr, pl.TR, fix, epi);
   HMC_HIP(hipGetLastError());
   if (timed) time_end(*ws.tc, ws.kind, ev);
-  if (pl.split()) {
+  if (pl.split() && !fix) {
     const int64_t work = (int64_t)N * (pl.TR / 4);
     const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
     if (timed) time_begin(*ws.tc, ws.tail_kind, ev);
@@ -1140,14 +1198,19 @@ static size_t split_elems_cfg(int cfg, int nsplit, int M, int N, int K, int n_cu
 }
 
 // kind / tail_kind >= 0: time the GEMM kernel and its split-tail epilogue in
-// those timing groups of c
+// those timing groups of c; fixup: tiles cut into two k-ranges sum their
+// partials inside the GEMM kernel (problem option split_fixup)
 template <class Epi>
 static void launch_bf16(Chains& c, int cfg, int split, const uint16_t* A, int64_t lda, const uint16_t* B0,
                         const uint16_t* B1, int64_t ldb, int M, int N, int K, const Epi& epi, int kind = -1,
-                        int tail_kind = -1) {
+                        int tail_kind = -1, bool fixup = false) {
   const Problem& p = *c.p;
   SplitWs ws;
   if (p.split_tail && c.split_ws) ws = SplitWs{c.split_ws, c.split_ws_elems, p.n_cu};
+  if (fixup && p.split_fixup && c.fix && ws.ptr) {
+    ws.fix = c.fix;
+    c.fix_used = true;
+  }
   if (kind >= 0) {
     ws.tc = &c;
     ws.kind = kind;
@@ -1287,7 +1350,7 @@ static void premult_kick_drift(Chains& c, double kick, double drift, double drif
     const int cur = c.dx_cur;
     EpiLeap<float> e{(float*)c.xt, (float*)c.pt, c.dxp[1 - cur], nullptr, c.ldx, c.eps, c.beta, kick, drift,
                      drift2, md, d, B, c.xa, c.yref, nullptr};
-    launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, c.dxp[cur], nullptr, c.ldx, d, B, (int)p.ldh, e, 4, 7);
+    launch_bf16(c, p.gemm_cfg_grad, 1, p.Hh, p.ldh, c.dxp[cur], nullptr, c.ldx, d, B, (int)p.ldh, e, 4, 7, true);
     c.dx_cur = 1 - cur;
   } else {
     // FP64: g = H x - b from a copy of the positions (the epilogue updates xt)

@@ -40,6 +40,7 @@ struct Problem {
   int gemm_cfg = 1;         // bf16 GEMM tile configuration of the residual GEMM
   int gemm_cfg_grad = 1;    // ... of the gradient GEMM
   int split_tail = 1;       // bf16 GEMMs: split the last partial round of tiles along K
+  int split_fixup = 0;      // premultiplied bf16 stage: two k-ranges summed inside the GEMM
   int n_cu = 0;             // compute units of the device (split-tail planning)
   int fused_ct = 0;         // fused engine: largest chains per workgroup (0: as LDS allows)
   // FP64 endpoint misfit through H = G^T S^-1 G (dense GEMM engine): -1 when
@@ -134,6 +135,8 @@ struct Chains {
   double* partial = nullptr;// endpoint partial sums [mblocks][B_pad]
   float* split_ws = nullptr;// bf16 GEMM split-tail partial planes
   size_t split_ws_elems = 0;
+  int* fix = nullptr;       // two-k-range fix-up: [0] error flag, [1 + tile] counters (kFixTiles)
+  bool fix_used = false;
   double* split_ws64 = nullptr;  // FP64 GEMM split-tail partial planes
   size_t split_ws64_elems = 0;
   int64_t n_mblocks = 0;

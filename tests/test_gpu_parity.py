@@ -298,6 +298,44 @@ def test_split_tail_matches_unsplit(precision, cfg, B, pm, order):
         assert _rel(grad["split"][b], grad["plain"][b]) < same
 
 
+@pytest.mark.parametrize("B,order", [(256, 0), (200, 0), (256, 1)])
+def test_split_fixup_equals_split_epilogue(B, order):
+    """Premultiplied bf16 stages whose tiles are all cut into two k-ranges
+    (256x128 tiles of d = 2200 planned for 36 compute units: 18 tiles x 2):
+    summing the two partials inside the GEMM (the second workgroup of a tile
+    adds the first one's published partial) gives the same bits as the split
+    epilogue kernel, whichever workgroup arrives first; repeated proposals
+    reuse the reset tile counters.  200 chains leave a partly empty chain
+    tile; order 1 deals the two k-ranges to different XCDs."""
+    rng = np.random.default_rng(23)
+    n, d, L = 2500, 2200, 5
+    G = bf16_to_f32(bf16_round(rng.normal(size=(n, d)) / np.sqrt(d))).astype(np.float64)
+    dd = G @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(G, dd, 0.01, dtype="bfloat16")])
+    X0 = (rng.normal(size=(B, d)) * 0.3).astype(np.float32).astype(np.float64)
+    res = {}
+    for fix in (0, 1):
+        prob = build_problem(post, precision="bf16", engine="gemm",
+                             options={"gemm_config": 3, "premult_trajectory": 1, "gemm_split_cus": 36,
+                                      "tile_order": order, "split_fixup": fix})
+        ch = Chains(prob, B)
+        ch.set_state(X0)
+        ch.set_anchor()
+        r = np.random.default_rng(5)
+        outs = []
+        for k in range(3):
+            P0 = r.normal(size=(B, d)).astype(np.float32).astype(np.float64)
+            e = 0.002 * r.uniform(0.5, 1.5, size=B)
+            u = r.uniform(size=B)
+            outs.append(ch.propose(P0, e, u, L, "lf", trajectory=True))
+        res[fix] = outs
+        ch.close()
+        prob.close()
+    for a, b in zip(res[0], res[1]):
+        for key in ("X1", "P1", "H0", "H1"):
+            assert np.array_equal(a[key], b[key]), key
+
+
 @pytest.mark.parametrize("precision,g_dtype,d,sym", [("bf16", "bfloat16", 300, 1), ("f64", None, 300, 1),
                                                      ("f64", None, 300, 0), ("f64", None, 100, 1),
                                                      ("f64", None, 520, 1), ("bf16", "bfloat16", 640, 1)])
