@@ -243,7 +243,7 @@ def _plain_fp64_leg(args, torch, post, X0, lam_max, mu_np, sd_np, premult):
     eps, done = _tune(ch, B, L, args.eps_scale / np.sqrt(lam_max), sub, args.seed + 7, "fp64 ")
     K = args.plain_steps
     E = max(K, args.ess_draws)
-    elapsed, acc = _timed(ch, B, L, eps, K, args.seed + 7, done, torch, trace=E)
+    elapsed, acc = _timed(ch, B, L, eps, K, args.seed + 7, done, torch, trace=E, every=args.timing_every)
     kt = _kernel_times(ch)
     if E > K:   # untimed continuation of the same chains for the ESS estimate
         ch.run(E - K, L, np.full(B, eps), seed=args.seed + 7, first_proposal=done + K)
@@ -390,7 +390,7 @@ def run_dense(args):
     # detached when a chunk's acceptance falls below the fallback threshold
     E = max(args.steps, args.ess_draws)
     ch.set_trace(E)
-    ch.enable_timing(True)
+    ch.enable_timing(True, every=args.timing_every)
     ch.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -543,6 +543,8 @@ def run_dense(args):
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel": gemm_name, "avg_launch_us": avg_s * 1e6, "launches": gemm_n,
                      "algorithmic_flop_per_launch": flop_launch,
+                     "event_sample": f"HIP events around the launches of every {args.timing_every}th timed "
+                                     f"proposal ({gemm_n} stage launches); an event pair costs the stream a few us",
                      "endpoint_fp64_avg_us": kt[1][0] / max(kt[1][1], 1) * 1e3, "stage": stage},
     }
 
@@ -581,9 +583,9 @@ def _tune(ch, B, L, eps, args, seed, log_prefix=""):
     return eps, done + args.warmup
 
 
-def _timed(ch, B, L, eps, steps, seed, first, torch=None, trace=None):
+def _timed(ch, B, L, eps, steps, seed, first, torch=None, trace=None, every=1):
     ch.set_trace(steps if trace is None else trace)
-    ch.enable_timing(True)
+    ch.enable_timing(True, every=every)
     ch.sync()
     if torch is not None:
         torch.cuda.synchronize()
@@ -936,6 +938,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="dense", choices=["dense", "tomography", "small", "uneven", "copy"])
     ap.add_argument("--problems", type=int, default=256)
+    ap.add_argument("--timing-every", type=int, default=5,
+                    help="kernel timing events on every k-th timed proposal (dense workload)")
     ap.add_argument("--adapt-integrators", type=int, default=1, choices=[0, 1],
                     help="uneven batch: warm-up choice of the 4-stage integrator for the least-moving problems")
     ap.add_argument("--adapt-pilot", type=int, default=30, help="proposals per adaptation pilot")

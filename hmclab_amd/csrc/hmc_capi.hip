@@ -171,7 +171,7 @@ static void set_data(Problem& P, int64_t n, const double* d_obs, const double* v
 void time_begin(Chains& c, int kind, hipEvent_t& a) {
   (void)kind;
   a = nullptr;
-  if (!c.timing) return;
+  if (!c.timing || c.timing_skip) return;
   if (c.ev_next >= c.ev_pool.size()) {
     hipEvent_t e;
     HMC_HIP(hipEventCreate(&e));
@@ -888,6 +888,7 @@ int hmc_chains_run(hmc_chains* hc, int64_t proposals, int L, int integrator, dou
         HMC_REQUIRE(c.hist_n < c.hist_cap, HMC_ERR_STATE, "tuning history capacity exceeded");
         hslot = c.hist_n++;
       }
+      c.timing_skip = c.timing && (prop % c.timing_every) != 0;
       if (gemm)
         gemm_proposal(c, false, L, integrator, seed, prop, randomize, autotune, target, learning_rate, slot,
                       hslot);
@@ -900,6 +901,7 @@ int hmc_chains_run(hmc_chains* hc, int64_t proposals, int L, int integrator, dou
                       nullptr, slot);
       if (c.timing && c.ev_pending.size() > 2048) time_collect(c);
     }
+    c.timing_skip = false;
     HMC_HIP(hipMemcpyAsync(stepsize, c.step, c.B * 8, hipMemcpyDeviceToHost, P.stream));
     if (accepted_total)
       HMC_HIP(hipMemcpyAsync(accepted_total, c.acc_count, c.B * 8, hipMemcpyDeviceToHost, P.stream));
@@ -1050,10 +1052,11 @@ int hmc_chains_trace_device(hmc_chains* hc, void** x_ptr, double** chi_ptr, int*
 
 int hmc_chains_enable_timing(hmc_chains* hc, int on) {
   return guarded("hmc_chains_enable_timing", [&] {
-    HMC_REQUIRE(hc, HMC_ERR_ARG, "NULL chains");
+    HMC_REQUIRE(hc && on >= 0, HMC_ERR_ARG, "invalid argument");
     Chains& c = hc->c;
     time_collect(c);
     c.timing = on != 0;
+    c.timing_every = on > 1 ? on : 1;
     for (int k = 0; k < kTimingKinds; ++k) {
       c.t_ms[k] = 0.0;
       c.t_n[k] = 0;

@@ -164,6 +164,8 @@ struct Chains {
 
   // timing
   bool timing = false;
+  int timing_every = 1;     // events only on proposals with index % timing_every == 0
+  bool timing_skip = false; // the current proposal records no events
   // kinds: 0 residual GEMM (fused engine: the whole-proposal kernel),
   // 1 endpoint misfit, 2 element-wise / Metropolis, 3 gradient GEMM with the
   // kick/drift epilogue (residual form), 4 premultiplied stage GEMM with its

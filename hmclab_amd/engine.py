@@ -250,8 +250,10 @@ class Chains:
         call("hmc_chains_trace_device", self._h, C.byref(xp), C.byref(cp), C.byref(dt))
         return xp.value, C.cast(cp, C.c_void_p).value, dt.value
 
-    def enable_timing(self, on: bool = True):
-        call("hmc_chains_enable_timing", self._h, int(bool(on)))
+    def enable_timing(self, on: bool = True, every: int = 1):
+        """Kernel timing with HIP events; every > 1 records only the launches
+        of proposals whose index is a multiple of it (hmc_chains_run)."""
+        call("hmc_chains_enable_timing", self._h, int(every) if on else 0)
 
     def timing(self, kind: int):
         ms, n = C.c_double(0.0), C.c_int64(0)

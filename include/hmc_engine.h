@@ -313,7 +313,9 @@ int hmc_chains_reset_trace(hmc_chains* c);
  * 5 = surrogate forward and input-gradient GEMMs, 6 = surrogate kick/drift,
  * 7 = the bf16 premultiplied stage's split-tail epilogue kernel.
  * total_ms and launches may be NULL.  Timing is off until
- * hmc_chains_enable_timing(c,1). */
+ * hmc_chains_enable_timing(c, k): k = 1 records every launch, k > 1 only the
+ * launches of proposals whose index is a multiple of k in hmc_chains_run
+ * (each event pair costs the stream a few microseconds), 0 turns it off. */
 int hmc_chains_enable_timing(hmc_chains* c, int on);
 int hmc_chains_timing(hmc_chains* c, int kind, double* total_ms, int64_t* launches);
 
