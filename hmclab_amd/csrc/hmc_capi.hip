@@ -93,21 +93,35 @@ int Problem::resolved_engine() const {
 }
 
 // ---- SELL builder ------------------------------------------------------------
-static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
-                       const std::vector<int64_t>& ptr, const std::vector<int32_t>& idx,
-                       const std::vector<double>& val, int traj_dt, int tgt_dt) {
-  S.rows = rows;
-  S.cols = cols;
-  // 16-bit column indices: a sparse target runs on the fused engine, whose LDS
-  // tile holds (rows + cols) values per chain, far below 65536 of either
-  HMC_REQUIRE(cols <= 65535 && rows <= 65535, HMC_ERR_UNSUPPORTED,
-              "sparse operator with more than 65535 rows or columns exceeds the fused engine's LDS tile");
-  S.nslices = (rows + 63) / 64;
+// Row order of a SELL-64 operator: rows sorted by decreasing length (stable),
+// 64 consecutive rows per slice; position p = slice * 64 + lane holds row
+// order[p].
+static std::vector<int32_t> sell_order(const std::vector<int64_t>& ptr, int64_t rows) {
   std::vector<int32_t> order(rows);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
     return (ptr[a + 1] - ptr[a]) > (ptr[b + 1] - ptr[b]);
   });
+  return order;
+}
+
+// Column indices are stored as colmap[column] (the fused engine keeps both
+// operands in LDS in SELL position order).
+static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
+                       const std::vector<int64_t>& ptr, const std::vector<int32_t>& idx,
+                       const std::vector<double>& val, const std::vector<int32_t>& order,
+                       const std::vector<int32_t>& colmap, int traj_dt, int tgt_dt) {
+  S.rows = rows;
+  S.cols = cols;
+  // 16-bit column positions: a sparse target runs on the fused engine, whose
+  // LDS tile holds (rows + cols) values per chain, far below 65536 of either
+  HMC_REQUIRE(cols <= kSellMaxDim && rows <= kSellMaxDim, HMC_ERR_UNSUPPORTED,
+              "sparse operator with more than 65472 rows or columns exceeds the fused engine's LDS tile");
+  S.nslices = (rows + 63) / 64;
+  // Each lane's entries are stored in groups of kSellGroup consecutive k
+  // (entry k of lane r at base + (k / G) * 64 * G + r * G + k % G), so slice
+  // widths are rounded up to a multiple of the group; padding entries hold
+  // column 0 and value 0.
   std::vector<int32_t> base(S.nslices), width(S.nslices), perm(S.nslices * 64, -1);
   int64_t total = 0;
   int maxw = 0;
@@ -120,13 +134,16 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
         w = std::max<int>(w, (int)(ptr[order[k] + 1] - ptr[order[k]]));
       }
     }
+    w = (w + kSellGroup - 1) / kSellGroup * kSellGroup;
     HMC_REQUIRE(total <= INT32_MAX, HMC_ERR_UNSUPPORTED, "sparse matrix too large for int32 SELL offsets");
     base[s] = (int32_t)total;
     width[s] = w;
     total += (int64_t)w * 64;
     maxw = std::max(maxw, w);
   }
-  HMC_REQUIRE(total <= INT32_MAX, HMC_ERR_UNSUPPORTED, "sparse matrix too large for int32 SELL offsets");
+  // the fused engine reads the arrays with buffer loads at 32-bit byte offsets
+  HMC_REQUIRE(total * 8 + 8192 <= INT32_MAX, HMC_ERR_UNSUPPORTED,
+              "sparse matrix too large for the fused engine's 32-bit buffer offsets");
   S.total = total;
   S.max_width = maxw;
   std::vector<uint16_t> col(std::max<int64_t>(total, 1), 0);
@@ -139,8 +156,8 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
       const int32_t row = perm[s * 64 + r];
       if (row < 0) continue;
       for (int64_t k = 0; k < ptr[row + 1] - ptr[row]; ++k) {
-        const int64_t o = base[s] + k * 64 + r;
-        col[o] = (uint16_t)idx[ptr[row] + k];
+        const int64_t o = base[s] + (k / kSellGroup) * 64 * kSellGroup + r * kSellGroup + k % kSellGroup;
+        col[o] = (uint16_t)colmap[idx[ptr[row] + k]];
         store_elem(vtraj.data(), traj_dt, o, val[ptr[row] + k]);
         if (!alias) store_elem(vtgt.data(), tgt_dt, o, val[ptr[row] + k]);
       }
@@ -397,7 +414,6 @@ int hmc_problem_set_csr(hmc_problem* hp, const int32_t* indptr, const int32_t* i
     const int traj = P.precision;
     const int tgt = target_store(traj, dtype);
     P.tgt_dtype = tgt;
-    build_sell(P, P.sg, n, d, ptr, idx, val, traj, tgt);
     // transpose (counting sort by column, rows kept in increasing order)
     std::vector<int64_t> tptr(d + 1, 0);
     for (int64_t k = 0; k < nnz; ++k) tptr[idx[k] + 1]++;
@@ -411,8 +427,28 @@ int hmc_problem_set_csr(hmc_problem* hp, const int32_t* indptr, const int32_t* i
         tidx[o] = (int32_t)i;
         tval[o] = val[k];
       }
-    build_sell(P, P.sgt, d, n, tptr, tidx, tval, traj, traj);
+    // positions: data row i sits at forward-slice position ppos[i], parameter
+    // j at backward-slice position qpos[j]; each operator's columns are stored
+    // as positions of the other
+    const std::vector<int32_t> ford = sell_order(ptr, n), bord = sell_order(tptr, d);
+    std::vector<int32_t> ppos(n), qpos(d);
+    for (int64_t p = 0; p < n; ++p) ppos[ford[p]] = (int32_t)p;
+    for (int64_t q = 0; q < d; ++q) qpos[bord[q]] = (int32_t)q;
+    build_sell(P, P.sg, n, d, ptr, idx, val, ford, qpos, traj, tgt);
+    build_sell(P, P.sgt, d, n, tptr, tidx, tval, bord, ppos, traj, traj);
     set_data(P, n, d_obs, var, scalar);
+    {
+      const int64_t np = P.sg.nslices * 64;
+      std::vector<double> pd(np, 0.0), pv(np, 1.0);
+      for (int64_t p = 0; p < n; ++p) {
+        pd[p] = d_obs[ford[p]];
+        pv[p] = scalar ? var[0] : var[ford[p]];
+      }
+      P.pos_dobs = (double*)upload(P.owned, pd.data(), np * 8, P.stream);
+      P.pos_var = (double*)upload(P.owned, pv.data(), np * 8, P.stream);
+      P.pos_model = (double*)dev_alloc(P.owned, (size_t)kPosModelArrays * P.sgt.nslices * 64 * 8);
+      P.pos_model_version = -1;
+    }
     P.op = 2;
   });
 }
@@ -429,6 +465,7 @@ int hmc_problem_add_prior(hmc_problem* hp, int kind, const double* a, const doub
     P.prior_b[k] = (double*)upload(P.owned, b, P.d * 8, P.stream);
     P.misfit_const += constant;
     P.n_priors++;
+    P.model_version++;
   });
 }
 
@@ -441,6 +478,7 @@ int hmc_problem_set_bounds(hmc_problem* hp, const double* lower, const double* u
         HMC_REQUIRE(upper[j] > lower[j], HMC_ERR_ARG, "Bounds vectors are incompatible.");
     P.lo = lower ? (double*)upload(P.owned, lower, P.d * 8, P.stream) : nullptr;
     P.hi = upper ? (double*)upload(P.owned, upper, P.d * 8, P.stream) : nullptr;
+    P.model_version++;
   });
 }
 
@@ -464,6 +502,7 @@ int hmc_problem_set_mass(hmc_problem* hp, int kind, const double* diagonal) {
       throw Error{HMC_ERR_ARG, "unknown mass matrix kind"};
     }
     P.mass_kind = kind;
+    P.model_version++;
   });
 }
 

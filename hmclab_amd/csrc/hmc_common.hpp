@@ -151,6 +151,11 @@ __device__ inline double philox_exchange(uint64_t seed, uint32_t chain, uint64_t
 constexpr int kMaxPriors = 4;
 constexpr int kTimingKinds = 8;   // kernel timing groups of a chain set
 constexpr int kFixTiles = 4096;   // tail tiles the bf16 GEMM's two-k-range fix-up counters cover
+constexpr int kSellGroup = 8;     // entries per lane group of the fused engine's SELL-64 layout
+constexpr int kSellMaxDim = 65472;  // rows/columns of a SELL operator (16-bit positions < 65472)
+// position-ordered model arrays of the fused sparse engine: prior a and b per
+// term, inverse mass, mass square root, lower and upper bound
+constexpr int kPosModelArrays = 2 * kMaxPriors + 4;
 
 struct DevPrior {
   int kind;           // HMC_PRIOR_NORMAL (a = means, b = inverse variances)

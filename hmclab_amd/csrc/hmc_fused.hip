@@ -22,11 +22,13 @@ constexpr int FU_NT = 512;
 constexpr int FU_NW = FU_NT / 64;
 
 struct DevSell {
+  int64_t rows;       // operator rows (positions >= rows are padding)
   const int32_t* base;
   const int32_t* width;
   const int32_t* perm;
   const uint16_t* col;
   int64_t nslices;
+  int64_t total;      // stored entries (slices x 64 x width)
 };
 
 struct DevDense {
@@ -42,55 +44,129 @@ template <>
 struct TypeOf<HMC_F32> { using T = float; };
 
 // One SELL row of a lane: acc[c] += sum_k val[k] * in[col[k]][c], k ascending.
-// The column/value loads of SELL_U consecutive k are issued before their LDS
-// gathers so that many L2 loads are in flight per lane (the sum order is
-// unchanged).
-constexpr int SELL_U = 8;
-template <typename Acc, typename V, typename Tin, int CT>
-__device__ inline void sell_row(const uint16_t* __restrict__ colp, const V* __restrict__ valp, int64_t base,
-                                int32_t width, int lane, const Tin* __restrict__ in, Acc acc[CT]) {
-  int32_t k = 0;
-  for (; k + SELL_U <= width; k += SELL_U) {
-    int32_t cl[SELL_U];
-    Acc vl[SELL_U];
+// Slices store each lane's entries in groups of SELL_G consecutive k: entry k
+// of lane r sits at base + (k / SELL_G) * 64 * SELL_G + r * SELL_G + k % SELL_G,
+// and slice widths are multiples of SELL_G, so one group is one 16-byte column
+// load and 16-byte value loads (three vector-memory instructions per 8 entries
+// for float32 values instead of sixteen).  The sum order is k ascending.
+constexpr int SELL_G = kSellGroup;
+
+// Buffer resource over n bytes of a matrix array (gfx9 raw buffer, stride 0:
+// loads at byte offsets >= n return 0 and never fault).
+__device__ inline __amdgpu_buffer_rsrc_t sell_rsrc(const void* p, int64_t n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)n, 0x00020000);
+}
+
+// 16 bytes at byte offset voff + soff of a buffer.  The builtin's vector
+// result is only ever bit-cast whole to a HIP vector type: subscripting it
+// (q[i]) compiles to wrong code with this toolchain.
+__device__ inline uint4 buf_load16(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+// SELL_G consecutive values of one lane from byte offset voff + soff
+template <typename Acc, typename V>
+__device__ inline void load_group(__amdgpu_buffer_rsrc_t r, int voff, int soff, Acc out[SELL_G]) {
 #pragma unroll
-    for (int u = 0; u < SELL_U; ++u) {
-      const int64_t idx = base + (int64_t)(k + u) * 64 + lane;
-      cl[u] = colp[idx];
-      vl[u] = (Acc)valp[idx];
+  for (int h = 0; h < SELL_G * (int)sizeof(V) / 16; ++h) {
+    const uint4 q = buf_load16(r, voff + 16 * h, soff);
+    if constexpr (sizeof(V) == 4) {
+      const float4 f = __builtin_bit_cast(float4, q);
+      out[4 * h] = (Acc)f.x;
+      out[4 * h + 1] = (Acc)f.y;
+      out[4 * h + 2] = (Acc)f.z;
+      out[4 * h + 3] = (Acc)f.w;
+    } else {
+      const double2 f = __builtin_bit_cast(double2, q);
+      out[2 * h] = (Acc)f.x;
+      out[2 * h + 1] = (Acc)f.y;
     }
-#pragma unroll
-    for (int u = 0; u < SELL_U; ++u)
-#pragma unroll
-      for (int c = 0; c < CT; ++c) acc[c] += vl[u] * (Acc)in[(int64_t)cl[u] * CT + c];
-  }
-  for (; k < width; ++k) {
-    const int64_t idx = base + (int64_t)k * 64 + lane;
-    const int32_t col = colp[idx];
-    const Acc v = (Acc)valp[idx];
-#pragma unroll
-    for (int c = 0; c < CT; ++c) acc[c] += v * (Acc)in[(int64_t)col * CT + c];
   }
 }
 
-// out-row callback receives (slice, row (-1 padding), acc[CT])
-template <typename Acc, typename V, typename Tin, int CT, class F>
+// acc[c] += v * in[col][c] for the CT chains of the tile, the CT values read
+// as one LDS vector where their size allows
+template <typename Acc, typename Tin, int CT>
+__device__ inline void gather_fma(const Tin* __restrict__ in, uint32_t col, Acc v, Acc acc[CT]) {
+  const Tin* q = in + col * CT;
+  if constexpr (CT == 2 && sizeof(Tin) == 4) {
+    const float2 t = *reinterpret_cast<const float2*>(q);
+    acc[0] += v * (Acc)t.x;
+    acc[1] += v * (Acc)t.y;
+  } else if constexpr (CT == 4 && sizeof(Tin) == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(q);
+    acc[0] += v * (Acc)t.x;
+    acc[1] += v * (Acc)t.y;
+    acc[2] += v * (Acc)t.z;
+    acc[3] += v * (Acc)t.w;
+  } else if constexpr (CT == 2 && sizeof(Tin) == 8) {
+    const double2 t = *reinterpret_cast<const double2*>(q);
+    acc[0] += v * (Acc)t.x;
+    acc[1] += v * (Acc)t.y;
+  } else {
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] += v * (Acc)q[c];
+  }
+}
+
+// The loads of group g + 1 are issued before the gathers of group g, so the
+// matrix-load latency overlaps the LDS gathers and FMAs of the group before.
+// Matrix loads are buffer loads: a per-lane byte offset plus a wave-uniform
+// slice/group offset that advances on the scalar unit.  The prefetch after the
+// last group reads the next slice's first group (or zeros past the array end)
+// and is never used.
+template <typename Acc, typename V, typename Tin, int CT>
+__device__ inline void sell_row(const DevSell& S, const V* __restrict__ valp, int64_t base, int32_t width,
+                                int lane, const Tin* __restrict__ in, Acc acc[CT]) {
+  if (width <= 0) return;
+  const __amdgpu_buffer_rsrc_t rc = sell_rsrc(S.col, S.total * 2);
+  const __amdgpu_buffer_rsrc_t rv = sell_rsrc(valp, S.total * (int64_t)sizeof(V));
+  const int vc = lane * SELL_G * 2, vv = lane * SELL_G * (int)sizeof(V);
+  int sc = (int)base * 2, sv = (int)base * (int)sizeof(V);
+  uint4 cw = buf_load16(rc, vc, sc);
+  Acc vl[SELL_G];
+  load_group<Acc, V>(rv, vv, sv, vl);
+  for (int32_t k = 0; k < width; k += SELL_G) {
+    sc += 64 * SELL_G * 2;
+    sv += 64 * SELL_G * (int)sizeof(V);
+    const uint4 cn = buf_load16(rc, vc, sc);
+    Acc vn[SELL_G];
+    load_group<Acc, V>(rv, vv, sv, vn);
+    const uint32_t wd[4] = {cw.x, cw.y, cw.z, cw.w};
+#pragma unroll
+    for (int u = 0; u < SELL_G; ++u) {
+      const uint32_t cl = (u & 1) ? (wd[u >> 1] >> 16) : (wd[u >> 1] & 0xffffu);
+      gather_fma<Acc, Tin, CT>(in, cl, vl[u], acc);
+    }
+    cw = cn;
+#pragma unroll
+    for (int u = 0; u < SELL_G; ++u) vl[u] = vn[u];
+  }
+}
+
+// Forward passes over the wave's slices.  Output position pos = slice * 64 +
+// lane (-1 for padding lanes): the sparse operator's rows in SELL position
+// order, the dense operator's rows in order.  pre(pos) runs before the row, so
+// the loads it issues overlap the row's; then f(pos, pre(pos), acc[CT]).
+template <typename Acc, typename V, typename Tin, int CT, class Pre, class F>
 __device__ inline void sell_pass(const DevSell& S, const V* __restrict__ val,
-                                 const Tin* __restrict__ in, int w, int lane, F&& f) {
+                                 const Tin* __restrict__ in, int w, int lane, Pre&& pre, F&& f) {
   for (int64_t s = w; s < S.nslices; s += FU_NW) {
     const int32_t base = S.base[s], width = S.width[s];
-    const int32_t row = S.perm[s * 64 + lane];
+    const int64_t pos = (s * 64 + lane < S.rows) ? s * 64 + lane : -1;
+    const auto pv = pre(pos);
     Acc acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = Acc(0);
-    sell_row<Acc, V, Tin, CT>(S.col, val, base, width, lane, in, acc);
-    f(s, (int64_t)row, acc);
+    sell_row<Acc, V, Tin, CT>(S, val, base, width, lane, in, acc);
+    f(pos, pv, acc);
   }
 }
 
 // One dense row of a lane over the k-major layout (element (row, k) at
 // mp[k * ld]): acc[c] += sum_k mp[k * ld] * in[k][c], k ascending, with the
 // matrix loads of SELL_U consecutive k issued before their use.
+constexpr int SELL_U = 8;
 template <typename Acc, typename V, typename Tin, int CT>
 __device__ inline void dense_row(const V* __restrict__ mp, int64_t ld, int64_t ncols,
                                  const Tin* __restrict__ in, Acc acc[CT]) {
@@ -111,17 +187,19 @@ __device__ inline void dense_row(const V* __restrict__ mp, int64_t ld, int64_t n
   }
 }
 
-template <typename Acc, typename V, typename Tin, int CT, class F>
+template <typename Acc, typename V, typename Tin, int CT, class Pre, class F>
 __device__ inline void dense_pass(const DevDense& D, const V* __restrict__ M,
-                                  const Tin* __restrict__ in, int w, int lane, F&& f) {
+                                  const Tin* __restrict__ in, int w, int lane, Pre&& pre, F&& f) {
   const int64_t nsl = (D.nrows + 63) / 64;
   for (int64_t s = w; s < nsl; s += FU_NW) {
     const int64_t row = s * 64 + lane;
+    const int64_t pos = row < D.nrows ? row : -1;
+    const auto pv = pre(pos);
     Acc acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[c] = Acc(0);
     if (row < D.nrows) dense_row<Acc, V, Tin, CT>(M + row, D.ld, D.ncols, in, acc);
-    f(s, row < D.nrows ? row : (int64_t)-1, acc);
+    f(pos, pv, acc);
   }
 }
 
@@ -142,29 +220,91 @@ struct Ops {
   int64_t ldh;
   const double* hb;
 
-  template <typename Acc, typename V, typename Tin, int CT, class F>
-  __device__ void forward(const V* val, const Tin* in, int w, int lane, F&& f) const {
-    if constexpr (SPARSE) sell_pass<Acc, V, Tin, CT>(fwd_s, val, in, w, lane, f);
-    else dense_pass<Acc, V, Tin, CT>(fwd_d, val, in, w, lane, f);
+  template <typename Acc, typename V, typename Tin, int CT, class Pre, class F>
+  __device__ void forward(const V* val, const Tin* in, int w, int lane, Pre&& pre, F&& f) const {
+    if constexpr (SPARSE) sell_pass<Acc, V, Tin, CT>(fwd_s, val, in, w, lane, pre, f);
+    else dense_pass<Acc, V, Tin, CT>(fwd_d, val, in, w, lane, pre, f);
   }
-  template <typename Acc, typename V, typename Tin, int CT, class F>
-  __device__ void backward(const V* val, const Tin* in, int w, int lane, F&& f) const {
-    if constexpr (SPARSE) sell_pass<Acc, V, Tin, CT>(bwd_s, val, in, w, lane, f);
-    else dense_pass<Acc, V, Tin, CT>(bwd_d, val, in, w, lane, f);
+  // Parameter slots: the LDS row (Xs[x][CT]) and model-array index of a
+  // parameter.  Sparse: the parameter's backward-slice position q (the model
+  // arrays passed to the kernel are the position-ordered copies); dense: the
+  // parameter index itself.
+  // slot owned by (wave slice s, lane), -1 for padding lanes
+  __device__ int64_t slot_of(int64_t s, int lane) const {
+    const int64_t x = s * 64 + lane;
+    return x < d ? x : -1;
   }
-  // owned parameter of (wave slice s, lane)
-  __device__ int64_t param_of(int64_t s, int lane) const {
-    if constexpr (SPARSE) return bwd_s.perm[s * 64 + lane];
-    else {
-      const int64_t j = s * 64 + lane;
-      return j < d ? j : -1;
-    }
+  // parameter index of slot x (0 <= x < d)
+  __device__ int64_t param_at(int64_t x) const {
+    if constexpr (SPARSE) return bwd_s.perm[x];
+    else return x;
   }
   __device__ int64_t param_slices() const {
     if constexpr (SPARSE) return bwd_s.nslices;
     else return (d + 63) / 64;
   }
 };
+
+template <typename T>
+struct T2 {
+  T a, b;
+};
+
+// Model constants of one parameter slot, loaded before the slot's backward
+// row so that their latency overlaps the row's streaming.  The helpers below
+// apply the same conversions and operations as prior_grad, out_of_bounds,
+// kgrad and corrector (absent bounds become -inf/+inf and unit mass 1, which
+// leave every comparison and product unchanged).
+template <typename T>
+struct ParamCst {
+  T a[kMaxPriors], b[kMaxPriors];
+  T minv;
+  double lo, hi;
+};
+
+template <typename T>
+__device__ inline ParamCst<T> param_cst(const DevModel& md, int64_t x) {
+  ParamCst<T> c;
+#pragma unroll
+  for (int k = 0; k < kMaxPriors; ++k) {
+    const bool on = k < md.n_priors && x >= 0;
+    c.a[k] = on ? (T)md.pr[k].a[x] : T(0);
+    c.b[k] = on ? (T)md.pr[k].b[x] : T(0);
+  }
+  c.minv = (md.minv && x >= 0) ? (T)md.minv[x] : T(1);
+  c.lo = (md.lo && x >= 0) ? md.lo[x] : -INFINITY;
+  c.hi = (md.hi && x >= 0) ? md.hi[x] : INFINITY;
+  return c;
+}
+
+template <typename T>
+__device__ inline T prior_grad_c(const DevModel& md, const ParamCst<T>& pc, T x) {
+  T g = T(0);
+#pragma unroll
+  for (int k = 0; k < kMaxPriors; ++k) {
+    if (k >= md.n_priors) break;
+    if (md.pr[k].kind == HMC_PRIOR_NORMAL) {
+      g += -pc.b[k] * (pc.a[k] - x);
+    } else {
+      T sg = (x > pc.a[k]) ? T(1) : ((x < pc.a[k]) ? T(-1) : T(0));
+      g += sg * pc.b[k];
+    }
+  }
+  return g;
+}
+
+template <typename T>
+__device__ inline void corrector_c(const ParamCst<T>& pc, T& x, T& p) {
+  const T lo = (T)pc.lo, hi = (T)pc.hi;
+  if (x < lo) {
+    x += T(2) * (lo - x);
+    p *= T(-1);
+  }
+  if (x > hi) {
+    x += T(2) * (hi - x);
+    p *= T(-1);
+  }
+}
 
 struct FusedArgs {
   void* x0;            // [B][ldx] state dtype
@@ -212,7 +352,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   __shared__ double s_k0[CT], s_eps[CT], s_chi1[CT], s_k1[CT], s_beta[CT];
   __shared__ int s_ok[CT];
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t c0 = (int64_t)blockIdx.x * CT;
   const int64_t nps = op.param_slices();
   T* x0 = reinterpret_cast<T*>(a.x0);
@@ -235,11 +375,13 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     s_eps[tid] = e;
     s_beta[tid] = (a.beta && c < a.B) ? a.beta[c] : 1.0;
   }
-  // ---- load positions of the tile
-  for (int64_t j = tid; j < op.d; j += FU_NT)
+  // ---- load positions of the tile (slot x holds parameter param_at(x))
+  for (int64_t x = tid; x < op.d; x += FU_NT) {
+    const int64_t j = op.param_at(x);
 #pragma unroll
     for (int c = 0; c < CT; ++c)
-      Xs[j * CT + c] = (c0 + c < a.B) ? x0[(c0 + c) * a.ldx + j] : T(0);
+      Xs[x * CT + c] = (c0 + c < a.B) ? x0[(c0 + c) * a.ldx + j] : T(0);
+  }
   __syncthreads();
 
   // ---- momentum draw (owned parameters), K0, first drift
@@ -250,24 +392,25 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
 #pragma unroll
   for (int ks = 0; ks < KMAX; ++ks) {
     const int64_t s = w + (int64_t)ks * FU_NW;
-    const int64_t j = (s < nps) ? op.param_of(s, lane) : -1;
+    const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
+    const int64_t j = x >= 0 ? op.param_at(x) : -1;
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
       T pv = T(0);
-      if (j >= 0 && c0 + c < a.B) {
+      if (x >= 0 && c0 + c < a.B) {
         if (a.p_host) {
           pv = (T)a.p_host[(c0 + c) * a.ldx + j];
         } else {
           double z0, z1;
           philox_normal2(a.seed, (uint32_t)(c0 + c), (uint64_t)a.prop, (uint32_t)(j >> 1), z0, z1);
           const double z = (j & 1) ? z1 : z0;
-          pv = (T)(md.msqrt ? md.msqrt[j] * z : z);
+          pv = (T)(md.msqrt ? md.msqrt[x] * z : z);
         }
-        kloc[c] += kterm(md, j, (double)pv);
-        T xv = Xs[j * CT + c];
-        xv += (T)(a.first_drift * s_eps[c]) * kgrad<T>(md, j, pv);
-        corrector<T>(md, j, xv, pv);
-        Xs[j * CT + c] = xv;
+        kloc[c] += kterm(md, x, (double)pv);
+        T xv = Xs[x * CT + c];
+        xv += (T)(a.first_drift * s_eps[c]) * kgrad<T>(md, x, pv);
+        corrector<T>(md, x, xv, pv);
+        Xs[x * CT + c] = xv;
       }
       preg[ks][c] = pv;
     }
@@ -281,25 +424,31 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
 
   // ---- stages: gradient, kick, drift (+ drift2)
   for (int st = 0; st < a.L * a.P; ++st) {
+    // the lane index as an opaque value per stage: without it every per-slot
+    // address and model constant (invariant across stages) is hoisted out of
+    // the loop and held in registers, which costs occupancy
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
     const int step = st / a.P, k = st - step * a.P;
     const double* sc = (step + 1 < a.L) ? a.pat[k] : a.pat_last[k];
     const double kick = sc[0], drift = sc[1], drift2 = sc[2];
     // kick (gradient g without the prior), drift, reflect for owned slice ks
-    auto update = [&](int ks, int64_t j, const T* g) {
+    // at slot x, with the slot's model constants pc
+    auto update = [&](int ks, int64_t x, const T* g, const ParamCst<T>& pc) {
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
         const double e = s_eps[c];
-        T xv = Xs[j * CT + c], pv = preg[ks][c];
-        T gg = g[c] + prior_grad<T>(md, j, xv);
-        if (out_of_bounds(md, j, (double)xv)) gg = (T)INFINITY;
+        T xv = Xs[x * CT + c], pv = preg[ks][c];
+        T gg = g[c] + prior_grad_c<T>(md, pc, xv);
+        if ((double)xv < pc.lo || (double)xv > pc.hi) gg = (T)INFINITY;
         pv -= (T)(a.beta ? kick * e * s_beta[c] : kick * e) * gg;
-        xv += (T)(drift * e) * kgrad<T>(md, j, pv);
-        corrector<T>(md, j, xv, pv);
+        xv += (T)(drift * e) * (pc.minv * pv);
+        corrector_c<T>(pc, xv, pv);
         if (drift2 != 0.0) {
-          xv += (T)(drift2 * e) * kgrad<T>(md, j, pv);
-          corrector<T>(md, j, xv, pv);
+          xv += (T)(drift2 * e) * (pc.minv * pv);
+          corrector_c<T>(pc, xv, pv);
         }
-        Xs[j * CT + c] = xv;
+        Xs[x * CT + c] = xv;
         preg[ks][c] = pv;
       }
     };
@@ -313,7 +462,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
           const int64_t s = w + (int64_t)ks * FU_NW;
 #pragma unroll
           for (int c = 0; c < CT; ++c) gk[ks][c] = T(0);
-          const int64_t row = s * 64 + lane;
+          const int64_t row = s * 64 + ln;
           if (s < nps && row < op.d) {
             dense_row<T, T, T, CT>(op.h_val + row, op.ldh, op.d, Xs, gk[ks]);
             const T bj = (T)op.hb[row];
@@ -326,19 +475,22 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
         for (int ks = 0; ks < KMAX; ++ks) {
           const int64_t s = w + (int64_t)ks * FU_NW;
           if (s < nps) {
-            const int64_t j = op.param_of(s, lane);
-            if (j >= 0) update(ks, j, gk[ks]);
+            const int64_t x = op.slot_of(s, ln);
+            if (x >= 0) update(ks, x, gk[ks], param_cst<T>(md, x));
           }
         }
         __syncthreads();
         continue;
       }
     }
-    op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, lane, [&](int64_t, int64_t i, T* acc) {
+    // data and variance of output position i (position-ordered arrays)
+    auto data_at = [&](int64_t i) {
+      return i >= 0 ? T2<T>{(T)a.dobs[i], (T)a.var[i]} : T2<T>{T(0), T(1)};
+    };
+    op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, ln, data_at, [&](int64_t i, T2<T> dv, T* acc) {
       if (i >= 0) {
-        const T di = (T)a.dobs[i], vi = (T)a.var[i];
 #pragma unroll
-        for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - di) / vi;
+        for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - dv.a) / dv.b;
       }
     });
     __syncthreads();
@@ -347,17 +499,18 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     for (int ks = 0; ks < KMAX; ++ks) {
       const int64_t s = w + (int64_t)ks * FU_NW;
       if (s < nps) {
+        const int64_t x = op.slot_of(s, ln);
+        const ParamCst<T> pc = param_cst<T>(md, x);
         T g[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) g[c] = T(0);
         if constexpr (SPARSE) {
-          sell_row<T, T, T, CT>(op.bwd_s.col, op.bwd_val, op.bwd_s.base[s], op.bwd_s.width[s], lane, Rs, g);
+          sell_row<T, T, T, CT>(op.bwd_s, op.bwd_val, op.bwd_s.base[s], op.bwd_s.width[s], ln, Rs, g);
         } else {
-          const int64_t row = s * 64 + lane;
+          const int64_t row = s * 64 + ln;
           if (row < op.d) dense_row<T, T, T, CT>(op.bwd_val + row, op.bwd_d.ld, op.n, Rs, g);
         }
-        const int64_t j = op.param_of(s, lane);
-        if (j >= 0) update(ks, j, g);
+        if (x >= 0) update(ks, x, g, pc);
       }
     }
     __syncthreads();
@@ -368,13 +521,16 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
 #pragma unroll
   for (int c = 0; c < CT; ++c) dm[c] = 0.0;
   if (op.n > 0) {
-    op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, [&](int64_t, int64_t i, double* acc) {
+    auto data_at = [&](int64_t i) {
+      return i >= 0 ? T2<double>{a.dobs[i], a.var[i]} : T2<double>{0.0, 1.0};
+    };
+    op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, data_at, [&](int64_t i, T2<double> dv,
+                                                                                  double* acc) {
       if (i >= 0) {
-        const double di = a.dobs[i], vi = a.var[i];
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-          const double rr = acc[c] - di;
-          dm[c] += rr * rr / vi;
+          const double rr = acc[c] - dv.a;
+          dm[c] += rr * rr / dv.b;
         }
       }
     });
@@ -386,14 +542,14 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
 #pragma unroll
   for (int ks = 0; ks < KMAX; ++ks) {
     const int64_t s = w + (int64_t)ks * FU_NW;
-    const int64_t j = (s < nps) ? op.param_of(s, lane) : -1;
-    if (j >= 0) {
+    const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
+    if (x >= 0) {
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        const double xv = (double)Xs[j * CT + c];
-        pm[c] += prior_misfit(md, j, xv);
-        if (out_of_bounds(md, j, xv)) ob[c] = 1.0;
-        kk[c] += kterm(md, j, (double)preg[ks][c]);
+        const double xv = (double)Xs[x * CT + c];
+        pm[c] += prior_misfit(md, x, xv);
+        if (out_of_bounds(md, x, xv)) ob[c] = 1.0;
+        kk[c] += kterm(md, x, (double)preg[ks][c]);
       }
     }
   }
@@ -448,14 +604,15 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
 #pragma unroll
   for (int ks = 0; ks < KMAX; ++ks) {
     const int64_t s = w + (int64_t)ks * FU_NW;
-    const int64_t j = (s < nps) ? op.param_of(s, lane) : -1;
-    if (j >= 0) {
+    const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
+    if (x >= 0) {
+      const int64_t j = op.param_at(x);
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
         const int64_t ch = c0 + c;
         if (ch < a.B) {
           const int64_t o = ch * a.ldx + j;
-          const T xv = Xs[j * CT + c];
+          const T xv = Xs[x * CT + c];
           if (xo) xo[o] = xv;
           if (po) po[o] = preg[ks][c];
           if (s_ok[c]) x0[o] = xv;
@@ -476,21 +633,23 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
   T* Xs = reinterpret_cast<T*>(smem_raw);
   T* Rs = Xs + round_up(op.d, 2) * CT;
   __shared__ double scratch[FU_NW + 1];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t c0 = (int64_t)blockIdx.x * CT;
   const int64_t nps = op.param_slices();
-  for (int64_t j = tid; j < op.d; j += FU_NT)
+  for (int64_t x = tid; x < op.d; x += FU_NT) {
+    const int64_t j = op.param_at(x);
 #pragma unroll
-    for (int c = 0; c < CT; ++c) Xs[j * CT + c] = (c0 + c < B) ? X[(c0 + c) * ldx + j] : T(0);
+    for (int c = 0; c < CT; ++c) Xs[x * CT + c] = (c0 + c < B) ? X[(c0 + c) * ldx + j] : T(0);
+  }
   __syncthreads();
   // chain out-of-bounds flags (whole-vector semantics of misfit_bounds)
   double obl[CT];
 #pragma unroll
   for (int c = 0; c < CT; ++c) obl[c] = 0.0;
-  for (int64_t j = tid; j < op.d; j += FU_NT)
+  for (int64_t x = tid; x < op.d; x += FU_NT)
 #pragma unroll
     for (int c = 0; c < CT; ++c)
-      if (out_of_bounds(md, j, (double)Xs[j * CT + c])) obl[c] = 1.0;
+      if (out_of_bounds(md, x, (double)Xs[x * CT + c])) obl[c] = 1.0;
   double obc[CT];
 #pragma unroll
   for (int c = 0; c < CT; ++c) obc[c] = block_sum<FU_NT>(obl[c], scratch);
@@ -500,19 +659,21 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
 #pragma unroll
     for (int c = 0; c < CT; ++c) dm[c] = pm[c] = 0.0;
     if (op.n > 0) {
-      op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, [&](int64_t, int64_t i, double* acc) {
+      auto data_at = [&](int64_t i) { return i >= 0 ? T2<double>{dobs[i], var[i]} : T2<double>{0.0, 1.0}; };
+      op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, data_at, [&](int64_t i, T2<double> dv,
+                                                                                    double* acc) {
         if (i >= 0) {
 #pragma unroll
           for (int c = 0; c < CT; ++c) {
-            const double rr = acc[c] - dobs[i];
-            dm[c] += rr * rr / var[i];
+            const double rr = acc[c] - dv.a;
+            dm[c] += rr * rr / dv.b;
           }
         }
       });
     }
-    for (int64_t j = tid; j < op.d; j += FU_NT)
+    for (int64_t x = tid; x < op.d; x += FU_NT)
 #pragma unroll
-      for (int c = 0; c < CT; ++c) pm[c] += prior_misfit(md, j, (double)Xs[j * CT + c]);
+      for (int c = 0; c < CT; ++c) pm[c] += prior_misfit(md, x, (double)Xs[x * CT + c]);
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
       const double sd = block_sum<FU_NT>(dm[c], scratch);
@@ -526,11 +687,11 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
   }
   if (g_out) {
     if (op.n > 0) {
-      op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, lane, [&](int64_t, int64_t i, T* acc) {
+      auto data_at = [&](int64_t i) { return i >= 0 ? T2<T>{(T)dobs[i], (T)var[i]} : T2<T>{T(0), T(1)}; };
+      op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, lane, data_at, [&](int64_t i, T2<T> dv, T* acc) {
         if (i >= 0) {
-          const T di = (T)dobs[i], vi = (T)var[i];
 #pragma unroll
-          for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - di) / vi;
+          for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - dv.a) / dv.b;
         }
       });
       __syncthreads();
@@ -541,18 +702,19 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
       for (int c = 0; c < CT; ++c) g[c] = T(0);
       if (op.n > 0) {
         if constexpr (SPARSE) {
-          sell_row<T, T, T, CT>(op.bwd_s.col, op.bwd_val, op.bwd_s.base[s], op.bwd_s.width[s], lane, Rs, g);
+          sell_row<T, T, T, CT>(op.bwd_s, op.bwd_val, op.bwd_s.base[s], op.bwd_s.width[s], lane, Rs, g);
         } else {
           const int64_t row = s * 64 + lane;
           if (row < op.d) dense_row<T, T, T, CT>(op.bwd_val + row, op.bwd_d.ld, op.n, Rs, g);
         }
       }
-      const int64_t j = op.param_of(s, lane);
-      if (j >= 0) {
+      const int64_t x = op.slot_of(s, lane);
+      if (x >= 0) {
+        const int64_t j = op.param_at(x);
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
           if (c0 + c < B) {
-            double gg = (double)(g[c] + prior_grad<T>(md, j, Xs[j * CT + c]));
+            double gg = (double)(g[c] + prior_grad<T>(md, x, Xs[x * CT + c]));
             if (obc[c] > 0.0) gg += INFINITY;
             g_out[(c0 + c) * ldx + j] = gg;
           }
@@ -568,16 +730,22 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
 static int fused_ct(const Problem& p, int64_t B, size_t& lds_bytes) {
   const size_t es = dtype_size(p.state_dtype);
   int dev = 0;
-  hipGetDevice(&dev);
+  HMC_HIP(hipGetDevice(&dev));
   int maxlds = 65536;
-  hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  HMC_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   const size_t budget = (size_t)maxlds - 2048;
-  // float32 prefers 2 chains per workgroup: measured on the 64x64 tomography
-  // problem (2000 rays, 1024 chains) 1.13 ms per proposal against 1.63 ms
-  // with 4 (half the LDS per workgroup, two workgroups per CU)
+  // float32 sparse operators prefer 4 chains per workgroup once there are
+  // enough chains for a workgroup per CU: each matrix byte streamed from L2
+  // serves 4 chains (64x64 tomography, 2000 rays, 1024 chains: 0.93 ms per
+  // proposal against 1.34 ms with 2); otherwise, and for dense operators,
+  // 2 chains (half the LDS per workgroup, two workgroups per CU)
+  int n_cu = 256;
+  HMC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int cands_sparse_f32[] = {4, 2, 1};
   const int cands_f32[] = {2, 4, 1};
   const int cands_f64[] = {2, 1};
-  const int* cands = p.state_dtype == HMC_F64 ? cands_f64 : cands_f32;
+  const int* cands = p.state_dtype == HMC_F64 ? cands_f64
+                     : (p.op == 2 && B >= 4 * (int64_t)n_cu) ? cands_sparse_f32 : cands_f32;
   const int ncand = p.state_dtype == HMC_F64 ? 2 : 3;
   for (int i = 0; i < ncand; ++i) {
     const int ct = cands[i];
@@ -614,8 +782,8 @@ static Ops<T, VT, SPARSE> make_ops(const Problem& p) {
   op.n = p.n;
   op.d = p.d;
   if constexpr (SPARSE) {
-    op.fwd_s = DevSell{p.sg.base, p.sg.width, p.sg.perm, p.sg.col, p.sg.nslices};
-    op.bwd_s = DevSell{p.sgt.base, p.sgt.width, p.sgt.perm, p.sgt.col, p.sgt.nslices};
+    op.fwd_s = DevSell{p.sg.rows, p.sg.base, p.sg.width, p.sg.perm, p.sg.col, p.sg.nslices, p.sg.total};
+    op.bwd_s = DevSell{p.sgt.rows, p.sgt.base, p.sgt.width, p.sgt.perm, p.sgt.col, p.sgt.nslices, p.sgt.total};
     op.fwd_val = (const T*)p.sg.val;
     op.bwd_val = (const T*)p.sgt.val;
     op.fwd_tgt = (const VT*)p.sg.val_tgt;
@@ -661,13 +829,59 @@ static void fused_premult_build(Chains& c) {
   }
 }
 
+// Position-ordered model arrays of a sparse problem: slot q holds parameter
+// perm[q] (padding slots: prior weights 0, inverse mass 0, mass root 0, bounds
+// -inf/+inf, which leave their never-read LDS rows at 0)
+__global__ void k_gather_model(const int32_t* __restrict__ perm, int64_t npos, DevModel src,
+                               double* __restrict__ dst) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < npos; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = perm[q];
+    for (int k = 0; k < src.n_priors; ++k) {
+      dst[(2 * k) * npos + q] = j >= 0 ? src.pr[k].a[j] : 0.0;
+      dst[(2 * k + 1) * npos + q] = j >= 0 ? src.pr[k].b[j] : 0.0;
+    }
+    const int m = 2 * kMaxPriors;
+    if (src.minv) dst[m * npos + q] = j >= 0 ? src.minv[j] : 0.0;
+    if (src.msqrt) dst[(m + 1) * npos + q] = j >= 0 ? src.msqrt[j] : 0.0;
+    if (src.lo) dst[(m + 2) * npos + q] = j >= 0 ? src.lo[j] : -INFINITY;
+    if (src.hi) dst[(m + 3) * npos + q] = j >= 0 ? src.hi[j] : INFINITY;
+  }
+}
+
+// The element-wise model in the kernels' slot order: the problem's own arrays
+// for dense operators, the position-ordered copies (gathered again after any
+// change of priors, bounds or mass) for sparse ones.
+static DevModel slot_model(Problem& p) {
+  DevModel m = p.model();
+  if (p.op != 2) return m;
+  const int64_t npos = p.sgt.nslices * 64;
+  if (p.pos_model_version != p.model_version) {
+    const int blocks = (int)std::min<int64_t>((npos + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_gather_model, dim3(blocks), dim3(256), 0, p.stream, (const int32_t*)p.sgt.perm, npos, m,
+                       p.pos_model);
+    HMC_HIP(hipGetLastError());
+    p.pos_model_version = p.model_version;
+  }
+  DevModel q = m;
+  for (int k = 0; k < m.n_priors; ++k) {
+    q.pr[k].a = p.pos_model + (2 * k) * npos;
+    q.pr[k].b = p.pos_model + (2 * k + 1) * npos;
+  }
+  const int mi = 2 * kMaxPriors;
+  q.minv = m.minv ? p.pos_model + mi * npos : nullptr;
+  q.msqrt = m.msqrt ? p.pos_model + (mi + 1) * npos : nullptr;
+  q.lo = m.lo ? p.pos_model + (mi + 2) * npos : nullptr;
+  q.hi = m.hi ? p.pos_model + (mi + 3) * npos : nullptr;
+  return q;
+}
+
 template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
 static void launch_propose(Chains& c, const FusedArgs& a, size_t lds) {
   Problem& p = *c.p;
   auto kern = k_fused_propose<T, VT, SPARSE, CT, KMAX>;
   if (lds > 65536) HMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int grid = (int)((c.B + CT - 1) / CT);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(FU_NT), lds, p.stream, make_ops<T, VT, SPARSE>(p), a, p.model());
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(FU_NT), lds, p.stream, make_ops<T, VT, SPARSE>(p), a, slot_model(p));
   HMC_HIP(hipGetLastError());
 }
 
@@ -678,7 +892,8 @@ static void launch_eval(Chains& c, const void* X, double* chi, double* g, size_t
   if (lds > 65536) HMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int grid = (int)((c.B + CT - 1) / CT);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(FU_NT), lds, p.stream, make_ops<T, VT, SPARSE>(p),
-                     (const T*)X, c.ldx, c.B, p.dobs, p.var, p.misfit_const, chi, g, p.model());
+                     (const T*)X, c.ldx, c.B, SPARSE ? p.pos_dobs : p.dobs, SPARSE ? p.pos_var : p.var,
+                     p.misfit_const, chi, g, slot_model(p));
   HMC_HIP(hipGetLastError());
 }
 
@@ -752,8 +967,8 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
   a.hist_slot = hist_slot;
   a.ldx = c.ldx;
   a.B = c.B;
-  a.dobs = p.dobs;
-  a.var = p.var;
+  a.dobs = p.op == 2 ? p.pos_dobs : p.dobs;   // the sparse forward pass writes SELL positions
+  a.var = p.op == 2 ? p.pos_var : p.var;
   a.mconst = p.misfit_const;
   a.seed = seed;
   a.prop = prop;

@@ -72,6 +72,15 @@ struct Problem {
   double* dobs = nullptr;   // [n]
   double* var = nullptr;    // [n] data variance
 
+  // fused sparse engine: data and model arrays in SELL position order (data
+  // row of forward-slice position p = sg.perm[p], parameter of backward-slice
+  // position q = sgt.perm[q]); padding positions hold neutral values
+  double* pos_dobs = nullptr;   // [sg.nslices * 64], padding 0
+  double* pos_var = nullptr;    // [sg.nslices * 64], padding 1
+  double* pos_model = nullptr;  // [kPosModelArrays][sgt.nslices * 64]
+  int model_version = 0;        // bumped by the prior, bounds and mass setters
+  int pos_model_version = -1;   // model_version pos_model was gathered at
+
   // element-wise part
   int n_priors = 0;
   int prior_kind[kMaxPriors] = {0, 0, 0, 0};

@@ -162,9 +162,11 @@ int hmc_problem_set_engine(hmc_problem* p, int engine);
  *                 -1 (default) follows the endpoint (the reference's n > d
  *                 rule); 0 keeps the residual form
  *   "fused_tile_chains" 0, 1, 2 or 4: chains one workgroup of the fused
- *                 engine carries (0 = automatic: 2 if the LDS tile fits,
- *                 else 4 (float32) or 1; a value that does not fit the LDS
- *                 tile or exceeds the chains is an error at the first call)
+ *                 engine carries (0 = automatic: float32 sparse operators
+ *                 with at least 4 chains per compute unit 4, else 2 if the
+ *                 LDS tile fits, else 4 (float32) or 1; a value that does
+ *                 not fit the LDS tile or exceeds the chains is an error at
+ *                 the first call)
  *   "gemm_split_cus" n: compute units the split-tail plan assumes (0 = the
  *                 device's count, the default; a smaller n plans for a share
  *                 of the device)
