@@ -84,37 +84,44 @@ __device__ inline void load_group(__amdgpu_buffer_rsrc_t r, int voff, int soff, 
   }
 }
 
-// acc[c] += v * in[col][c] for the CT chains of the tile, the CT values read
-// as one LDS vector where their size allows
-template <typename Acc, typename Tin, int CT>
-__device__ inline void gather_fma(const Tin* __restrict__ in, uint32_t col, Acc v, Acc acc[CT]) {
-  const Tin* q = in + col * CT;
-  if constexpr (CT == 2 && sizeof(Tin) == 4) {
-    const float2 t = *reinterpret_cast<const float2*>(q);
-    acc[0] += v * (Acc)t.x;
-    acc[1] += v * (Acc)t.y;
-  } else if constexpr (CT == 4 && sizeof(Tin) == 4) {
-    const float4 t = *reinterpret_cast<const float4*>(q);
-    acc[0] += v * (Acc)t.x;
-    acc[1] += v * (Acc)t.y;
-    acc[2] += v * (Acc)t.z;
-    acc[3] += v * (Acc)t.w;
-  } else if constexpr (CT == 2 && sizeof(Tin) == 8) {
-    const double2 t = *reinterpret_cast<const double2*>(q);
-    acc[0] += v * (Acc)t.x;
-    acc[1] += v * (Acc)t.y;
-  } else {
-#pragma unroll
-    for (int c = 0; c < CT; ++c) acc[c] += v * (Acc)q[c];
-  }
+// LDS byte address base + SCALE * (16-bit half HI of w): one v_mad_u32_u16 with
+// op_sel choosing the half (no separate extract and shift of the column)
+template <int HI, int SCALE>
+__device__ inline uint32_t lds_slot(uint32_t w, uint32_t base) {
+  uint32_t r;
+  if constexpr (HI) asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(r) : "v"(w), "I"(SCALE), "v"(base));
+  else asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(r) : "v"(w), "I"(SCALE), "v"(base));
+  return r;
 }
 
-// The loads of group g + 1 are issued before the gathers of group g, so the
-// matrix-load latency overlaps the LDS gathers and FMAs of the group before.
-// Matrix loads are buffer loads: a per-lane byte offset plus a wave-uniform
-// slice/group offset that advances on the scalar unit.  The prefetch after the
-// last group reads the next slice's first group (or zeros past the array end)
-// and is never used.
+// The CT values of one LDS row as one vector read
+template <typename Tin, int CT>
+struct LdsRow {
+  typedef Tin type __attribute__((ext_vector_type(CT)));
+};
+template <typename Tin>
+struct LdsRow<Tin, 1> {
+  typedef Tin type;
+};
+
+template <typename Tin, int CT>
+__device__ inline typename LdsRow<Tin, CT>::type lds_row(uint32_t ad) {
+  typedef typename LdsRow<Tin, CT>::type R;
+  return *(const __attribute__((address_space(3))) R*)(uintptr_t)ad;
+}
+
+template <typename Acc, typename Tin, int CT>
+__device__ inline Acc row_elem(const typename LdsRow<Tin, CT>::type& t, int c) {
+  if constexpr (CT == 1) return (Acc)t;
+  else return (Acc)t[c];
+}
+
+// One SELL row of a lane.  The loads of group k + 1 are issued before the
+// gathers of group k, so the matrix-load latency overlaps the LDS gathers and
+// FMAs of the group before.  Matrix loads are buffer loads: a per-lane byte
+// offset plus a wave-uniform slice/group offset that advances on the scalar
+// unit.  The prefetch after the last group reads the next slice's first group
+// (or zeros past the array end) and is never used.
 template <typename Acc, typename V, typename Tin, int CT>
 __device__ inline void sell_row(const DevSell& S, const V* __restrict__ valp, int64_t base, int32_t width,
                                 int lane, const Tin* __restrict__ in, Acc acc[CT]) {
@@ -122,6 +129,8 @@ __device__ inline void sell_row(const DevSell& S, const V* __restrict__ valp, in
   const __amdgpu_buffer_rsrc_t rc = sell_rsrc(S.col, S.total * 2);
   const __amdgpu_buffer_rsrc_t rv = sell_rsrc(valp, S.total * (int64_t)sizeof(V));
   const int vc = lane * SELL_G * 2, vv = lane * SELL_G * (int)sizeof(V);
+  const uint32_t inb = (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)in;
+  constexpr int RB = CT * (int)sizeof(Tin);   // bytes of one LDS row
   int sc = (int)base * 2, sv = (int)base * (int)sizeof(V);
   uint4 cw = buf_load16(rc, vc, sc);
   Acc vl[SELL_G];
@@ -132,12 +141,21 @@ __device__ inline void sell_row(const DevSell& S, const V* __restrict__ valp, in
     const uint4 cn = buf_load16(rc, vc, sc);
     Acc vn[SELL_G];
     load_group<Acc, V>(rv, vv, sv, vn);
+    // all addresses, then all LDS reads, then the FMAs (k ascending per chain)
     const uint32_t wd[4] = {cw.x, cw.y, cw.z, cw.w};
+    uint32_t ad[SELL_G];
 #pragma unroll
-    for (int u = 0; u < SELL_G; ++u) {
-      const uint32_t cl = (u & 1) ? (wd[u >> 1] >> 16) : (wd[u >> 1] & 0xffffu);
-      gather_fma<Acc, Tin, CT>(in, cl, vl[u], acc);
+    for (int h = 0; h < SELL_G / 2; ++h) {
+      ad[2 * h] = lds_slot<0, RB>(wd[h], inb);
+      ad[2 * h + 1] = lds_slot<1, RB>(wd[h], inb);
     }
+    typename LdsRow<Tin, CT>::type t[SELL_G];
+#pragma unroll
+    for (int u = 0; u < SELL_G; ++u) t[u] = lds_row<Tin, CT>(ad[u]);
+#pragma unroll
+    for (int u = 0; u < SELL_G; ++u)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[c] += vl[u] * row_elem<Acc, Tin, CT>(t[u], c);
     cw = cn;
 #pragma unroll
     for (int u = 0; u < SELL_G; ++u) vl[u] = vn[u];
