@@ -165,6 +165,9 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
   S.base = (int32_t*)upload(P.owned, base.data(), base.size() * 4, P.stream);
   S.width = (int32_t*)upload(P.owned, width.data(), width.size() * 4, P.stream);
   S.perm = (int32_t*)upload(P.owned, perm.data(), perm.size() * 4, P.stream);
+  std::vector<int32_t> inv(std::max<int64_t>(rows, 1), 0);
+  for (int64_t p = 0; p < rows; ++p) inv[order[p]] = (int32_t)p;
+  S.inv = (int32_t*)upload(P.owned, inv.data(), inv.size() * 4, P.stream);
   S.col = (uint16_t*)upload(P.owned, col.data(), col.size() * 2, P.stream);
   S.val = upload(P.owned, vtraj.data(), vtraj.size(), P.stream);
   S.val_tgt = alias ? S.val : upload(P.owned, vtgt.data(), vtgt.size(), P.stream);

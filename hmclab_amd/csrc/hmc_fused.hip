@@ -26,6 +26,7 @@ struct DevSell {
   const int32_t* base;
   const int32_t* width;
   const int32_t* perm;
+  const int32_t* inv;      // position of each original row
   const uint16_t* col;
   int64_t nslices;
   int64_t total;      // stored entries (slices x 64 x width)
@@ -257,6 +258,11 @@ struct Ops {
     if constexpr (SPARSE) return bwd_s.perm[x];
     else return x;
   }
+  // slot of parameter j (0 <= j < d)
+  __device__ int64_t slot_at(int64_t j) const {
+    if constexpr (SPARSE) return bwd_s.inv[j];
+    else return j;
+  }
   __device__ int64_t param_slices() const {
     if constexpr (SPARSE) return bwd_s.nslices;
     else return (d + 63) / 64;
@@ -393,9 +399,10 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     s_eps[tid] = e;
     s_beta[tid] = (a.beta && c < a.B) ? a.beta[c] : 1.0;
   }
-  // ---- load positions of the tile (slot x holds parameter param_at(x))
-  for (int64_t x = tid; x < op.d; x += FU_NT) {
-    const int64_t j = op.param_at(x);
+  // ---- load positions of the tile (parameter j into slot slot_at(j); the
+  // global reads run in parameter order)
+  for (int64_t j = tid; j < op.d; j += FU_NT) {
+    const int64_t x = op.slot_at(j);
 #pragma unroll
     for (int c = 0; c < CT; ++c)
       Xs[x * CT + c] = (c0 + c < a.B) ? x0[(c0 + c) * a.ldx + j] : T(0);
@@ -615,27 +622,36 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     s_ok[c] = ok;
   }
   __syncthreads();
-  // ---- write back accepted positions, trace, optional trajectory outputs
+  // ---- write back accepted positions, trace and trajectory end positions in
+  // parameter order (coalesced global writes), trajectory end momenta from
+  // the owning slots' registers
   T* xo = reinterpret_cast<T*>(a.x_out);
   T* po = reinterpret_cast<T*>(a.p_out);
   T* tr = reinterpret_cast<T*>(a.tr_x);
+  for (int64_t j = tid; j < op.d; j += FU_NT) {
+    const int64_t x = op.slot_at(j);
 #pragma unroll
-  for (int ks = 0; ks < KMAX; ++ks) {
-    const int64_t s = w + (int64_t)ks * FU_NW;
-    const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
-    if (x >= 0) {
-      const int64_t j = op.param_at(x);
+    for (int c = 0; c < CT; ++c) {
+      const int64_t ch = c0 + c;
+      if (ch < a.B) {
+        const int64_t o = ch * a.ldx + j;
+        const T xv = Xs[x * CT + c];
+        if (xo) xo[o] = xv;
+        if (s_ok[c]) x0[o] = xv;
+        if (a.slot >= 0) tr[(a.slot * a.B + ch) * op.d + j] = s_ok[c] ? xv : x0[o];
+      }
+    }
+  }
+  if (po) {
 #pragma unroll
-      for (int c = 0; c < CT; ++c) {
-        const int64_t ch = c0 + c;
-        if (ch < a.B) {
-          const int64_t o = ch * a.ldx + j;
-          const T xv = Xs[x * CT + c];
-          if (xo) xo[o] = xv;
-          if (po) po[o] = preg[ks][c];
-          if (s_ok[c]) x0[o] = xv;
-          if (a.slot >= 0) tr[(a.slot * a.B + ch) * op.d + j] = s_ok[c] ? xv : x0[o];
-        }
+    for (int ks = 0; ks < KMAX; ++ks) {
+      const int64_t s = w + (int64_t)ks * FU_NW;
+      const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
+      if (x >= 0) {
+        const int64_t j = op.param_at(x);
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+          if (c0 + c < a.B) po[(c0 + c) * a.ldx + j] = preg[ks][c];
       }
     }
   }
@@ -654,8 +670,8 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t c0 = (int64_t)blockIdx.x * CT;
   const int64_t nps = op.param_slices();
-  for (int64_t x = tid; x < op.d; x += FU_NT) {
-    const int64_t j = op.param_at(x);
+  for (int64_t j = tid; j < op.d; j += FU_NT) {
+    const int64_t x = op.slot_at(j);
 #pragma unroll
     for (int c = 0; c < CT; ++c) Xs[x * CT + c] = (c0 + c < B) ? X[(c0 + c) * ldx + j] : T(0);
   }
@@ -800,8 +816,8 @@ static Ops<T, VT, SPARSE> make_ops(const Problem& p) {
   op.n = p.n;
   op.d = p.d;
   if constexpr (SPARSE) {
-    op.fwd_s = DevSell{p.sg.rows, p.sg.base, p.sg.width, p.sg.perm, p.sg.col, p.sg.nslices, p.sg.total};
-    op.bwd_s = DevSell{p.sgt.rows, p.sgt.base, p.sgt.width, p.sgt.perm, p.sgt.col, p.sgt.nslices, p.sgt.total};
+    op.fwd_s = DevSell{p.sg.rows, p.sg.base, p.sg.width, p.sg.perm, p.sg.inv, p.sg.col, p.sg.nslices, p.sg.total};
+    op.bwd_s = DevSell{p.sgt.rows, p.sgt.base, p.sgt.width, p.sgt.perm, p.sgt.inv, p.sgt.col, p.sgt.nslices, p.sgt.total};
     op.fwd_val = (const T*)p.sg.val;
     op.bwd_val = (const T*)p.sgt.val;
     op.fwd_tgt = (const VT*)p.sg.val_tgt;
@@ -966,8 +982,8 @@ void fused_propose(Chains& c, bool host_p, int L, int integrator, uint64_t seed,
   FusedArgs a{};
   a.x0 = c.x0;
   a.chi0 = c.chi0;
-  a.x_out = c.xt;
-  a.p_out = c.pt;
+  a.x_out = host_p ? c.xt : nullptr;   // trajectory end states: only a single propose returns them
+  a.p_out = host_p ? c.pt : nullptr;
   a.p_host = host_p ? c.p_host : nullptr;
   a.eps = c.eps;
   a.u = c.u;

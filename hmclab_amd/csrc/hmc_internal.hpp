@@ -17,6 +17,7 @@ struct Sell {
   int32_t* base = nullptr;   // [nslices] element offset of the slice
   int32_t* width = nullptr;  // [nslices] entries per lane in the slice
   int32_t* perm = nullptr;   // [nslices*64] original row of (slice, lane), -1 = padding
+  int32_t* inv = nullptr;    // [rows] position (slice * 64 + lane) of each original row
   uint16_t* col = nullptr;   // [total] column indices (the fused engine's LDS tile keeps them below 65536)
   void* val = nullptr;       // [total] trajectory precision
   void* val_tgt = nullptr;   // [total] target precision (aliases val when equal)
