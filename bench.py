@@ -651,6 +651,22 @@ def run_tomography(args):
     launch_bytes = L * step_bytes + (nnz * (sv + 4) + d * B * sx) + d * B * sx
     avg_s = k_ms / 1e3 / max(k_n, 1)
     achieved = launch_bytes / avg_s
+    # HBM bytes per launch from the committed counter profile (FETCH_SIZE +
+    # WRITE_SIZE, calibrated on a copy kernel), for the configuration it was
+    # measured with; the matrix streams from L2, so the profile's L2-to-L1
+    # read bytes are reported beside it
+    traffic = l2_bytes = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_tomography.json")
+    if os.path.exists(tfile):
+        try:
+            for prof in json.load(open(tfile)).get("profiles", []):
+                o = prof.get("options", {})
+                if (o.get("precision") == args.precision and o.get("chains") == B and o.get("steps_per_traj") == L
+                        and args.fused_tile_chains in o.get("fused_tile_chains", [])):
+                    traffic = prof["bytes_per_launch"]["fused_propose"]
+                    l2_bytes = prof.get("l2_to_l1_read_bytes_per_launch")
+        except Exception:
+            traffic = l2_bytes = None
     result = {
         "metric": METRIC, "value": float(np.min(ess)) / elapsed, "unit": "ESS/s", "n_gpus": 1,
         "steps": args.steps, "warmup": done, "ms_per_step": elapsed / args.steps * 1e3,
@@ -663,9 +679,10 @@ def run_tomography(args):
                    "posterior_rms_mean_err_over_sd": float(np.sqrt(np.mean(((mean - mu_np) / sd) ** 2))),
                    "posterior_rms_std_rel_err": float(np.sqrt(np.mean((std / sd - 1.0) ** 2)))},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
-                     "frac": achieved / PEAK["hbm"], "traffic": None,
+                     "frac": achieved / PEAK["hbm"], "traffic": traffic,
                      "kernel": "k_fused_propose<sparse> (whole proposal)", "avg_launch_us": avg_s * 1e6,
-                     "algorithmic_bytes_per_launch": launch_bytes},
+                     "algorithmic_bytes_per_launch": launch_bytes,
+                     "l2_to_l1_read_bytes_per_launch": l2_bytes},
     }
     ch.close()
     prob.close()
