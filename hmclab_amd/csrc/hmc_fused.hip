@@ -18,8 +18,15 @@
 
 namespace hmc {
 
-constexpr int FU_NT = 512;
-constexpr int FU_NW = FU_NT / 64;
+// Threads per workgroup: 1024 (16 waves) for float32 sparse operators, whose
+// 4-chain tiles fill a CU's LDS with one workgroup (16 waves overlap the
+// matrix-load -> gather -> FMA chain of each row; 0.85 -> 0.79 ms per
+// proposal on the 64x64 tomography), 512 otherwise (float64 and the dense
+// operators would spill at the 128 VGPRs of 16 waves)
+template <typename T, bool SPARSE>
+constexpr int fu_nt() { return (SPARSE && sizeof(T) == 4) ? 1024 : 512; }
+// waves per workgroup of the fused engine for a problem (host side)
+inline int fu_nw(const Problem& p) { return (p.op == 2 && p.state_dtype == HMC_F32) ? 16 : 8; }
 
 struct DevSell {
   int64_t rows;       // operator rows (positions >= rows are padding)
@@ -167,10 +174,10 @@ __device__ inline void sell_row(const DevSell& S, const V* __restrict__ valp, in
 // lane (-1 for padding lanes): the sparse operator's rows in SELL position
 // order, the dense operator's rows in order.  pre(pos) runs before the row, so
 // the loads it issues overlap the row's; then f(pos, pre(pos), acc[CT]).
-template <typename Acc, typename V, typename Tin, int CT, class Pre, class F>
+template <typename Acc, typename V, typename Tin, int CT, int NW, class Pre, class F>
 __device__ inline void sell_pass(const DevSell& S, const V* __restrict__ val,
                                  const Tin* __restrict__ in, int w, int lane, Pre&& pre, F&& f) {
-  for (int64_t s = w; s < S.nslices; s += FU_NW) {
+  for (int64_t s = w; s < S.nslices; s += NW) {
     const int32_t base = S.base[s], width = S.width[s];
     const int64_t pos = (s * 64 + lane < S.rows) ? s * 64 + lane : -1;
     const auto pv = pre(pos);
@@ -206,11 +213,11 @@ __device__ inline void dense_row(const V* __restrict__ mp, int64_t ld, int64_t n
   }
 }
 
-template <typename Acc, typename V, typename Tin, int CT, class Pre, class F>
+template <typename Acc, typename V, typename Tin, int CT, int NW, class Pre, class F>
 __device__ inline void dense_pass(const DevDense& D, const V* __restrict__ M,
                                   const Tin* __restrict__ in, int w, int lane, Pre&& pre, F&& f) {
   const int64_t nsl = (D.nrows + 63) / 64;
-  for (int64_t s = w; s < nsl; s += FU_NW) {
+  for (int64_t s = w; s < nsl; s += NW) {
     const int64_t row = s * 64 + lane;
     const int64_t pos = row < D.nrows ? row : -1;
     const auto pv = pre(pos);
@@ -239,10 +246,10 @@ struct Ops {
   int64_t ldh;
   const double* hb;
 
-  template <typename Acc, typename V, typename Tin, int CT, class Pre, class F>
+  template <typename Acc, typename V, typename Tin, int CT, int NW, class Pre, class F>
   __device__ void forward(const V* val, const Tin* in, int w, int lane, Pre&& pre, F&& f) const {
-    if constexpr (SPARSE) sell_pass<Acc, V, Tin, CT>(fwd_s, val, in, w, lane, pre, f);
-    else dense_pass<Acc, V, Tin, CT>(fwd_d, val, in, w, lane, pre, f);
+    if constexpr (SPARSE) sell_pass<Acc, V, Tin, CT, NW>(fwd_s, val, in, w, lane, pre, f);
+    else dense_pass<Acc, V, Tin, CT, NW>(fwd_d, val, in, w, lane, pre, f);
   }
   // Parameter slots: the LDS row (Xs[x][CT]) and model-array index of a
   // parameter.  Sparse: the parameter's backward-slice position q (the model
@@ -367,12 +374,13 @@ struct FusedArgs {
 };
 
 template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
-__global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, FusedArgs a,
+__global__ __launch_bounds__((fu_nt<T, SPARSE>())) void k_fused_propose(Ops<T, VT, SPARSE> op, FusedArgs a,
                                                          DevModel md) {
+  constexpr int NT = fu_nt<T, SPARSE>(), NW = NT / 64;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* Xs = reinterpret_cast<T*>(smem_raw);                         // [d][CT]
   T* Rs = Xs + round_up(op.d, 2) * CT;                            // [n][CT]
-  __shared__ double scratch[FU_NW + 1];
+  __shared__ double scratch[NW + 1];
   __shared__ double s_k0[CT], s_eps[CT], s_chi1[CT], s_k1[CT], s_beta[CT];
   __shared__ int s_ok[CT];
 
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   }
   // ---- load positions of the tile (parameter j into slot slot_at(j); the
   // global reads run in parameter order)
-  for (int64_t j = tid; j < op.d; j += FU_NT) {
+  for (int64_t j = tid; j < op.d; j += NT) {
     const int64_t x = op.slot_at(j);
 #pragma unroll
     for (int c = 0; c < CT; ++c)
@@ -416,7 +424,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   for (int c = 0; c < CT; ++c) kloc[c] = 0.0;
 #pragma unroll
   for (int ks = 0; ks < KMAX; ++ks) {
-    const int64_t s = w + (int64_t)ks * FU_NW;
+    const int64_t s = w + (int64_t)ks * NW;
     const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
     const int64_t j = x >= 0 ? op.param_at(x) : -1;
 #pragma unroll
@@ -442,7 +450,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   }
 #pragma unroll
   for (int c = 0; c < CT; ++c) {
-    const double s = block_sum<FU_NT>(kloc[c], scratch);
+    const double s = block_sum<NT>(kloc[c], scratch);
     if (tid == 0) s_k0[c] = 0.5 * s;
   }
   __syncthreads();
@@ -484,7 +492,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
         T gk[KMAX][CT];
 #pragma unroll
         for (int ks = 0; ks < KMAX; ++ks) {
-          const int64_t s = w + (int64_t)ks * FU_NW;
+          const int64_t s = w + (int64_t)ks * NW;
 #pragma unroll
           for (int c = 0; c < CT; ++c) gk[ks][c] = T(0);
           const int64_t row = s * 64 + ln;
@@ -498,7 +506,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < KMAX; ++ks) {
-          const int64_t s = w + (int64_t)ks * FU_NW;
+          const int64_t s = w + (int64_t)ks * NW;
           if (s < nps) {
             const int64_t x = op.slot_of(s, ln);
             if (x >= 0) update(ks, x, gk[ks], param_cst<T>(md, x));
@@ -512,7 +520,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     auto data_at = [&](int64_t i) {
       return i >= 0 ? T2<T>{(T)a.dobs[i], (T)a.var[i]} : T2<T>{T(0), T(1)};
     };
-    op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, ln, data_at, [&](int64_t i, T2<T> dv, T* acc) {
+    op.template forward<T, T, T, CT, NW>(op.fwd_val, Xs, w, ln, data_at, [&](int64_t i, T2<T> dv, T* acc) {
       if (i >= 0) {
 #pragma unroll
         for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - dv.a) / dv.b;
@@ -522,7 +530,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     // backward over owned slices (unrolled so momenta stay in registers)
 #pragma unroll
     for (int ks = 0; ks < KMAX; ++ks) {
-      const int64_t s = w + (int64_t)ks * FU_NW;
+      const int64_t s = w + (int64_t)ks * NW;
       if (s < nps) {
         const int64_t x = op.slot_of(s, ln);
         const ParamCst<T> pc = param_cst<T>(md, x);
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
     auto data_at = [&](int64_t i) {
       return i >= 0 ? T2<double>{a.dobs[i], a.var[i]} : T2<double>{0.0, 1.0};
     };
-    op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, data_at, [&](int64_t i, T2<double> dv,
+    op.template forward<double, VT, T, CT, NW>(op.fwd_tgt, Xs, w, lane, data_at, [&](int64_t i, T2<double> dv,
                                                                                   double* acc) {
       if (i >= 0) {
 #pragma unroll
@@ -566,7 +574,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   for (int c = 0; c < CT; ++c) pm[c] = kk[c] = ob[c] = 0.0;
 #pragma unroll
   for (int ks = 0; ks < KMAX; ++ks) {
-    const int64_t s = w + (int64_t)ks * FU_NW;
+    const int64_t s = w + (int64_t)ks * NW;
     const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
     if (x >= 0) {
 #pragma unroll
@@ -580,10 +588,10 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   }
 #pragma unroll
   for (int c = 0; c < CT; ++c) {
-    const double sd = block_sum<FU_NT>(dm[c], scratch);
-    const double sp = block_sum<FU_NT>(pm[c], scratch);
-    const double sb = block_sum<FU_NT>(ob[c], scratch);
-    const double sk = block_sum<FU_NT>(kk[c], scratch);
+    const double sd = block_sum<NT>(dm[c], scratch);
+    const double sp = block_sum<NT>(pm[c], scratch);
+    const double sb = block_sum<NT>(ob[c], scratch);
+    const double sk = block_sum<NT>(kk[c], scratch);
     if (tid == 0) {
       double chi = 0.5 * sd + (sp + a.mconst);
       if (sb > 0.0) chi += INFINITY;
@@ -628,7 +636,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   T* xo = reinterpret_cast<T*>(a.x_out);
   T* po = reinterpret_cast<T*>(a.p_out);
   T* tr = reinterpret_cast<T*>(a.tr_x);
-  for (int64_t j = tid; j < op.d; j += FU_NT) {
+  for (int64_t j = tid; j < op.d; j += NT) {
     const int64_t x = op.slot_at(j);
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -645,7 +653,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
   if (po) {
 #pragma unroll
     for (int ks = 0; ks < KMAX; ++ks) {
-      const int64_t s = w + (int64_t)ks * FU_NW;
+      const int64_t s = w + (int64_t)ks * NW;
       const int64_t x = (s < nps) ? op.slot_of(s, lane) : -1;
       if (x >= 0) {
         const int64_t j = op.param_at(x);
@@ -659,18 +667,19 @@ __global__ __launch_bounds__(FU_NT) void k_fused_propose(Ops<T, VT, SPARSE> op, 
 
 // misfit (FP64) and gradient (trajectory precision) of model rows
 template <typename T, typename VT, bool SPARSE, int CT, int KMAX>
-__global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, const T* X, int64_t ldx,
+__global__ __launch_bounds__((fu_nt<T, SPARSE>())) void k_fused_eval(Ops<T, VT, SPARSE> op, const T* X, int64_t ldx,
                                                       int64_t B, const double* dobs, const double* var,
                                                       double mconst, double* chi_out, double* g_out,
                                                       DevModel md) {
+  constexpr int NT = fu_nt<T, SPARSE>(), NW = NT / 64;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* Xs = reinterpret_cast<T*>(smem_raw);
   T* Rs = Xs + round_up(op.d, 2) * CT;
-  __shared__ double scratch[FU_NW + 1];
+  __shared__ double scratch[NW + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t c0 = (int64_t)blockIdx.x * CT;
   const int64_t nps = op.param_slices();
-  for (int64_t j = tid; j < op.d; j += FU_NT) {
+  for (int64_t j = tid; j < op.d; j += NT) {
     const int64_t x = op.slot_at(j);
 #pragma unroll
     for (int c = 0; c < CT; ++c) Xs[x * CT + c] = (c0 + c < B) ? X[(c0 + c) * ldx + j] : T(0);
@@ -680,13 +689,13 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
   double obl[CT];
 #pragma unroll
   for (int c = 0; c < CT; ++c) obl[c] = 0.0;
-  for (int64_t x = tid; x < op.d; x += FU_NT)
+  for (int64_t x = tid; x < op.d; x += NT)
 #pragma unroll
     for (int c = 0; c < CT; ++c)
       if (out_of_bounds(md, x, (double)Xs[x * CT + c])) obl[c] = 1.0;
   double obc[CT];
 #pragma unroll
-  for (int c = 0; c < CT; ++c) obc[c] = block_sum<FU_NT>(obl[c], scratch);
+  for (int c = 0; c < CT; ++c) obc[c] = block_sum<NT>(obl[c], scratch);
 
   if (chi_out) {
     double dm[CT], pm[CT];
@@ -694,7 +703,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
     for (int c = 0; c < CT; ++c) dm[c] = pm[c] = 0.0;
     if (op.n > 0) {
       auto data_at = [&](int64_t i) { return i >= 0 ? T2<double>{dobs[i], var[i]} : T2<double>{0.0, 1.0}; };
-      op.template forward<double, VT, T, CT>(op.fwd_tgt, Xs, w, lane, data_at, [&](int64_t i, T2<double> dv,
+      op.template forward<double, VT, T, CT, NW>(op.fwd_tgt, Xs, w, lane, data_at, [&](int64_t i, T2<double> dv,
                                                                                     double* acc) {
         if (i >= 0) {
 #pragma unroll
@@ -705,13 +714,13 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
         }
       });
     }
-    for (int64_t x = tid; x < op.d; x += FU_NT)
+    for (int64_t x = tid; x < op.d; x += NT)
 #pragma unroll
       for (int c = 0; c < CT; ++c) pm[c] += prior_misfit(md, x, (double)Xs[x * CT + c]);
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
-      const double sd = block_sum<FU_NT>(dm[c], scratch);
-      const double sp = block_sum<FU_NT>(pm[c], scratch);
+      const double sd = block_sum<NT>(dm[c], scratch);
+      const double sp = block_sum<NT>(pm[c], scratch);
       if (tid == 0 && c0 + c < B) {
         double chi = 0.5 * sd + (sp + mconst);
         if (obc[c] > 0.0) chi += INFINITY;
@@ -722,7 +731,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
   if (g_out) {
     if (op.n > 0) {
       auto data_at = [&](int64_t i) { return i >= 0 ? T2<T>{(T)dobs[i], (T)var[i]} : T2<T>{T(0), T(1)}; };
-      op.template forward<T, T, T, CT>(op.fwd_val, Xs, w, lane, data_at, [&](int64_t i, T2<T> dv, T* acc) {
+      op.template forward<T, T, T, CT, NW>(op.fwd_val, Xs, w, lane, data_at, [&](int64_t i, T2<T> dv, T* acc) {
         if (i >= 0) {
 #pragma unroll
           for (int c = 0; c < CT; ++c) Rs[i * CT + c] = (acc[c] - dv.a) / dv.b;
@@ -730,7 +739,7 @@ __global__ __launch_bounds__(FU_NT) void k_fused_eval(Ops<T, VT, SPARSE> op, con
       });
       __syncthreads();
     }
-    for (int64_t s = w; s < nps; s += FU_NW) {
+    for (int64_t s = w; s < nps; s += NW) {
       T g[CT];
 #pragma unroll
       for (int c = 0; c < CT; ++c) g[c] = T(0);
@@ -797,8 +806,10 @@ static int fused_ct(const Problem& p, int64_t B, size_t& lds_bytes) {
 
 static int fused_kmax(const Problem& p) {
   const int64_t nslices = (p.d + 63) / 64;
-  const int64_t per_wave = (nslices + FU_NW - 1) / FU_NW;
+  const int nw = fu_nw(p);
+  const int64_t per_wave = (nslices + nw - 1) / nw;
   if (per_wave <= 2) return 2;
+  if (nw == 16 && per_wave <= 4) return 4;   // instantiated for the 16-wave (float32 sparse) kernels only
   if (per_wave <= 8) return 8;
   if (per_wave <= 16) return 16;
   throw Error{HMC_ERR_UNSUPPORTED, "fused engine supports up to 8192 parameters"};
@@ -806,7 +817,7 @@ static int fused_kmax(const Problem& p) {
 
 bool fused_supported(const Problem& p) {
   if (p.precision == HMC_BF16) return false;
-  if ((p.d + 63) / 64 > (int64_t)16 * FU_NW) return false;
+  if ((p.d + 63) / 64 > (int64_t)16 * 8) return false;   // 16 slices per wave of 8 (8192 parameters)
   return true;
 }
 
@@ -842,7 +853,7 @@ static Ops<T, VT, SPARSE> make_ops(const Problem& p) {
 // reference's rule); the endpoint keeps the FP64 residual form.
 bool fused_premult(const Problem& p) {
   return p.op == 1 && p.n > 0 && p.premult_traj != 0 && (p.premult_traj > 0 || p.n > p.d) &&
-         (p.d + 63) / 64 <= 2 * FU_NW;   // at most 2 slices per wave (KMAX = 2)
+         (p.d + 63) / 64 <= 2 * 8;   // at most 2 slices per wave of 8 (KMAX = 2)
 }
 
 __global__ void k_h_to_f32(const double* __restrict__ H, int64_t count, float* __restrict__ out) {
@@ -915,7 +926,8 @@ static void launch_propose(Chains& c, const FusedArgs& a, size_t lds) {
   auto kern = k_fused_propose<T, VT, SPARSE, CT, KMAX>;
   if (lds > 65536) HMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int grid = (int)((c.B + CT - 1) / CT);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(FU_NT), lds, p.stream, make_ops<T, VT, SPARSE>(p), a, slot_model(p));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(fu_nt<T, SPARSE>()), lds, p.stream, make_ops<T, VT, SPARSE>(p), a,
+                     slot_model(p));
   HMC_HIP(hipGetLastError());
 }
 
@@ -925,7 +937,7 @@ static void launch_eval(Chains& c, const void* X, double* chi, double* g, size_t
   auto kern = k_fused_eval<T, VT, SPARSE, CT, KMAX>;
   if (lds > 65536) HMC_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int grid = (int)((c.B + CT - 1) / CT);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(FU_NT), lds, p.stream, make_ops<T, VT, SPARSE>(p),
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(fu_nt<T, SPARSE>()), lds, p.stream, make_ops<T, VT, SPARSE>(p),
                      (const T*)X, c.ldx, c.B, SPARSE ? p.pos_dobs : p.dobs, SPARSE ? p.pos_var : p.var,
                      p.misfit_const, chi, g, slot_model(p));
   HMC_HIP(hipGetLastError());
@@ -935,6 +947,13 @@ static void launch_eval(Chains& c, const void* X, double* chi, double* g, size_t
 template <class F>
 static void dispatch(const Problem& p, int ct, int kmax, F&& f) {
   auto by_k = [&](auto t_tag, auto v_tag, auto s_tag, auto ct_tag) {
+    using T = typename decltype(t_tag)::type;
+    if constexpr (decltype(s_tag)::value && std::is_same<T, float>::value) {
+      if (kmax == 4) {
+        f(t_tag, v_tag, s_tag, ct_tag, std::integral_constant<int, 4>{});
+        return;
+      }
+    }
     if (kmax == 2) f(t_tag, v_tag, s_tag, ct_tag, std::integral_constant<int, 2>{});
     else if (kmax == 8) f(t_tag, v_tag, s_tag, ct_tag, std::integral_constant<int, 8>{});
     else f(t_tag, v_tag, s_tag, ct_tag, std::integral_constant<int, 16>{});
