@@ -140,16 +140,8 @@ __device__ inline void sell_row(const DevSell& S, const V* __restrict__ valp, in
   const uint32_t inb = (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)in;
   constexpr int RB = CT * (int)sizeof(Tin);   // bytes of one LDS row
   int sc = (int)base * 2, sv = (int)base * (int)sizeof(V);
-  uint4 cw = buf_load16(rc, vc, sc);
-  Acc vl[SELL_G];
-  load_group<Acc, V>(rv, vv, sv, vl);
-  for (int32_t k = 0; k < width; k += SELL_G) {
-    sc += 64 * SELL_G * 2;
-    sv += 64 * SELL_G * (int)sizeof(V);
-    const uint4 cn = buf_load16(rc, vc, sc);
-    Acc vn[SELL_G];
-    load_group<Acc, V>(rv, vv, sv, vn);
-    // all addresses, then all LDS reads, then the FMAs (k ascending per chain)
+  // one group: all addresses, then all LDS reads, then the FMAs (k ascending per chain)
+  auto group = [&](const uint4& cw, const Acc* vl) {
     const uint32_t wd[4] = {cw.x, cw.y, cw.z, cw.w};
     uint32_t ad[SELL_G];
 #pragma unroll
@@ -164,9 +156,26 @@ __device__ inline void sell_row(const DevSell& S, const V* __restrict__ valp, in
     for (int u = 0; u < SELL_G; ++u)
 #pragma unroll
       for (int c = 0; c < CT; ++c) acc[c] += vl[u] * row_elem<Acc, Tin, CT>(t[u], c);
-    cw = cn;
-#pragma unroll
-    for (int u = 0; u < SELL_G; ++u) vl[u] = vn[u];
+  };
+  // two register sets in turn (no copies between groups)
+  uint4 ca = buf_load16(rc, vc, sc), cb;
+  Acc va[SELL_G], vb[SELL_G];
+  load_group<Acc, V>(rv, vv, sv, va);
+  for (int32_t k = 0;;) {
+    sc += 64 * SELL_G * 2;
+    sv += 64 * SELL_G * (int)sizeof(V);
+    cb = buf_load16(rc, vc, sc);
+    load_group<Acc, V>(rv, vv, sv, vb);
+    group(ca, va);
+    k += SELL_G;
+    if (k >= width) break;
+    sc += 64 * SELL_G * 2;
+    sv += 64 * SELL_G * (int)sizeof(V);
+    ca = buf_load16(rc, vc, sc);
+    load_group<Acc, V>(rv, vv, sv, va);
+    group(cb, vb);
+    k += SELL_G;
+    if (k >= width) break;
   }
 }
 
