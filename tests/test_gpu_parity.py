@@ -90,6 +90,52 @@ def test_sparse_misfit_gradient(precision):
         assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
 
 
+def test_sparse_model_changes_after_launches():
+    """The fused sparse engine keeps prior, mass and bounds arrays in its own
+    slot order, gathered again after any change: a Laplace prior, bounds and
+    a Diagonal mass added after earlier evaluations and proposals must show up
+    in the next launch (closed-form prior terms, bounds as infinite misfit,
+    and the oracle's Diagonal-mass proposal)."""
+    from hmclab_amd import _lib
+
+    post, orc, rng = _dense_case(11, 300, 200, prior="normal", sparse=True)
+    d = 200
+    prob = build_problem(post, precision="f64")
+    X = rng.normal(size=(4, d)) * 0.5
+    chi0, g0 = prob.misfit_batch(X), prob.gradient_batch(X)
+    ch = Chains(prob, 4)
+    ch.set_state(X)
+    p0 = rng.normal(size=(4, d))
+    ch.propose(p0, np.full(4, 0.01), np.full(4, 0.5), 3)
+    # a second prior term: Laplace with dispersions s (device b = 1/s), no constant
+    mu2, s2 = rng.normal(size=d) * 0.1, rng.uniform(0.3, 1.0, size=d)
+    prob.add_prior(_lib.PRIOR_LAPLACE, mu2, 1.0 / s2, 0.0)
+    chi1, g1 = prob.misfit_batch(X), prob.gradient_batch(X)
+    for b in range(4):
+        assert _rel(chi1[b] - chi0[b], np.sum(np.abs(X[b] - mu2) / s2)) < 1e-10
+        assert _rel(g1[b] - g0[b], np.sign(X[b] - mu2) / s2) < 1e-10
+    # bounds: unchanged inside, infinite outside
+    lo, hi = -3.0 * np.ones(d), 3.0 * np.ones(d)
+    prob.set_bounds(lo, hi)
+    assert _rel(prob.misfit_batch(X), chi1) < 1e-14
+    Xo = X.copy()
+    Xo[2, 17] = 3.5
+    assert np.isinf(prob.misfit_batch(Xo)[2]) and np.isfinite(prob.misfit_batch(Xo)[1])
+    # a Diagonal mass after the chains have proposed: the next proposal uses it
+    m = rng.uniform(0.5, 2.0, size=d)
+    prob.set_mass(m)
+    ch.set_state(X)
+    eps, u = 0.004, 0.5
+    res = ch.propose(p0, np.full(4, eps), np.full(4, u), 6, trajectory=True)
+    orc_full = O.BayesRule([orc.separate_distributions[0], O.Laplace(mu2.reshape(-1, 1), s2.reshape(-1, 1)),
+                            O.Uniform(lo, hi), orc.separate_distributions[-1]])
+    for b in range(4):
+        ref = O.propose(orc_full, O.Diagonal(m), X[b].reshape(-1, 1), None, p0[b].reshape(-1, 1), eps, 6, u)
+        assert _rel(res["X1"][b], ref["x1"][:, 0]) < 1e-10
+        assert abs(res["H0"][b] - ref["h0"]) <= 1e-10 * abs(ref["h0"])
+        assert abs(res["H1"][b] - ref["h1"]) <= 1e-10 * abs(ref["h1"])
+
+
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("n,d,B", [(520, 384, 3), (300, 700, 130)])
 def test_bf16_gemm_gradient_and_fp64_misfit(cfg, n, d, B):
