@@ -90,6 +90,33 @@ def test_sparse_misfit_gradient(precision):
         assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_sparse_many_parameter_slices(precision):
+    """d = 5000: 79 parameter slices, so each wave of the fused sparse kernel
+    owns up to 5 (float32, 16 waves: the 8-slice build) or 10 (float64, 8
+    waves: the 16-slice build); misfit, gradient and one 4-step proposal with
+    host momenta against the oracle."""
+    dtype = np.float64 if precision == "f64" else np.float32
+    d = 5000
+    post, orc, rng = _dense_case(12, 300, d, prior="normal", sparse=True, density=0.02, g_dtype=dtype)
+    prob = build_problem(post, precision=precision)
+    X = (rng.normal(size=(3, d)) * 0.3).astype(dtype).astype(np.float64)
+    chi, g = prob.misfit_batch(X), prob.gradient_batch(X)
+    for b in range(3):
+        x = X[b].reshape(-1, 1)
+        assert _rel(chi[b], orc.misfit(x)) < (1e-10 if precision == "f64" else 1e-6)
+        assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
+    ch = Chains(prob, 3)
+    ch.set_state(X)
+    p0 = rng.normal(size=(3, d)).astype(dtype).astype(np.float64)
+    eps, u = 0.002, 0.5
+    res = ch.propose(p0, np.full(3, eps), np.full(3, u), 4, trajectory=True)
+    for b in range(3):
+        ref = O.propose(orc, O.Unit(d), X[b].reshape(-1, 1), None, p0[b].reshape(-1, 1), eps, 4, u)
+        assert _rel(res["X1"][b], ref["x1"][:, 0]) < TOL[precision]
+        assert abs(res["H0"][b] - ref["h0"]) <= (1e-10 if precision == "f64" else 1e-6) * abs(ref["h0"])
+
+
 def test_sparse_model_changes_after_launches():
     """The fused sparse engine keeps prior, mass and bounds arrays in its own
     slot order, gathered again after any change: a Laplace prior, bounds and
