@@ -1,6 +1,8 @@
 // C-ABI entry points of libhmc.so.  Every function catches engine errors and
 // returns a status code; hmc_last_error() returns the thread's last message.
 #include <algorithm>
+#include <array>
+#include <functional>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -105,8 +107,80 @@ static std::vector<int32_t> sell_order(const std::vector<int64_t>& ptr, int64_t 
   return order;
 }
 
+// Entry order of one slice.  At every step k each 16-lane group of the fused
+// engine gathers 16 LDS rows; when their positions differ modulo 16, 16-byte
+// rows (4 float32 or 2 float64 chains) fall on distinct LDS bank groups.  Per
+// step and group, Kuhn's augmenting-path matching assigns residues to lanes
+// from their remaining entries (residues with more entries left tried first);
+// a lane left unmatched takes a padding entry on a free residue if it has one,
+// else its next entry on its fullest residue.  seq[r][k] = entry of lane r at
+// step k (index into its row), or -(1 + residue) for a padding entry.
+static void sell_slice_order(const std::vector<std::vector<int32_t>>& cols, int W,
+                             std::vector<std::vector<int32_t>>& seq) {
+  constexpr int NL = 64, NG = 16;
+  seq.assign(NL, std::vector<int32_t>(W, 0));
+  std::vector<std::array<std::vector<int32_t>, NG>> rem(NL);
+  std::vector<std::array<int, NG>> head(NL);
+  std::vector<int> pads(NL);
+  for (int r = 0; r < NL; ++r) {
+    head[r].fill(0);
+    for (int e = 0; e < (int)cols[r].size(); ++e) rem[r][cols[r][e] & (NG - 1)].push_back(e);
+    pads[r] = W - (int)cols[r].size();
+  }
+  auto left = [&](int r, int q) { return (int)rem[r][q].size() - head[r][q]; };
+  for (int k = 0; k < W; ++k) {
+    for (int g = 0; g < NL / NG; ++g) {
+      int by_res[NG], by_lane[NG];
+      std::array<std::array<int, NG>, NG> pref;   // residues of each lane, most entries left first
+      for (int i = 0; i < NG; ++i) {
+        by_res[i] = by_lane[i] = -1;
+        std::iota(pref[i].begin(), pref[i].end(), 0);
+        const int r = g * NG + i;
+        std::stable_sort(pref[i].begin(), pref[i].end(), [&](int a, int b) { return left(r, a) > left(r, b); });
+      }
+      std::array<bool, NG> seen;
+      std::function<bool(int)> augment = [&](int i) -> bool {
+        const int r = g * NG + i;
+        for (int q : pref[i]) {
+          if (left(r, q) <= 0 || seen[q]) continue;
+          seen[q] = true;
+          if (by_res[q] < 0 || augment(by_res[q])) {
+            by_res[q] = i;
+            by_lane[i] = q;
+            return true;
+          }
+        }
+        return false;
+      };
+      for (int i = 0; i < NG; ++i) {
+        seen.fill(false);
+        augment(i);
+      }
+      for (int i = 0; i < NG; ++i) {
+        const int r = g * NG + i;
+        int q = by_lane[i];
+        if (q >= 0 && by_res[q] == i) {
+          seq[r][k] = rem[r][q][head[r][q]++];
+        } else if (pads[r] > 0) {
+          int fr = 0;
+          while (fr < NG - 1 && by_res[fr] >= 0) ++fr;
+          if (by_res[fr] < 0) by_res[fr] = i;
+          --pads[r];
+          seq[r][k] = -(1 + fr);
+        } else {
+          q = pref[i][0];
+          for (int t = 0; t < NG; ++t)
+            if (left(r, t) > left(r, q)) q = t;
+          seq[r][k] = rem[r][q][head[r][q]++];
+        }
+      }
+    }
+  }
+}
+
 // Column indices are stored as colmap[column] (the fused engine keeps both
-// operands in LDS in SELL position order).
+// operands in LDS in SELL position order); each slice's entries are ordered
+// by sell_slice_order.
 static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
                        const std::vector<int64_t>& ptr, const std::vector<int32_t>& idx,
                        const std::vector<double>& val, const std::vector<int32_t>& order,
@@ -151,17 +225,31 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
   std::vector<unsigned char> vtgt;
   const bool alias = (traj_dt == tgt_dt);
   if (!alias) vtgt.assign(std::max<int64_t>(total, 1) * dtype_size(tgt_dt), 0);
-  for (int64_t s = 0; s < S.nslices; ++s)
+  std::vector<std::vector<int32_t>> lane_cols(64), seq;
+  for (int64_t s = 0; s < S.nslices; ++s) {
     for (int r = 0; r < 64; ++r) {
       const int32_t row = perm[s * 64 + r];
-      if (row < 0) continue;
-      for (int64_t k = 0; k < ptr[row + 1] - ptr[row]; ++k) {
+      lane_cols[r].clear();
+      if (row >= 0)
+        for (int64_t k = ptr[row]; k < ptr[row + 1]; ++k) lane_cols[r].push_back(colmap[idx[k]]);
+    }
+    sell_slice_order(lane_cols, width[s], seq);
+    for (int r = 0; r < 64; ++r) {
+      const int32_t row = perm[s * 64 + r];
+      for (int k = 0; k < width[s]; ++k) {
         const int64_t o = base[s] + (k / kSellGroup) * 64 * kSellGroup + r * kSellGroup + k % kSellGroup;
-        col[o] = (uint16_t)colmap[idx[ptr[row] + k]];
-        store_elem(vtraj.data(), traj_dt, o, val[ptr[row] + k]);
-        if (!alias) store_elem(vtgt.data(), tgt_dt, o, val[ptr[row] + k]);
+        const int32_t e = seq[r][k];
+        if (e >= 0) {
+          col[o] = (uint16_t)lane_cols[r][e];
+          store_elem(vtraj.data(), traj_dt, o, val[ptr[row] + e]);
+          if (!alias) store_elem(vtgt.data(), tgt_dt, o, val[ptr[row] + e]);
+        } else {
+          const int32_t q = -(e + 1);               // padding: value 0 on a free residue
+          col[o] = (uint16_t)(q < cols ? q : 0);
+        }
       }
     }
+  }
   S.base = (int32_t*)upload(P.owned, base.data(), base.size() * 4, P.stream);
   S.width = (int32_t*)upload(P.owned, width.data(), width.size() * 4, P.stream);
   S.perm = (int32_t*)upload(P.owned, perm.data(), perm.size() * 4, P.stream);
