@@ -116,9 +116,14 @@ static std::vector<int32_t> sell_order(const std::vector<int64_t>& ptr, int64_t 
 // else its next entry on its fullest residue.  seq[r][k] = entry of lane r at
 // step k (index into its row), or -(1 + residue) for a padding entry.
 static void sell_slice_order(const std::vector<std::vector<int32_t>>& cols, int W,
-                             std::vector<std::vector<int32_t>>& seq) {
+                             std::vector<std::vector<int32_t>>& seq, bool reorder) {
   constexpr int NL = 64, NG = 16;
   seq.assign(NL, std::vector<int32_t>(W, 0));
+  if (!reorder) {   // CSR order, padding (column 0) after each row's entries
+    for (int r = 0; r < NL; ++r)
+      for (int k = 0; k < W; ++k) seq[r][k] = k < (int)cols[r].size() ? k : -1;
+    return;
+  }
   std::vector<std::array<std::vector<int32_t>, NG>> rem(NL);
   std::vector<std::array<int, NG>> head(NL);
   std::vector<int> pads(NL);
@@ -225,6 +230,9 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
   std::vector<unsigned char> vtgt;
   const bool alias = (traj_dt == tgt_dt);
   if (!alias) vtgt.assign(std::max<int64_t>(total, 1) * dtype_size(tgt_dt), 0);
+  // the ordering costs about 8e3 operations per 64 stored entries: beyond 2^24
+  // entries (seconds of setup) the slices keep CSR order
+  const bool reorder = total <= ((int64_t)1 << 24);
   std::vector<std::vector<int32_t>> lane_cols(64), seq;
   for (int64_t s = 0; s < S.nslices; ++s) {
     for (int r = 0; r < 64; ++r) {
@@ -233,7 +241,7 @@ static void build_sell(Problem& P, Sell& S, int64_t rows, int64_t cols,
       if (row >= 0)
         for (int64_t k = ptr[row]; k < ptr[row + 1]; ++k) lane_cols[r].push_back(colmap[idx[k]]);
     }
-    sell_slice_order(lane_cols, width[s], seq);
+    sell_slice_order(lane_cols, width[s], seq, reorder);
     for (int r = 0; r < 64; ++r) {
       const int32_t row = perm[s * 64 + r];
       for (int k = 0; k < width[s]; ++k) {
