@@ -117,6 +117,47 @@ def test_sparse_many_parameter_slices(precision):
         assert abs(res["H0"][b] - ref["h0"]) <= (1e-10 if precision == "f64" else 1e-6) * abs(ref["h0"])
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_sparse_degenerate_rows_and_residues(precision):
+    """Edge cases of the fused sparse layout: empty data rows, parameters no
+    ray touches, and rows whose columns all share one residue modulo 16 (the
+    entry ordering then cannot avoid repeated LDS bank groups and falls back
+    to its conflict path); misfit, gradient and one proposal against the
+    oracle."""
+    dtype = np.float64 if precision == "f64" else np.float32
+    rng = np.random.default_rng(21)
+    n, d = 200, 300
+    G = np.zeros((n, d))
+    for i in range(n):
+        if i % 7 == 3:
+            continue                                   # empty data row
+        if i % 3 == 0:
+            cols = rng.choice(np.arange(0, 288, 16), size=12, replace=False)   # one residue mod 16
+        else:
+            cols = rng.choice(d - 40, size=rng.integers(1, 60), replace=False)  # parameters >= 260 untouched
+        G[i, cols] = rng.normal(size=cols.size)
+    G = G.astype(dtype)
+    dd = G.astype(np.float64) @ rng.normal(size=(d, 1)) + rng.normal(0, 0.1, size=(n, 1))
+    Gs = sp.csr_matrix(G)
+    post = D.BayesRule([D.Normal(np.zeros((d, 1)), 1.0), D.LinearMatrix(Gs, dd, 0.01, dtype=dtype)])
+    orc = O.BayesRule([O.Normal(np.zeros((d, 1)), 1.0),
+                       O.LinearMatrixSimple(Gs, dd, 0.01, dtype=dtype, premultiplication=False)])
+    prob = build_problem(post, precision=precision)
+    X = (rng.normal(size=(5, d)) * 0.5).astype(dtype).astype(np.float64)
+    chi, g = prob.misfit_batch(X), prob.gradient_batch(X)
+    for b in range(5):
+        x = X[b].reshape(-1, 1)
+        assert _rel(chi[b], orc.misfit(x)) < (1e-10 if precision == "f64" else 1e-6)
+        assert _rel(g[b], orc.gradient(x)[:, 0]) < TOL[precision]
+    ch = Chains(prob, 5)
+    ch.set_state(X)
+    p0 = rng.normal(size=(5, d)).astype(dtype).astype(np.float64)
+    res = ch.propose(p0, np.full(5, 0.003), np.full(5, 0.5), 5, trajectory=True)
+    for b in range(5):
+        ref = O.propose(orc, O.Unit(d), X[b].reshape(-1, 1), None, p0[b].reshape(-1, 1), 0.003, 5, 0.5)
+        assert _rel(res["X1"][b], ref["x1"][:, 0]) < TOL[precision]
+
+
 def test_sparse_model_changes_after_launches():
     """The fused sparse engine keeps prior, mass and bounds arrays in its own
     slot order, gathered again after any change: a Laplace prior, bounds and
